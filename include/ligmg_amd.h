@@ -1,0 +1,197 @@
+/*
+ * ligmg_amd.h - C ABI of the MI355X graph-Laplacian AMG solver.
+ *
+ * This is the drop-in boundary for LigMG's setup + solve path.  The
+ * reference exposes it as a C++ template class and a PETSc PC:
+ *
+ *   ligmg::Solver<T,F,DER>(A, opts)      include/ligmg/ligmg.hpp:17-28
+ *   Solver::solve(b) / solve(b, from)    include/ligmg/ligmg.hpp:31-38
+ *   ConvergenceInfo<T,F>                 include/ligmg/util.hpp:143-170
+ *   Options / default_options()          include/ligmg/options.hpp:80-136
+ *   PCCreate_LIGMG / PCSetUp / PCApply   include/ligmg/petsc_shim.hpp:18-74
+ *
+ * Here a solver is an opaque handle built from one CSR matrix (int32 row
+ * pointers and column indices, double values, sorted columns, diagonal
+ * stored) that may live in host or device memory.  All entry points return
+ * an int status (0 = LIGMG_OK) and never throw; ligmg_last_error() gives a
+ * thread-local message for the last failure.  A handle owns all device
+ * memory of its hierarchy; calls on one handle must be serialised.
+ */
+#ifndef LIGMG_AMD_H
+#define LIGMG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+enum {
+  LIGMG_OK = 0,
+  LIGMG_ERR_INVALID = 1,     /* bad argument / not a Laplacian-shaped CSR   */
+  LIGMG_ERR_UNSUPPORTED = 2, /* option value not implemented                */
+  LIGMG_ERR_DEVICE = 3,      /* HIP runtime failure                          */
+  LIGMG_ERR_NOMEM = 4,       /* allocation failed                            */
+  LIGMG_ERR_NUMERIC = 5      /* NaN/Inf in input or a singular coarse level  */
+};
+
+/* Krylov methods (options.hpp:205-206 KSPType values used by the reference) */
+enum { LIGMG_KSP_CG = 0, LIGMG_KSP_FCG = 1 };
+/* options.hpp:10-12 */
+enum { LIGMG_SMOOTHER_JACOBI = 0, LIGMG_SMOOTHER_CHEBYSHEV = 1 };
+enum {
+  LIGMG_AFF_NONE = 0, LIGMG_AFF_ROW = 1, LIGMG_AFF_COLUMN = 2,
+  LIGMG_AFF_ROWCOLUMN = 3, LIGMG_AFF_SYM = 4, LIGMG_AFF_MAX = 5
+};
+/* options.hpp:61 */
+enum { LIGMG_STRENGTH_AFFINITY = 0, LIGMG_STRENGTH_ALGEBRAIC_DISTANCE = 1,
+       LIGMG_STRENGTH_NONE = 2 };
+/* arithmetic of the preconditioner (the outer Krylov loop is always FP64) */
+enum { LIGMG_PREC_FP64 = 0, LIGMG_PREC_MIXED = 1 };
+/* where the caller's arrays live */
+enum { LIGMG_MEM_HOST = 0, LIGMG_MEM_DEVICE = 1 };
+/* level kinds (levels.hpp:8-28 name(): "elim", "agg", "coarse") */
+enum { LIGMG_LEVEL_ELIM = 0, LIGMG_LEVEL_AGG = 1, LIGMG_LEVEL_COARSE = 2 };
+/* solve status (the reference ignores the KSP converged reason, cg.hpp:98) */
+enum {
+  LIGMG_CONVERGED = 0,
+  LIGMG_DIVERGED_ITS = 1,
+  LIGMG_DIVERGED_BREAKDOWN = 2,
+  LIGMG_DIVERGED_NAN = 3
+};
+
+/* Mirrors ligmg::Options (options.hpp:80-134) field names and defaults for
+ * every field that affects setup or solve.  Fields marked (new) have no
+ * counterpart in the reference. */
+typedef struct ligmg_options {
+  int32_t verbose;              /* false                                     */
+  int32_t always_elim;          /* true: try an elimination level each time  */
+  int32_t top_elim;             /* true: apply level-0 elimination once      */
+  int32_t constant_correction;  /* false                                     */
+  int32_t strength_metric;      /* LIGMG_STRENGTH_AFFINITY                   */
+  int32_t max_levels;           /* 50                                        */
+  int64_t min_size;             /* 4000 (nnz bound of the last agg level)    */
+  int32_t elim_degree;          /* 4                                         */
+  double tol;                   /* 1e-8                                      */
+  double thresh;                /* 1e-8 strength prune threshold             */
+  int32_t agg_iters;            /* 10                                        */
+  int32_t iters;                /* 500                                       */
+  int32_t pre_smooth;           /* 2                                         */
+  int32_t post_smooth;          /* 2                                         */
+  int32_t num_tvs;              /* 10 (at most 16 here)                      */
+  int32_t tvs_smooth;           /* 4                                         */
+  int32_t smoother_type;        /* LIGMG_SMOOTHER_CHEBYSHEV                  */
+  int32_t kcycle;               /* false                                     */
+  int32_t outer_ksp_type;       /* LIGMG_KSP_CG                              */
+  int32_t kcycle_ksp_type;      /* LIGMG_KSP_FCG                             */
+  int32_t kcycle_iters;         /* 2                                         */
+  int64_t votes_to_seed;        /* 1                                         */
+  int64_t max_agg_size;         /* -1 (only -1 supported)                    */
+  double agg_iter_factor;       /* 0.5                                       */
+  double agg_init_factor;       /* 0.5                                       */
+  double cheby_lower;           /* 0.3                                       */
+  double cheby_upper;           /* 1.1                                       */
+  int32_t aff_scaling;          /* LIGMG_AFF_MAX                             */
+  double elim_factor;           /* 0.9                                       */
+  int32_t elim_iters;           /* 1 (only 1 supported)                      */
+  int32_t neighbor_of_neighbor; /* false (only false supported)              */
+  /* (new) */
+  uint64_t seed;                /* 0: seed of test vectors / eigen estimate  */
+  int32_t cheby_eig_iters;      /* 10: CG-Lanczos steps for lambda_max       */
+  int32_t precision;            /* LIGMG_PREC_FP64                           */
+  int32_t use_graphs;           /* 1: replay cycles from captured hipGraphs  */
+  int32_t reorder;              /* 1: solve-phase locality permutation       */
+} ligmg_options;
+
+/* ConvergenceInfo (util.hpp:143-170) plus the fields the reference only
+ * prints (full-system residual, multigrid.hpp:357-366) and status. */
+typedef struct ligmg_info {
+  int32_t iters;
+  int32_t status;              /* LIGMG_CONVERGED / ...                        */
+  int32_t history_len;         /* entries written to the history buffer        */
+  int32_t base_level;          /* 1 when the level-0 elimination was applied   */
+  double rel_residual;         /* last / first of the reduced-system history   */
+  double full_rel_residual;    /* ||b - L x|| / ||b|| on the caller's system   */
+  double time;                 /* solve wall time (s)                          */
+  double work;                 /* iters * (1 + cycle complexity)               */
+  double cycle_complexity;     /* work / iters                                 */
+  double cf, ecf, wda;
+  double setup_time;           /* s, from the handle                           */
+} ligmg_info;
+
+typedef struct ligmg_solver *ligmg_handle;
+
+/* default_options() (options.hpp:136) */
+void ligmg_options_default(ligmg_options *opts);
+
+/* Solver(A, opts): builds the hierarchy on the device (ligmg.hpp:17-28 ->
+ * make_hierarchy, multigrid.hpp:161-263).  opts may be NULL (defaults).
+ * ncomp/comp_ptr (may be 0/NULL) name contiguous vertex blocks that are
+ * independent graphs: Krylov inner products and the constant-nullspace
+ * projection are then taken per block (batched solves). */
+int ligmg_create(int64_t n, const int32_t *rowptr, const int32_t *colind,
+                 const double *vals, int mem_kind, const ligmg_options *opts,
+                 int32_t ncomp, const int64_t *comp_ptr, ligmg_handle *out);
+
+/* Solver::solve(b) (ligmg.hpp:31-33 -> ligmg::solve, multigrid.hpp:315-370).
+ * b and x have length n in mem_kind memory.  history (host, may be NULL)
+ * receives up to history_cap reduced-system residual norms (incl. r0). */
+int ligmg_solve(ligmg_handle h, const double *b, double *x, int mem_kind,
+                ligmg_info *info, double *history, int32_t history_cap);
+
+/* k right-hand sides at once (column-major, leading dimension n), one
+ * hierarchy traversal per cycle with SpMM kernels; every (block, column)
+ * pair runs its own flexible-CG coefficients and convergence test.
+ * info receives the worst column; iters_out (host, may be NULL) the
+ * per-column iteration counts. */
+int ligmg_solve_batch(ligmg_handle h, int32_t k, const double *B, double *X,
+                      int mem_kind, ligmg_info *info, int32_t *iters_out);
+
+/* Adjoint edge gradient for x = L(w)^+ b: for each stored off-diagonal
+ * (i,j) with i<j (upper triangle, row order), g = -(lam_i - lam_j)(x_i - x_j).
+ * x, lam and g are device pointers; g has one entry per upper edge. */
+int ligmg_edge_grad(ligmg_handle h, const double *x, const double *lam,
+                    double *g);
+
+/* Solver destructor / PCDestroy_LIGMG (petsc_shim.hpp:31-38). */
+int ligmg_destroy(ligmg_handle h);
+
+/* Thread-local message of the last failed call ("" if none). */
+const char *ligmg_last_error(void);
+
+/* ---- hierarchy introspection (print_hierarchy, multigrid.hpp:265-292;
+ *      dump_hierarchy, ligmg.hpp:47-52) ---------------------------------- */
+int ligmg_num_levels(ligmg_handle h, int32_t *out);
+int ligmg_setup_time(ligmg_handle h, double *out);
+int ligmg_level_info(ligmg_handle h, int32_t level, int32_t *kind, int64_t *n,
+                     int64_t *nnz, int64_t *n_coarse);
+/* matrix of a level copied to host arrays sized from ligmg_level_info */
+int ligmg_level_matrix(ligmg_handle h, int32_t level, int32_t *rowptr,
+                       int32_t *colind, double *vals);
+/* elimination level: new id or -1 per vertex; aggregation level: coarse id
+ * per vertex.  Host array of length n. */
+int ligmg_level_map(ligmg_handle h, int32_t level, int32_t *map);
+/* aggregation level: final AggState per vertex (ind, state), host arrays. */
+int ligmg_level_states(ligmg_handle h, int32_t level, int32_t *ind,
+                       int32_t *state);
+/* aggregation level: strength value per stored entry of the level matrix
+ * (0 where pruned or diagonal), host array of length nnz. */
+int ligmg_level_strength(ligmg_handle h, int32_t level, double *s);
+/* Chebyshev interval [lo, hi] of a level's smoother */
+int ligmg_level_cheby(ligmg_handle h, int32_t level, double *lo, double *hi);
+
+/* ---- stage entry points for parity tests (device execution) ---------- */
+/* Voting aggregation (aggregation.hpp:838-964) driven by caller-supplied
+ * strength values on the pattern of a CSR (host arrays).  Writes the
+ * coarse id per vertex and the final AggState; returns n_coarse. */
+int ligmg_aggregate_with_strength(int64_t n, const int32_t *rowptr,
+                                  const int32_t *colind, const double *s,
+                                  const ligmg_options *opts, int32_t *agg,
+                                  int32_t *ind, int32_t *state,
+                                  int64_t *n_coarse);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

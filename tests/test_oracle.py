@@ -1,0 +1,446 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) against what the mathematics and
+independent library routines fix.  The reference has no tests or golden
+vectors (its only fixture is examples/my_matrix.mtx, used below); it cannot
+be built here, so these checks are what pin the restatement."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle_ffi as O
+from ligmg_amd import _ffi, graphs
+
+
+def lap_from_edges(n, edges, w=None):
+    u = np.array([e[0] for e in edges], np.int32)
+    v = np.array([e[1] for e in edges], np.int32)
+    w = np.ones(len(edges)) if w is None else np.asarray(w, float)
+    return graphs.from_edges(n, u, v, w)
+
+
+def dense(g):
+    return g.to_scipy().toarray()
+
+
+def random_graph(n, p, seed, weights=True):
+    rng = np.random.default_rng(seed)
+    edges, w = [], []
+    for i in range(n):
+        for j in range(i + 1, n):
+            if rng.random() < p:
+                edges.append((i, j))
+                w.append(rng.uniform(0.5, 2.0) if weights else 1.0)
+    # chain to keep it connected
+    for i in range(n - 1):
+        if (i, i + 1) not in edges:
+            edges.append((i, i + 1))
+            w.append(1.0)
+    return lap_from_edges(n, edges, w)
+
+
+def grounded_solve(g, b):
+    """FP64 sparse-direct oracle: splu on L with vertex 0 grounded, mean
+    removed."""
+    A = g.to_scipy().tocsc()
+    x = np.zeros(g.n)
+    x[1:] = spla.splu(A[1:, 1:]).solve(b[1:] - 0.0)
+    return x - x.mean()
+
+
+# ---------------------------------------------------------------- canon_sum
+
+def test_canon_sum_exact_on_integers():
+    rng = np.random.default_rng(0)
+    for m in [0, 1, 5, 64, 65, 200, 1000]:
+        a = rng.integers(-1000, 1000, m).astype(float)
+        assert O.canon_sum(a) == float(a.sum())
+
+
+def test_canon_sum_order_rules():
+    # <= 64 terms: left to right (1e16 + 1 - 1e16 = 0 in that order)
+    assert O.canon_sum(np.array([1e16, 1.0, -1e16])) == 0.0
+    # > 64 terms: strided partials; the big terms land in partials 0 and 1,
+    # the unit in partial 65 % 64 = 1: (1e16 ... ) error stays within fsum bound
+    a = np.zeros(130)
+    a[0], a[64], a[1] = 1e16, -1e16, 1.0
+    s = O.canon_sum(a)
+    assert s == 1.0  # partial 0 cancels exactly, partial 1 holds the 1
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(10000)
+    assert abs(O.canon_sum(b) - math.fsum(b)) < 1e-12 * np.abs(b).sum()
+
+
+# ------------------------------------------------------- elimination (a1-a6)
+
+def test_degrees_count_nonzeros_minus_one():
+    g = random_graph(30, 0.15, 3)
+    rp, ci, v = O.csr_args(g)
+    deg = np.empty(g.n, np.int32)
+    O.lib().oracle_degrees(g.n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(v, O.f64),
+                           O.ptr(deg, O.i32))
+    A = g.to_scipy()
+    assert np.array_equal(deg, np.asarray((A != 0).sum(axis=1)).ravel() - 1)
+
+
+def elim_select(g, maxdeg=4):
+    rp, ci, v = O.csr_args(g)
+    f = np.empty(g.n, np.int32)
+    nk = O.i64()
+    O.lib().oracle_elim_select(g.n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(v, O.f64), maxdeg,
+                               O.ptr(f, O.i32), O.C.byref(nk))
+    return f.astype(bool), nk.value
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_elimination_selection_brute_force(seed):
+    """v is eliminated iff deg(v) <= 4 and every candidate in its closed
+    neighbourhood has a larger index (elimination.hpp:186-209), checked by
+    enumerating neighbourhoods of a small random graph with a dense matrix."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(4, 14))
+    g = random_graph(n, float(rng.uniform(0.1, 0.5)), seed)
+    D = dense(g)
+    deg = (D != 0).sum(1) - 1
+    cand = deg <= 4
+    expect = np.zeros(n, bool)
+    for i in range(n):
+        nb = [j for j in range(n) if D[i, j] != 0 and cand[j]]
+        expect[i] = bool(nb) and min(nb) == i
+    got, nk = elim_select(g)
+    assert np.array_equal(got, expect)
+    assert nk == n - expect.sum()
+    # eliminated set is independent
+    E = np.where(got)[0]
+    assert not np.any(D[np.ix_(E, E)] - np.diag(np.diag(D[np.ix_(E, E)])))
+
+
+def test_grid_eliminates_only_vertex_zero():
+    """On a row-major 2-D grid every vertex is a candidate but only vertex 0
+    is the smallest index of its closed neighbourhood, so the level is
+    rejected (kept > 0.9 n) and level 0 is an aggregation level."""
+    g = graphs.grid2d(16, 16)
+    f, nk = elim_select(g)
+    assert np.array_equal(np.where(f)[0], [0])
+    s = O.OracleSolver(g)
+    assert s.level_info(0)[0] == _ffi.LEVEL_AGG
+
+
+def test_schur_complement_and_interpolation():
+    """Elimination level: the coarse matrix is L_cc - L_ce L_ee^-1 L_ec
+    (pattern and values), P 1 = 1, and x = P x_c + Q b solves L x = b when
+    x_c solves the Schur system (elimination.hpp:33-52, 287-340)."""
+    g = graphs.rmat(8, 4, wkind=_ffi.W_UNIFORM, p0=0.5, p1=1.5, pendant_frac=0.3, seed=5)
+    s = O.OracleSolver(g)
+    assert s.level_info(0)[0] == _ffi.LEVEL_ELIM
+    newid = s.level_map(0)
+    kept = np.where(newid >= 0)[0]
+    el = np.where(newid < 0)[0]
+    assert np.array_equal(newid[kept], np.arange(len(kept)))
+    D = dense(g)
+    Lcc, Lce, Lee, Lec = D[np.ix_(kept, kept)], D[np.ix_(kept, el)], D[np.ix_(el, el)], D[np.ix_(el, kept)]
+    assert np.count_nonzero(Lee - np.diag(np.diag(Lee))) == 0
+    S = Lcc - Lce @ np.diag(1 / np.diag(Lee)) @ Lec
+    rp, ci, v = s.level_matrix(1)
+    C = sp.csr_matrix((v, ci, rp), shape=(len(kept), len(kept))).toarray()
+    assert np.allclose(C, S, rtol=1e-13, atol=1e-13 * np.abs(S).max())
+    # pattern: kept-kept entries plus every pair of neighbours of an eliminated vertex
+    pat = (Lcc != 0) | ((np.abs(Lce) @ np.abs(Lec)) != 0)
+    Cp = sp.csr_matrix((np.ones_like(v), ci, rp), shape=C.shape).toarray() != 0
+    assert np.array_equal(Cp, pat)
+    # exact interpolation through the level: solve on the Schur system
+    b = graphs.rhs(g.n, 1, 7)
+    bc = b[kept] - Lce @ (b[el] / np.diag(Lee))
+    xc = np.zeros(len(kept))
+    xc[1:] = np.linalg.solve(S[1:, 1:], bc[1:])
+    x = np.zeros(g.n)
+    x[kept] = xc
+    x[el] = (b[el] - Lec @ xc) / np.diag(Lee)
+    assert np.allclose(D @ x, b, atol=1e-9)
+
+
+def test_aggregation_galerkin_is_RAP():
+    g = graphs.grid2d(20, 17, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=3)
+    s = O.OracleSolver(g)
+    assert s.level_info(0)[0] == _ffi.LEVEL_AGG
+    agg = s.level_map(0)
+    nc = s.level_info(0)[3]
+    assert agg.min() == 0 and agg.max() == nc - 1 and len(np.unique(agg)) == nc
+    R = sp.csr_matrix((np.ones(g.n), (agg, np.arange(g.n))), shape=(nc, g.n))
+    Cref = (R @ g.to_scipy() @ R.T).tocsr()
+    Cref.sort_indices()
+    rp, ci, v = s.level_matrix(1)
+    assert np.array_equal(rp, Cref.indptr) and np.array_equal(ci, Cref.indices)
+    assert np.allclose(v, Cref.data, rtol=1e-13, atol=1e-13)
+
+
+# ----------------------------------------------------------- strength (a7-a10)
+
+def test_tv_init_exact_mean():
+    X = np.empty(1000 * 10)
+    O.lib().oracle_tv_init(1000, 42, 3, 10, O.ptr(X, O.f64))
+    X = X.reshape(1000, 10)
+    assert np.all(np.abs(X.sum(0)) < 1e-12)
+    # uniform on [0,1) shifted by its mean: range and variance 1/12
+    assert np.all(np.abs(X) < 0.6)
+    assert np.allclose(X.std(0), 1 / math.sqrt(12), rtol=0.1)
+    # columns are independent draws
+    assert np.all(np.abs(np.corrcoef(X.T) - np.eye(10)) < 0.15)
+
+
+def test_strength_matches_numpy_formula():
+    g = random_graph(40, 0.12, 9)
+    n, T = g.n, 10
+    X = np.empty(n * T)
+    O.lib().oracle_tv_init(n, 1, 0, T, O.ptr(X, O.f64))
+    X0 = X.reshape(n, T).copy()
+    rp, ci, v = O.csr_args(g)
+    S = np.empty(g.nnz)
+    cs = np.empty(n)
+    assert O.lib().oracle_strength(n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(v, O.f64),
+                                   O.ptr(X, O.f64), T, 4, _ffi.STRENGTH_AFFINITY, _ffi.AFF_MAX,
+                                   1e-8, O.ptr(S, O.f64), O.ptr(cs, O.f64)) == 0
+    # numpy: 4 weighted-Jacobi sweeps with b = 0 on the negative entries
+    A = dense(g)
+    d = np.diag(A)
+    N = np.where(A < 0, A, 0.0)
+    Y = X0.copy()
+    for _ in range(4):
+        Y = (2 / 3) * (-(N @ Y)) / d[:, None] + (1 / 3) * Y
+    assert np.allclose(X.reshape(n, T), Y, rtol=1e-12, atol=1e-15)
+    nrm = (Y * Y).sum(1)
+    rows = np.repeat(np.arange(n), np.diff(g.rowptr))
+    cols = g.colind
+    top = (Y[rows] * Y[cols]).sum(1)
+    raw = np.where(rows != cols, top ** 2 / (nrm[cols] ** 2 * nrm[rows] ** 2), 0.0)
+    Raw = sp.csr_matrix((raw, cols, g.rowptr), shape=(n, n)).toarray()
+    rmax, cmax = Raw.max(1), Raw.max(0)
+    sc = raw / np.maximum(rmax[rows], cmax[cols])
+    sc[sc < 1e-8] = 0.0
+    assert np.allclose(S, sc, rtol=1e-10, atol=0)
+    Sm = sp.csr_matrix((S, cols, g.rowptr), shape=(n, n)).toarray()
+    assert np.allclose(cs, Sm.sum(0), rtol=1e-12)
+    assert S.max() <= 1.0 and np.all(S[rows == cols] == 0)
+
+
+# ------------------------------------------------------------ voting (a11-a14)
+
+def aggregate(g, S, **opts):
+    rp, ci, _ = O.csr_args(g)
+    S = np.ascontiguousarray(S, np.float64)
+    o = O.default_options(**opts)
+    agg = np.empty(g.n, np.int32)
+    ind = np.empty(g.n, np.int32)
+    st = np.empty(g.n, np.int32)
+    nc = O.i64()
+    rc = O.lib().oracle_aggregate_with_strength(g.n, O.ptr(rp, O.i32), O.ptr(ci, O.i32),
+                                                O.ptr(S, O.f64), O.C.byref(o), O.ptr(agg, O.i32),
+                                                O.ptr(ind, O.i32), O.ptr(st, O.i32),
+                                                O.C.byref(nc))
+    assert rc == 0
+    return agg, ind, st, nc.value
+
+
+def unit_strength(g):
+    rows = np.repeat(np.arange(g.n), np.diff(g.rowptr))
+    return np.where(rows != g.colind, 1.0, 0.0)
+
+
+def test_vote_star_hand_derived():
+    """Star 0-{1..5}, unit strengths.  Iter 0: leaves vote for 0, the centre
+    votes for the last leaf (ties go to the later neighbour, aggregation.hpp:
+    457); 0 and 5 vote for each other, so 0 becomes Seed and 5 Decided(0).
+    Iter 1: the other leaves see the Seed and join it; 0 becomes
+    DecidedRoot.  One aggregate."""
+    g = lap_from_edges(6, [(0, i) for i in range(1, 6)])
+    agg, ind, st, nc = aggregate(g, unit_strength(g))
+    assert nc == 1 and np.all(agg == 0) and np.all(ind == 0)
+    assert st[0] == _ffi.ST_DECIDEDROOT and np.all(st[1:] == _ffi.ST_DECIDED)
+
+
+def test_vote_path_hand_derived():
+    """Path 0-1-2-3, unit strengths.  Iter 0 votes: 0->1, 1->2 (tie, later),
+    2->3 (tie, later), 3->2: 2,3 are mutual so 2 = Seed, 3 = Decided(2).
+    Iter 1: 1 sees Seed 2 and joins; 0 votes for Undecided 1 (votes[1] += 1).
+    Vertex 0 never collects col_sum = 1 votes and stays a singleton."""
+    g = lap_from_edges(4, [(0, 1), (1, 2), (2, 3)])
+    agg, ind, st, nc = aggregate(g, unit_strength(g))
+    assert nc == 2
+    assert list(agg) == [0, 1, 1, 1] and list(ind) == [0, 2, 2, 2]
+    assert list(st) == [_ffi.ST_UNDECIDED, _ffi.ST_DECIDED, _ffi.ST_DECIDEDROOT, _ffi.ST_DECIDED]
+
+
+def test_vote_pruned_vertex_becomes_singleton_root():
+    """A vertex with all strengths pruned keeps the default vote, has
+    col_sum = 0 and votes = 0, so 0 >= 0 makes it a Seed at iteration 0 and
+    a DecidedRoot afterwards (aggregation.hpp:422-426, 742-748)."""
+    g = lap_from_edges(4, [(0, 1), (1, 2), (2, 3)])
+    S = unit_strength(g)
+    rows = np.repeat(np.arange(g.n), np.diff(g.rowptr))
+    S[(rows == 0) | (g.colind == 0)] = 0.0  # isolate vertex 0 in the strength graph
+    agg, ind, st, nc = aggregate(g, S)
+    assert st[0] == _ffi.ST_DECIDEDROOT and ind[0] == 0
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_aggregates_are_stars(seed):
+    """Every vertex maps to a root that is itself or a strength neighbour;
+    roots map to themselves; coarse ids ascend with the root index."""
+    g = graphs.rmat(9, 8, wkind=_ffi.W_LOGNORMAL, p0=0, p1=1.5, seed=seed)
+    s = O.OracleSolver(g, O.default_options(always_elim=0))
+    agg = s.level_map(0)
+    ind, st = s.level_states(0)
+    A = g.to_scipy()
+    for i in range(g.n):
+        r = ind[i]
+        assert ind[r] == r
+        assert r == i or A[i, r] != 0
+    roots = np.unique(ind)
+    assert np.array_equal(agg[roots], np.arange(len(roots)))
+
+
+# ----------------------------------------------------- smoother (a17) and eig
+
+def gen_eig(g):
+    A = dense(g)
+    return sla.eigh(A, np.diag(np.diag(A)))
+
+
+def test_chebyshev_polynomial_on_eigenvectors():
+    """With b = 0, m Chebyshev steps on [lo, hi] map a generalized
+    eigenvector (L v = lam D v) to T_m((theta-lam)/delta)/T_m(theta/delta) v
+    (Saad, Alg. 12.1)."""
+    g = random_graph(25, 0.2, 4)
+    lam, V = gen_eig(g)
+    lo, hi = 0.3 * lam[-1], 1.1 * lam[-1]
+    th, de = (hi + lo) / 2, (hi - lo) / 2
+    rp, ci, v = O.csr_args(g)
+    for m in [1, 2, 3, 5]:
+        for k in [1, 5, 12, 24]:
+            x = V[:, k].copy()
+            b = np.zeros(g.n)
+            O.lib().oracle_cheby_apply(g.n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(v, O.f64), lo,
+                                       hi, m, O.ptr(x, O.f64), O.ptr(b, O.f64))
+            Tm = np.polynomial.chebyshev.Chebyshev.basis(m)
+            factor = Tm((th - lam[k]) / de) / Tm(th / de)
+            assert np.allclose(x, factor * V[:, k], atol=1e-12)
+
+
+def test_eig_estimate_bounds():
+    for seed in range(4):
+        g = random_graph(60, 0.08, seed)
+        lam = gen_eig(g)[0][-1]
+        rp, ci, v = O.csr_args(g)
+        est = O.f64()
+        O.lib().oracle_eig_estimate(g.n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(v, O.f64), 0, 0,
+                                    10, O.C.byref(est))
+        assert 0.8 * lam <= est.value <= lam * (1 + 1e-10)
+    g = random_graph(8, 0.4, 1)  # n steps of Lanczos are exact
+    lam = gen_eig(g)[0][-1]
+    rp, ci, v = O.csr_args(g)
+    O.lib().oracle_eig_estimate(g.n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(v, O.f64), 3, 0,
+                                20, O.C.byref(est))
+    assert abs(est.value - lam) < 1e-8 * lam
+
+
+# ------------------------------------------------------------- coarse (a20)
+
+@pytest.mark.parametrize("connected", [True, False])
+def test_pinv_matches_numpy(connected):
+    g = random_graph(30, 0.1, 2)
+    if not connected:
+        g2 = random_graph(12, 0.3, 3)
+        g = graphs.union([g, g2, lap_from_edges(1, [])]) if False else graphs.union([g, g2])
+    rp, ci, v = O.csr_args(g)
+    out = np.empty((g.n, g.n))
+    assert O.lib().oracle_pinv(g.n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(v, O.f64),
+                               O.ptr(out, O.f64)) == 0
+    assert np.allclose(out, np.linalg.pinv(dense(g)), atol=1e-10)
+
+
+# ------------------------------------------------------ solve (a21-a25)
+
+def check_solution(g, x, b, info, tol=1e-6):
+    xr = grounded_solve(g, b)
+    xm = x - x.mean()
+    assert np.linalg.norm(xm - xr) <= tol * np.linalg.norm(xr)
+    r = b - g.to_scipy() @ x
+    assert np.linalg.norm(r) <= 2e-8 * np.linalg.norm(b)
+    assert info.status == _ffi.CONVERGED
+
+
+@pytest.mark.parametrize("case", ["my_matrix", "grid", "rmat_pend", "ba", "rgg"])
+@pytest.mark.parametrize("variant", ["vcycle_cg", "kcycle_fcg"])
+def test_solve_against_sparse_direct(case, variant, my_matrix):
+    g = {
+        "my_matrix": lambda: my_matrix,
+        "grid": lambda: graphs.grid2d(48, 40),
+        "rmat_pend": lambda: graphs.rmat_config(11, "lognormal", pendants=True),
+        "ba": lambda: graphs.barabasi_albert(3000, 5, seed=2),
+        "rgg": lambda: graphs.rgg(3000, math.sqrt(12 / (math.pi * 3000)), seed=1),
+    }[case]()
+    opts = O.default_options()
+    if variant == "kcycle_fcg":
+        opts.kcycle = 1
+        opts.outer_ksp_type = _ffi.KSP_FCG
+    s = O.OracleSolver(g, opts)
+    b = graphs.rhs(g.n, 1, 0)
+    x, info, hist = s.solve(b)
+    check_solution(g, x, b, info)
+    assert len(hist) == info.iters + 1
+    assert info.rel_residual == pytest.approx(hist[-1] / hist[0])
+    assert info.rel_residual <= 1e-8
+
+
+def test_vcycle_is_symmetric_positive():
+    """Same polynomial for pre- and post-smoothing and R = P^T make the
+    V-cycle a symmetric operator, which CG requires."""
+    g = graphs.rmat_config(9, "uniform", pendants=True)
+    s = O.OracleSolver(g)
+    base = 1 if s.level_info(0)[0] == _ffi.LEVEL_ELIM else 0
+    n = s.level_info(base)[1]
+    rng = np.random.default_rng(0)
+    u, w = rng.standard_normal(n), rng.standard_normal(n)
+    u -= u.mean()
+    w -= w.mean()
+    Mu, Mw = s.apply_cycle(base, u), s.apply_cycle(base, w)
+    assert abs(Mu @ w - u @ Mw) <= 1e-10 * abs(Mu @ w)
+    assert Mu @ u > 0
+
+
+def test_cycle_complexity_formula():
+    g = graphs.grid2d(64, 64)
+    s = O.OracleSolver(g)
+    o = s.opts
+    work = 0.0
+    for l in range(s.num_levels()):
+        k, n, nnz, nc = s.level_info(l)
+        if k == _ffi.LEVEL_AGG:
+            work += (o.pre_smooth + o.post_smooth + 1) * nnz + 3 * n
+        elif k == _ffi.LEVEL_COARSE:
+            work += nnz
+        else:  # nnz(P) + nnz(R) + nnz(Q) (elimination.hpp:58)
+            rp, _, _ = s.level_matrix(l)
+            el = s.level_map(l) < 0
+            nnzP = nc + (np.diff(rp)[el] - 1).sum()
+            work += 2 * nnzP + el.sum()
+    b = graphs.rhs(g.n, 1, 1)
+    x, info, _ = s.solve(b)
+    assert info.cycle_complexity == pytest.approx(1 + work / g.nnz)
+
+
+def test_blocks_solve_independently():
+    """Block-diagonal union with per-block inner products: each block's
+    solution equals that block solved alone."""
+    gs = [graphs.barabasi_albert(400, 3, seed=s) for s in range(3)]
+    u = graphs.union(gs)
+    b = graphs.rhs(u.n, 1, 5, comp_ptr=u.comp_ptr)
+    s = O.OracleSolver(u, comp_ptr=u.comp_ptr)
+    x, info, _ = s.solve(b)
+    for c, g in enumerate(gs):
+        lo, hi = u.comp_ptr[c], u.comp_ptr[c + 1]
+        xr = grounded_solve(g, b[lo:hi])
+        xm = x[lo:hi] - x[lo:hi].mean()
+        assert np.linalg.norm(xm - xr) <= 1e-6 * np.linalg.norm(xr)
