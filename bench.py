@@ -1,0 +1,184 @@
+"""Benchmark: time to 1e-8 relative residual of the FP32-preconditioned solve
+against the FP64-only solve on one MI355X, plus the Laplacian SpMV's
+achieved HBM bandwidth.
+
+Workload (default): R-MAT scale 23, edge factor 16, (a,b,c,d) =
+(.57,.19,.19,.05), symmetrised, self loops and duplicates removed, largest
+connected component, pendant paths of length uniform{1,2,3} at 10% of the
+vertices, edge weights lognormal(0, 1.5); one mean-zero N(0,1) right-hand
+side (seed 0).  A step is one full solve to 1e-8 relative residual from a
+device-resident right-hand side.
+
+Prints one JSON line (rank 0).  Usage:
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 23]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time to 1e-8 rel. residual: FP32-precond vs FP64 on GPU; SpMV HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(scale, nnz_full):
+    """The CPU oracle (single thread) on a bounded sample of the same
+    construction, scaled to the full workload by stored entries."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi
+    from ligmg_amd import graphs
+    g = graphs.rmat_config(scale, "lognormal", pendants=True)
+    b = graphs.rhs(g.n, 1, 0)
+    s = oracle_ffi.OracleSolver(g)
+    ts, iters = [], 0
+    for _ in range(2):
+        t = time.perf_counter()
+        _, info, _ = s.solve(b)
+        ts.append(time.perf_counter() - t)
+        iters = info.iters
+    t = min(ts)
+    return {
+        "value": t * nnz_full / g.nnz,
+        "unit": "s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"CPU oracle FP64 V-cycle/CG solve to 1e-8 on the same construction at R-MAT "
+                   f"scale {scale} (n={g.n}, nnz={g.nnz}): {t:.3f} s, {iters} iterations; scaled "
+                   f"by nnz ratio {nnz_full / g.nnz:.1f} to the full workload"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=23)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-scale", type=int, default=18)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only (no sharding or collectives); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import ligmg_amd
+    from ligmg_amd import _ffi, graphs
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    t = time.perf_counter()
+    g = graphs.rmat_config(args.scale, "lognormal", pendants=True)
+    t_gen = time.perf_counter() - t
+    print(f"# graph n={g.n} nnz={g.nnz} generated in {t_gen:.1f}s", file=sys.stderr, flush=True)
+    b = torch.from_numpy(graphs.rhs(g.n, 1, 0)).to(dev)
+    A = (torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev),
+         torch.from_numpy(g.vals).to(dev))
+    torch.cuda.synchronize()
+
+    # FP64-only solve (the plain method, the drop-in): V-cycle + CG
+    verbose = int(os.environ.get("LIGMG_VERBOSE", "0"))
+    s64 = ligmg_amd.Solver(A, ligmg_amd.Options(verbose=verbose))
+    print(f"# fp64 setup {s64.setup_time:.2f}s levels {s64.num_levels()}", file=sys.stderr,
+          flush=True)
+    info64 = None
+    t64 = []
+    for _ in range(args.fp64_steps):
+        info64 = s64.solve(b)
+        t64.append(info64.time())
+    print(f"# fp64 solve {min(t64):.4f}s iters {info64.iters()} rel {info64.rel_residual():.2e} "
+          f"full {info64.full_rel_residual:.2e}", file=sys.stderr, flush=True)
+    base = info64.base_level
+    # SpMV roofline on the base-level FP64 operator (the outer-loop matrix)
+    lib = _ffi.solver_lib()
+    ms, nbytes = C.c_double(), C.c_int64()
+    rc = lib.ligmg_bench_spmv(s64._h, base, _ffi.PREC_FP64, 20, C.byref(ms), C.byref(nbytes))
+    assert rc == 0
+    spmv64 = {"ms": ms.value, "bytes": nbytes.value, "gbs": nbytes.value / ms.value / 1e6}
+    rc = lib.ligmg_bench_spmv(s64._h, base, _ffi.PREC_MIXED, 20, C.byref(ms), C.byref(nbytes))
+    assert rc == 0
+    spmv32 = {"ms": ms.value, "bytes": nbytes.value, "gbs": nbytes.value / ms.value / 1e6}
+    lvl = s64.level_info(base)
+    setup64 = s64.setup_time
+    x64 = info64.solution
+    s64.close()
+    del s64
+    torch.cuda.empty_cache()
+
+    # FP32 hierarchy + cycle inside the FP64 flexible-CG outer loop
+    s32 = ligmg_amd.Solver(A, ligmg_amd.Options(precision=_ffi.PREC_MIXED,
+                                                 outer_ksp_type=_ffi.KSP_FCG, verbose=verbose))
+    print(f"# mixed setup {s32.setup_time:.2f}s", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        s32.solve(b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = []
+    for _ in range(args.steps):
+        infos.append(s32.solve(b))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t32 = [i.time() for i in infos]
+    info32 = infos[-1]
+    err = float(torch.linalg.norm((info32.solution - info32.solution.mean()) -
+                                  (x64 - x64.mean())) / torch.linalg.norm(x64 - x64.mean()))
+    print(f"# mixed solve {min(t32):.4f}s iters {info32.iters()} rel {info32.rel_residual():.2e} "
+          f"full {info32.full_rel_residual:.2e} diff_vs_fp64 {err:.2e}", file=sys.stderr,
+          flush=True)
+    assert info32.rel_residual() <= 1e-8 and info64.rel_residual() <= 1e-8
+
+    ms_step = 1e3 * wall / args.steps
+    out = {
+        "metric": METRIC,
+        "value": ms_step / 1e3,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 outer / f32 preconditioner",
+        "data": "synthetic",
+        "config": {
+            "workload": f"rmat{args.scale}_lcc_pendants_lognormal1.5_fp32precond_fcg",
+            "n": g.n, "nnz": g.nnz, "rhs": 1, "rtol": 1e-8,
+            "base_level_n": lvl[1], "base_level_nnz": lvl[2],
+        },
+        "fp64_solve_s": min(t64),
+        "fp32_precond_solve_s": min(t32),
+        "speedup_fp32_vs_fp64": min(t64) / min(t32),
+        "iters_fp32": info32.iters(), "iters_fp64": info64.iters(),
+        "rel_residual_fp32": info32.rel_residual(), "full_rel_residual_fp32": info32.full_rel_residual,
+        "rel_residual_fp64": info64.rel_residual(), "full_rel_residual_fp64": info64.full_rel_residual,
+        "solution_diff_fp32_vs_fp64": err,
+        "setup_s_fp64": setup64, "setup_s_mixed": s32.setup_time,
+        "graph_gen_s": t_gen,
+        "spmv_fp64": spmv64, "spmv_fp32": spmv32,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": spmv64["gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": spmv64["gbs"] / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_spmv_units (FP64 base-level Laplacian SpMV)",
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_scale, g.nnz)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

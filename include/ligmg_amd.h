@@ -181,6 +181,15 @@ int ligmg_level_strength(ligmg_handle h, int32_t level, double *s);
 /* Chebyshev interval [lo, hi] of a level's smoother */
 int ligmg_level_cheby(ligmg_handle h, int32_t level, double *lo, double *hi);
 
+/* Times `iters` back-to-back launches of level `level`'s SpMV y = A x
+ * (precision LIGMG_PREC_FP64: FP64 values and vectors; LIGMG_PREC_MIXED:
+ * FP32 copies) on the solver's stream with HIP events.  Returns the average
+ * milliseconds per launch and the algorithmic bytes of one launch,
+ * nnz*(s_v + 4) + (n+1)*4 + 2*n*s_x.  Stands in for the reference's SpMV
+ * micro-benchmark (src/bench_spmv.cpp:1-50). */
+int ligmg_bench_spmv(ligmg_handle h, int32_t level, int32_t precision,
+                     int32_t iters, double *avg_ms, int64_t *bytes);
+
 /* ---- stage entry points for parity tests (device execution) ---------- */
 /* Voting aggregation (aggregation.hpp:838-964) driven by caller-supplied
  * strength values on the pattern of a CSR (host arrays).  Writes the

@@ -87,6 +87,7 @@ SOLVER_PROTOS = {
     "ligmg_level_states": (C.c_int, [HANDLE, i32, P(i32), P(i32)]),
     "ligmg_level_strength": (C.c_int, [HANDLE, i32, P(f64)]),
     "ligmg_level_cheby": (C.c_int, [HANDLE, i32, P(f64), P(f64)]),
+    "ligmg_bench_spmv": (C.c_int, [HANDLE, i32, i32, i32, P(f64), P(i64)]),
     "ligmg_aggregate_with_strength": (C.c_int, [i64, P(i32), P(i32), P(f64),
                                                 P(ligmg_options), P(i32), P(i32),
                                                 P(i32), P(i64)]),
@@ -134,6 +135,12 @@ def solver_lib():
     unloadable library is an error."""
     global _solver
     if _solver is None:
+        try:
+            # torch bundles its own libamdhip64.so.7; loading it first makes
+            # both libraries share one HIP runtime in this process.
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_SOLVER):
             raise RuntimeError(f"{LIB_SOLVER} is not built; run __graft_entry__.build()")
         _solver = bind(C.CDLL(LIB_SOLVER), SOLVER_PROTOS)

@@ -1,0 +1,136 @@
+// Shared HIP helpers: error handling, device buffers, wavefront utilities.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace lg {
+
+// Error carrying a C-ABI status code (include/ligmg_amd.h).
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define LG_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::lg::Error(3, std::string(#call) + ": " + hipGetErrorString(e_) + " at " + \
+                               __FILE__ + ":" + std::to_string(__LINE__));             \
+  } while (0)
+
+#define LG_CHECK(cond, code, msg)             \
+  do {                                                \
+    if (!(cond)) throw ::lg::Error((code), (msg)); \
+  } while (0)
+
+#define LG_LAUNCH_CHECK() LG_HIP(hipGetLastError())
+
+constexpr int WAVE = 64;
+
+// Owning device allocation.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) {
+      hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+      if (e != hipSuccess) {
+        p = nullptr;
+        n = 0;
+        throw Error(4, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+      }
+    }
+  }
+  void ensure(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T *get() const { return p; }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+template <class T>
+inline void to_device(T *dst, const T *src, size_t count, hipStream_t s) {
+  if (count) LG_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <class T>
+inline void to_host(T *dst, const T *src, size_t count, hipStream_t s) {
+  if (count) LG_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+template <class T>
+inline void d2d(T *dst, const T *src, size_t count, hipStream_t s) {
+  if (count) LG_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+template <class T>
+inline std::vector<T> download(const T *src, size_t count, hipStream_t s) {
+  std::vector<T> v(count);
+  to_host(v.data(), src, count, s);
+  LG_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+inline unsigned grid_for(int64_t work, int block, int64_t cap = 1 << 20) {
+  int64_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+inline int bits_for(uint64_t v) {  // number of bits to hold values < v (v >= 1)
+  int b = 0;
+  while (b < 64 && ((uint64_t)1 << b) < v) b++;
+  return b;
+}
+
+#if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double shfl_down_d(double v, int off) { return __shfl_down(v, off, WAVE); }
+__device__ __forceinline__ float shfl_down_f(float v, int off) { return __shfl_down(v, off, WAVE); }
+
+// The canonical tree used after 64 strided partial sums: p[l] += p[l+off]
+// for off = 32..1.  Lane 0 holds the result.
+__device__ __forceinline__ double wave_tree_sum(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_down(v, off, WAVE);
+  return v;
+}
+__device__ __forceinline__ float wave_tree_sum(float v) {
+  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_down(v, off, WAVE);
+  return v;
+}
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+#endif
+
+}  // namespace lg

@@ -1,0 +1,64 @@
+// Device-resident multigrid hierarchy.
+//
+// The reference keeps a vector of polymorphic levels holding CombBLAS
+// matrices (levels.hpp:8-28, multigrid.hpp:33-35) with P, R, Q as explicit
+// sparse matrices.  Here every level is a flat record of device arrays:
+// CSR with int32 indices, the inverse diagonal, and implicit transfer
+// operators (aggregate ids / member lists for aggregation, kept ids plus a
+// short P row per eliminated vertex for elimination).
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "../../include/ligmg_amd.h"
+#include "common.hpp"
+
+namespace lg {
+
+// Work units of the row-block SpMV: a unit is either a run of complete rows
+// holding at most UNIT_NNZ stored entries, or one UNIT_NNZ-chunk of a longer
+// row whose partial sum is combined afterwards in chunk order.
+constexpr int UNIT_NNZ = 2048;
+constexpr int SPMV_BLOCK = 256;
+
+struct Units {
+  int64_t nunits = 0, nlong = 0, nparts = 0;
+  DevBuf<int32_t> row0;   // first row
+  DevBuf<int32_t> nrows;  // > 0: complete rows; <= 0: -(partial slot) of a chunk
+  DevBuf<int32_t> koff;   // first stored position
+  DevBuf<int32_t> lrow;   // long rows
+  DevBuf<int32_t> lpart;  // nlong + 1 slot offsets
+  DevBuf<double> parts;   // partial sums (also used as float storage)
+};
+
+struct DCsr {
+  int64_t n = 0, ncols = 0, nnz = 0;
+  DevBuf<int32_t> rp, ci;
+  DevBuf<double> v;   // empty for pattern-only operators (unit values)
+  DevBuf<float> v32;  // FP32 copy (mixed precision)
+  Units U;
+};
+
+struct DLevel {
+  int kind = LIGMG_LEVEL_AGG;
+  DCsr A;
+  int64_t n = 0, nc = 0;
+  // elimination: P (n x nc) and R = P^T (nc x n) with values, Q diagonal
+  DCsr P, R;
+  DevBuf<int32_t> new_id;
+  DevBuf<double> q;
+  DevBuf<float> q32;
+  int64_t nnzP = 0, nnzQ = 0;
+  // aggregation: coarse id per vertex, members CSR (nc x n, unit values)
+  DevBuf<int32_t> agg, ind, state;
+  DevBuf<double> strength;
+  DCsr M;
+  DevBuf<double> dinv;
+  DevBuf<float> dinv32;
+  double lam = 0, cheb_lo = 0, cheb_hi = 0;
+  // coarse: dense pseudo-inverse, row-major n x n
+  DevBuf<double> pinv;
+  std::vector<int64_t> blocks;  // contiguous vertex blocks (independent graphs)
+};
+
+}  // namespace lg

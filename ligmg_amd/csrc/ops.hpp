@@ -1,0 +1,62 @@
+// Solve-phase device operators (row-block SpMV family, vector ops,
+// deterministic reductions).  T is the vector element type (double or
+// float); matrix values are read in the same precision (A.v or A.v32), or
+// as 1 for pattern-only operators.
+#pragma once
+#include "hierarchy.hpp"
+
+namespace lg {
+
+// Build the SpMV work units of A from its row pointers (host greedy pass).
+void build_units(DCsr &A, hipStream_t s);
+// FP32 copy of the values.
+void make_v32(DCsr &A, hipStream_t s);
+
+// y = A x
+template <class T> void op_spmv(const DCsr &A, const T *x, T *y, hipStream_t s);
+// r = b - A x
+template <class T> void op_residual(const DCsr &A, const T *x, const T *b, T *r, hipStream_t s);
+// One Chebyshev step: xn = xc + alpha (xc - xp) + beta dinv (b - A xc).
+// With xc == nullptr the current iterate is zero: xn = beta dinv b.
+template <class T>
+void op_cheb_step(const DCsr &A, const T *dinv, const T *b, const T *xc, const T *xp, T *xn,
+                  double alpha, double beta, hipStream_t s);
+// x = P xc + q b (elimination interpolation, P has values)
+template <class T>
+void op_interp(const DCsr &P, const T *q, const T *xc, const T *b, T *x, hipStream_t s);
+// x[i] += xc[agg[i]]
+template <class T>
+void op_prolong_add(const int32_t *agg, int64_t n, const T *xc, T *x, hipStream_t s);
+// y = pinv x (dense n x n, FP64 matrix; T vectors)
+template <class T>
+void op_dense_apply(const double *Pinv, int64_t n, const T *x, T *y, hipStream_t s);
+
+void op_d2f(const double *a, float *b, int64_t n, hipStream_t s);
+void op_f2d(const float *a, double *b, int64_t n, hipStream_t s);
+
+// Plan of a deterministic blocked reduction over vectors of length n split
+// into contiguous blocks (independent graphs).  Stage 1 sums fixed chunks
+// clipped to block boundaries (one workgroup each, fixed tree); stage 2
+// adds the chunks of each block in order, so results are run-to-run
+// reproducible.
+struct RedPlan {
+  int64_t n = 0;
+  int nb = 1, nseg = 0;
+  DevBuf<int64_t> seg_lo, seg_hi;  // nseg
+  DevBuf<int32_t> bseg;            // nb + 1
+  DevBuf<int64_t> bptr;            // nb + 1 block offsets
+  mutable DevBuf<double> part;     // nseg * 4
+  std::vector<int64_t> hbptr;
+  void build(int64_t n, const std::vector<int64_t> &blocks, hipStream_t s);
+  // out[c*np + j] = sum over block c of a_j[i] * b_j[i] (b_j == nullptr:
+  // 1), j < np (np <= 4), accumulated in FP64.
+  template <class T>
+  void dots(int np, const T *const *a, const T *const *b, double *out, hipStream_t s) const;
+  // out[c] = sum over block c of a[i]
+  template <class T> void sums(const T *a, double *out, hipStream_t s) const;
+};
+
+// a -= mean of a over each block, means taken from sums[c] (device)
+template <class T> void op_sub_block_mean(T *a, const RedPlan &P, const double *sums, hipStream_t s);
+
+}  // namespace lg

@@ -1,0 +1,310 @@
+// Deterministic device primitives (see prims.hpp).
+#include "prims.hpp"
+
+namespace lg {
+
+namespace {
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+// Block-wide exclusive scan of one value per thread; returns the exclusive
+// prefix of the calling thread and the block total in *total.
+template <class T>
+__device__ T block_exclusive_scan(T v, T *total) {
+  __shared__ T wsum[SCAN_BLOCK / WAVE];
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  T incl = v;
+  for (int off = 1; off < WAVE; off <<= 1) {
+    T o = __shfl_up(incl, off, WAVE);
+    if (lane >= off) incl += o;
+  }
+  if (lane == WAVE - 1) wsum[w] = incl;
+  __syncthreads();
+  T woff = 0, tot = 0;
+  for (int i = 0; i < SCAN_BLOCK / WAVE; i++) {
+    if (i < w) woff += wsum[i];
+    tot += wsum[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return woff + incl - v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const T *in, int64_t n, T *bsum) {
+  int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++)
+    if (base + i < n) s += in[base + i];
+  T tot;
+  block_exclusive_scan<T>(s, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+template <class T>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const T *in, T *out, int64_t n,
+                                                          const T *boff) {
+  int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  T v[SCAN_ITEMS];
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    v[i] = (base + i < n) ? in[base + i] : (T)0;
+    s += v[i];
+  }
+  T tot;
+  T pre = block_exclusive_scan<T>(s, &tot) + (boff ? boff[blockIdx.x] : (T)0);
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    if (base + i < n) out[base + i] = pre;
+    pre += v[i];
+  }
+}
+
+template <class T>
+int64_t exclusive_scan_impl(const T *in, T *out, int64_t n, hipStream_t s) {
+  if (n <= 0) return 0;
+  int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  // last element kept to form the total (out may alias in)
+  T last_in = 0;
+  LG_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(T), hipMemcpyDeviceToHost, s));
+  if (nb == 1) {
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(SCAN_BLOCK), 0, s, in, out, n, (const T *)nullptr);
+    LG_LAUNCH_CHECK();
+  } else {
+    DevBuf<T> bsum(nb);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, bsum.p);
+    LG_LAUNCH_CHECK();
+    exclusive_scan_impl<T>(bsum.p, bsum.p, nb, s);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, out, n,
+                       (const T *)bsum.p);
+    LG_LAUNCH_CHECK();
+    LG_HIP(hipStreamSynchronize(s));
+  }
+  T last_out = 0;
+  LG_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(T), hipMemcpyDeviceToHost, s));
+  LG_HIP(hipStreamSynchronize(s));
+  return (int64_t)last_out + (int64_t)last_in;
+}
+
+// ---------------------------------------------------------------- radix sort
+constexpr int RS_BLOCK = 256;
+constexpr int RS_ITEMS = 16;  // per lane
+constexpr int RS_WAVES = RS_BLOCK / WAVE;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;  // 4096
+constexpr int RS_WTILE = WAVE * RS_ITEMS;     // 1024 items per wave
+
+__global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const uint64_t *keys, int64_t n, int shift,
+                                                     int32_t *hist, int64_t nblocks) {
+  __shared__ int32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int i = threadIdx.x; i < RS_TILE; i += RS_BLOCK) {
+    int64_t k = base + i;
+    if (k < n) atomicAdd(&cnt[(keys[k] >> shift) & 255], 1);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const uint64_t *kin, const uint64_t *vin,
+                                                        uint64_t *kout, uint64_t *vout, int64_t n,
+                                                        int shift, const int32_t *hoff,
+                                                        int64_t nblocks) {
+  __shared__ int32_t cnt[RS_WAVES][256];
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_BLOCK) (&cnt[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t wbase = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * RS_WTILE;
+  uint64_t k[RS_ITEMS], v[RS_ITEMS];
+  int32_t loc[RS_ITEMS];
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; it++) {
+    int64_t idx = wbase + it * WAVE + lane;
+    bool valid = idx < n;
+    k[it] = valid ? kin[idx] : 0;
+    v[it] = valid ? vin[idx] : 0;
+    uint32_t d = (uint32_t)((k[it] >> shift) & 255);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      uint64_t m = __ballot((d >> b) & 1);
+      peers &= ((d >> b) & 1) ? m : ~m;
+    }
+    int32_t before = cnt[w][d];
+    int rank = __popcll(peers & lt);
+    loc[it] = valid ? before + rank : -1;
+    // leader (lowest peer) advances the counter; one writer per digit
+    bool leader = valid && ((peers & lt) == 0);
+    __builtin_amdgcn_wave_barrier();
+    if (leader) cnt[w][d] = before + __popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // per-digit offsets of each wave inside the tile
+  __shared__ int32_t woff[RS_WAVES][256];
+  {
+    int d = threadIdx.x;
+    int32_t acc = 0;
+    for (int ww = 0; ww < RS_WAVES; ww++) {
+      woff[ww][d] = acc;
+      acc += cnt[ww][d];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; it++) {
+    if (loc[it] < 0) continue;
+    uint32_t d = (uint32_t)((k[it] >> shift) & 255);
+    int64_t pos = (int64_t)hoff[(int64_t)d * nblocks + blockIdx.x] + woff[w][d] + loc[it];
+    kout[pos] = k[it];
+    vout[pos] = v[it];
+  }
+}
+
+// ------------------------------------------------------ segmented canon sum
+__global__ void k_seg_heads(const uint64_t *keys, int64_t n, int32_t *flags) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+// 64 segments per wave: a segment of at most 64 terms is summed left to
+// right by its own lane; longer ones by the whole wave, one at a time, with
+// 64 strided partials and the fixed tree (oracle canon_sum).
+__global__ void __launch_bounds__(256) k_seg_sum(const uint64_t *keys, const uint64_t *vals,
+                                                 const int32_t *start, int64_t nseg, int64_t n,
+                                                 uint64_t *out_keys, double *out_sums) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  const int64_t seg = wid * WAVE + lane;
+  const bool valid = seg < nseg;
+  int64_t a = 0, b = 0;
+  if (valid) {
+    a = start[seg];
+    b = (seg + 1 < nseg) ? start[seg + 1] : n;
+  }
+  double res = 0.0;
+  bool longseg = valid && (b - a > 64);
+  if (valid && !longseg) {
+    double s = 0.0;
+    for (int64_t i = a; i < b; i++) s += __longlong_as_double((long long)vals[i]);
+    res = s;
+  }
+  uint64_t mask = __ballot(longseg);
+  while (mask) {
+    int L = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    int64_t la = __shfl(a, L, WAVE), lb = __shfl(b, L, WAVE);
+    double p = 0.0;
+    for (int64_t i = la + lane; i < lb; i += WAVE) p += __longlong_as_double((long long)vals[i]);
+    p = wave_tree_sum(p);
+    p = __shfl(p, 0, WAVE);
+    if (lane == L) res = p;
+  }
+  if (valid) {
+    out_keys[seg] = keys[a];
+    out_sums[seg] = res;
+  }
+}
+
+__global__ void k_compact_write(const int32_t *flags, const int32_t *pos, int64_t n, int32_t *out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (flags[i]) out[pos[i]] = (int32_t)i;
+}
+
+template <class T>
+__global__ void k_fill(T *p, T v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace
+
+int64_t exclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n, hipStream_t s) {
+  return exclusive_scan_impl<int32_t>(in, out, n, s);
+}
+int64_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s) {
+  return exclusive_scan_impl<int64_t>(in, out, n, s);
+}
+
+void radix_sort_pairs(uint64_t *keys, uint64_t *vals, int64_t n, int bits, hipStream_t s) {
+  if (n <= 1 || bits <= 0) return;
+  LG_CHECK(n < ((int64_t)1 << 31), 1, "radix_sort_pairs: too many items");
+  const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+  DevBuf<uint64_t> k2(n), v2(n);
+  DevBuf<int32_t> hist(256 * nb);
+  uint64_t *ka = keys, *va = vals, *kb = k2.p, *vb = v2.p;
+  int passes = (bits + 7) / 8;
+  for (int p = 0; p < passes; p++) {
+    int shift = 8 * p;
+    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nb), dim3(RS_BLOCK), 0, s, ka, n, shift, hist.p, nb);
+    LG_LAUNCH_CHECK();
+    exclusive_scan_i32(hist.p, hist.p, 256 * nb, s);
+    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(RS_BLOCK), 0, s, ka, va, kb, vb, n,
+                       shift, hist.p, nb);
+    LG_LAUNCH_CHECK();
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != keys) {
+    d2d(keys, ka, n, s);
+    d2d(vals, va, n, s);
+  }
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+int64_t segmented_canon_sum(const uint64_t *keys, const uint64_t *vals, int64_t n,
+                            uint64_t *out_keys, double *out_sums, hipStream_t s) {
+  if (n <= 0) return 0;
+  LG_CHECK(n < ((int64_t)1 << 31), 1, "segmented_canon_sum: too many items");
+  DevBuf<int32_t> flags(n), pos(n);
+  hipLaunchKernelGGL(k_seg_heads, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, n, flags.p);
+  LG_LAUNCH_CHECK();
+  int64_t nseg = exclusive_scan_i32(flags.p, pos.p, n, s);
+  DevBuf<int32_t> start(nseg);
+  hipLaunchKernelGGL(k_compact_write, dim3(grid_for(n, 256)), dim3(256), 0, s, flags.p, pos.p, n,
+                     start.p);
+  LG_LAUNCH_CHECK();
+  int64_t waves = (nseg + WAVE - 1) / WAVE;
+  hipLaunchKernelGGL(k_seg_sum, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, keys, vals,
+                     start.p, nseg, n, out_keys, out_sums);
+  LG_LAUNCH_CHECK();
+  LG_HIP(hipStreamSynchronize(s));
+  return nseg;
+}
+
+int64_t compact_flags(const int32_t *flags, int64_t n, int32_t *out, hipStream_t s) {
+  if (n <= 0) return 0;
+  DevBuf<int32_t> pos(n);
+  int64_t cnt = exclusive_scan_i32(flags, pos.p, n, s);
+  hipLaunchKernelGGL(k_compact_write, dim3(grid_for(n, 256)), dim3(256), 0, s, flags, pos.p, n, out);
+  LG_LAUNCH_CHECK();
+  LG_HIP(hipStreamSynchronize(s));
+  return cnt;
+}
+
+void fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, v, n);
+  LG_LAUNCH_CHECK();
+}
+void fill_f64(double *p, double v, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, v, n);
+  LG_LAUNCH_CHECK();
+}
+void fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, v, n);
+  LG_LAUNCH_CHECK();
+}
+
+}  // namespace lg

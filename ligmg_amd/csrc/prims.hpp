@@ -1,0 +1,32 @@
+// Deterministic device primitives: scan, stable LSD radix sort, canonical
+// segmented sums.  All run on one stream; functions that return a host value
+// synchronise that stream.
+#pragma once
+#include "common.hpp"
+
+namespace lg {
+
+// Exclusive prefix sum of in[0..n) into out[0..n) (out may alias in).
+// Returns the total.
+int64_t exclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n, hipStream_t s);
+int64_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
+
+// Stable sort of (key, val) pairs by the low `bits` bits of the keys
+// (8-bit digits, least significant first).  Keys above 2^bits are not
+// allowed.  Sorts in place; tmp buffers are allocated internally.
+void radix_sort_pairs(uint64_t *keys, uint64_t *vals, int64_t n, int bits, hipStream_t s);
+
+// Canonical sum (the order of the CPU oracle's canon_sum) of the values of
+// each run of equal keys in a sorted key array.  Writes the run keys and
+// sums; returns the number of runs.  vals are doubles stored as raw bits.
+int64_t segmented_canon_sum(const uint64_t *keys, const uint64_t *vals, int64_t n,
+                            uint64_t *out_keys, double *out_sums, hipStream_t s);
+
+// out[k] = index of the k-th nonzero flag (ascending); returns the count.
+int64_t compact_flags(const int32_t *flags, int64_t n, int32_t *out, hipStream_t s);
+
+void fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s);
+void fill_f64(double *p, double v, int64_t n, hipStream_t s);
+void fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s);
+
+}  // namespace lg

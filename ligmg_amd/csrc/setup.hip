@@ -1,0 +1,1147 @@
+// Device setup: low-degree elimination, affinity strength, voting
+// aggregation, Galerkin products by contraction (sort + canonical segmented
+// sum), smoother set-up and the coarse pseudo-inverse.
+//
+// Every floating-point sum whose order could change a result uses the
+// canonical order: a row (or run of equal keys) of at most 64 terms is
+// summed left to right by one lane; a longer one by a whole wavefront as 64
+// strided partial sums combined by the tree p[l] += p[l+off], off = 32..1.
+// Masked row positions contribute +0.0.  Files compiled with
+// -ffp-contract=off, so products and sums round exactly as written.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "../../include/ligmg_amd.h"
+#include "ops.hpp"
+#include "prims.hpp"
+#include "rng.hpp"
+#include "setup.hpp"
+
+namespace lg {
+
+namespace {
+
+enum { ST_SEED = 0, ST_SEEDNEIGHBOR = 1, ST_DECIDED = 2, ST_DECIDEDROOT = 3, ST_UNDECIDED = 4 };
+// vote priorities: Seed > Undecided > Decided (aggregation.hpp:408-417)
+enum { PRIO_DECIDED = 0, PRIO_UNDECIDED = 1, PRIO_SEED = 2 };
+
+constexpr int MAXT = 16;
+
+// --------------------------------------------------------------------------
+// Generic canonical row kernel: one lane per row, rows longer than 64
+// entries done cooperatively by the wavefront.
+// --------------------------------------------------------------------------
+template <class Op>
+__global__ void __launch_bounds__(256) k_rows(int n, const int32_t *__restrict__ rp, Op op) {
+  using Acc = typename Op::Acc;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = row < n;
+  int a = 0, b = 0;
+  if (valid) {
+    a = rp[row];
+    b = rp[row + 1];
+  }
+  const bool lng = valid && (b - a > WAVE);
+  Acc acc = op.zero();
+  if (valid && !lng)
+    for (int k = a; k < b; k++) op.add(acc, row, k);
+  uint64_t mask = __ballot(lng);
+  while (mask) {
+    const int L = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    const int la = __shfl(a, L, WAVE), lb = __shfl(b, L, WAVE), lr = __shfl(row, L, WAVE);
+    Acc p = op.zero();
+    for (int k = la + lane; k < lb; k += WAVE) op.add(p, lr, k);
+    for (int off = 32; off >= 1; off >>= 1) {
+      Acc o = op.shfl_down(p, off);
+      p = op.merge(p, o);
+    }
+    p = op.shfl0(p);
+    if (lane == L) acc = p;
+  }
+  if (valid) op.finish(row, acc);
+}
+
+template <class Op>
+void run_rows(int64_t n, const int32_t *rp, const Op &op, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_rows<Op>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int)n, rp, op);
+  LG_LAUNCH_CHECK();
+}
+
+struct IntSum {
+  using Acc = int;
+  __device__ Acc zero() const { return 0; }
+  __device__ Acc merge(Acc a, Acc b) const { return a + b; }
+  __device__ Acc shfl_down(Acc a, int off) const { return __shfl_down(a, off, WAVE); }
+  __device__ Acc shfl0(Acc a) const { return __shfl(a, 0, WAVE); }
+};
+struct DblSum {
+  using Acc = double;
+  __device__ Acc zero() const { return 0.0; }
+  __device__ Acc merge(Acc a, Acc b) const { return a + b; }
+  __device__ Acc shfl_down(Acc a, int off) const { return __shfl_down(a, off, WAVE); }
+  __device__ Acc shfl0(Acc a) const { return __shfl(a, 0, WAVE); }
+};
+struct DblMax {  // std::max semantics: (a < b) ? b : a
+  using Acc = double;
+  __device__ Acc zero() const { return 0.0; }
+  __device__ Acc merge(Acc a, Acc b) const { return (a < b) ? b : a; }
+  __device__ Acc shfl_down(Acc a, int off) const { return __shfl_down(a, off, WAVE); }
+  __device__ Acc shfl0(Acc a) const { return __shfl(a, 0, WAVE); }
+};
+struct NoAcc {
+  using Acc = int;
+  __device__ Acc zero() const { return 0; }
+  __device__ Acc merge(Acc a, Acc) const { return a; }
+  __device__ Acc shfl_down(Acc a, int) const { return a; }
+  __device__ Acc shfl0(Acc a) const { return a; }
+};
+
+// ---- a1: compute_degrees (elimination.hpp:70-80) -----------------------------
+struct OpDeg : IntSum {
+  const double *v;
+  int32_t *deg;
+  __device__ void add(int &acc, int, int k) const { acc += (v[k] == 0) ? 0 : 1; }
+  __device__ void finish(int row, int acc) const { deg[row] = acc - 1; }
+};
+
+// ---- a2: IndxHashMin SpMV: smallest candidate index over A_ij != 0
+//      (elimination.hpp:122-154, 201-202)
+struct OpElimMin {
+  using Acc = int;
+  const int32_t *ci, *deg;
+  const double *v;
+  int maxd;
+  int32_t *elim;
+  __device__ Acc zero() const { return 0x7fffffff; }
+  __device__ Acc merge(Acc a, Acc b) const { return a < b ? a : b; }
+  __device__ Acc shfl_down(Acc a, int off) const { return __shfl_down(a, off, WAVE); }
+  __device__ Acc shfl0(Acc a) const { return __shfl(a, 0, WAVE); }
+  __device__ void add(int &acc, int, int k) const {
+    int j = ci[k];
+    if (v[k] != 0 && deg[j] <= maxd && j < acc) acc = j;
+  }
+  __device__ void finish(int row, int acc) const { elim[row] = (acc == row) ? 1 : 0; }
+};
+
+// Schur contraction entry count per row: kept rows keep their kept
+// columns; an eliminated row with m off-diagonal entries emits m*m.
+struct OpSchurCount : IntSum {
+  const int32_t *rp, *ci, *newid;
+  int32_t *cnt;
+  __device__ void add(int &acc, int, int k) const { acc += (newid[ci[k]] >= 0) ? 1 : 0; }
+  __device__ void finish(int row, int acc) const {
+    if (newid[row] >= 0) {
+      cnt[row] = acc;
+    } else {
+      int m = 0;
+      for (int k = rp[row]; k < rp[row + 1]; k++) m += (ci[k] != row);
+      cnt[row] = m * m;
+    }
+  }
+};
+
+// every off-diagonal neighbour of an eliminated vertex must be kept
+__global__ void k_elim_check(int n, const int32_t *rp, const int32_t *ci, const int32_t *newid,
+                             int32_t *bad) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n || newid[row] >= 0) return;
+  for (int k = rp[row]; k < rp[row + 1]; k++)
+    if (ci[k] != row && newid[ci[k]] < 0) atomicOr(bad, 1);
+}
+
+// P row lengths: kept 1, eliminated = off-diagonal entries
+__global__ void k_p_count(int n, const int32_t *rp, const int32_t *ci, const int32_t *newid,
+                          int32_t *cnt) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  if (newid[row] >= 0) {
+    cnt[row] = 1;
+    return;
+  }
+  int m = 0;
+  for (int k = rp[row]; k < rp[row + 1]; k++) m += (ci[k] != row);
+  cnt[row] = m;
+}
+
+__device__ __forceinline__ double row_diag(const int32_t *rp, const int32_t *ci, const double *v,
+                                           int row, bool *found) {
+  int lo = rp[row], hi = rp[row + 1] - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int c = ci[mid];
+    if (c == row) {
+      *found = true;
+      return v[mid];
+    }
+    if (c < row) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  *found = false;
+  return 0.0;
+}
+
+// ---- a4: P, Q assembly (elimination.hpp:287-331); P(e, new i) = 1/d * -L_ei
+__global__ void k_p_fill(int n, const int32_t *rp, const int32_t *ci, const double *v,
+                         const int32_t *newid, const int32_t *prp, int32_t *pci, double *pv,
+                         double *q) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  int o = prp[row];
+  if (newid[row] >= 0) {
+    pci[o] = newid[row];
+    pv[o] = 1.0;
+    q[row] = 0.0;
+    return;
+  }
+  bool f;
+  double d = row_diag(rp, ci, v, row, &f);
+  q[row] = 1 / d;
+  for (int k = rp[row]; k < rp[row + 1]; k++) {
+    int i = ci[k];
+    if (i == row) continue;
+    pci[o] = newid[i];
+    pv[o] = 1 / d * -v[k];
+    o++;
+  }
+}
+
+// ---- a6: Schur contraction entries, in the oracle's list order: rows in
+//      ascending order; a kept row lists its kept columns; an eliminated row
+//      e lists (a, b) -> L_ea * P(e, b) over its off-diagonal pairs.
+__global__ void __launch_bounds__(256) k_schur_fill(int n, const int32_t *rp, const int32_t *ci,
+                                                    const double *v, const int32_t *newid,
+                                                    const int32_t *off, const int32_t *prp,
+                                                    const double *pv, int cb, uint64_t *keys,
+                                                    uint64_t *vals) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = row < n;
+  int a = 0, b = 0;
+  bool kept = false;
+  if (valid) {
+    a = rp[row];
+    b = rp[row + 1];
+    kept = newid[row] >= 0;
+  }
+  const bool lng = valid && kept && (b - a > WAVE);
+  if (valid && !lng) {
+    int o = off[row];
+    if (kept) {
+      uint64_t r = (uint64_t)newid[row] << cb;
+      for (int k = a; k < b; k++) {
+        int c = newid[ci[k]];
+        if (c >= 0) {
+          keys[o] = r | (uint64_t)c;
+          vals[o] = (uint64_t)__double_as_longlong(v[k]);
+          o++;
+        }
+      }
+    } else {
+      int pk = prp[row];
+      for (int k = a; k < b; k++) {
+        if (ci[k] == row) continue;
+        double wa = v[k];
+        uint64_t na = (uint64_t)newid[ci[k]] << cb;
+        int pt = prp[row];
+        for (int k2 = a; k2 < b; k2++) {
+          if (ci[k2] == row) continue;
+          keys[o] = na | (uint64_t)newid[ci[k2]];
+          vals[o] = (uint64_t)__double_as_longlong(wa * pv[pt]);
+          o++;
+          pt++;
+        }
+        pk++;
+      }
+    }
+  }
+  uint64_t mask = __ballot(lng);
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  while (mask) {
+    const int L = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    const int la = __shfl(a, L, WAVE), lb = __shfl(b, L, WAVE), lr = __shfl(row, L, WAVE);
+    int o = off[lr];
+    const uint64_t r = (uint64_t)newid[lr] << cb;
+    for (int base = la; base < lb; base += WAVE) {
+      int k = base + lane;
+      int c = (k < lb) ? newid[ci[k]] : -1;
+      uint64_t m = __ballot(c >= 0);
+      if (c >= 0) {
+        int pos = o + __popcll(m & lt);
+        keys[pos] = r | (uint64_t)c;
+        vals[pos] = (uint64_t)__double_as_longlong(v[k]);
+      }
+      o += __popcll(m);
+    }
+  }
+}
+
+// CSR from sorted unique keys: rows counted, columns = low bits.
+__global__ void k_key_rows(const uint64_t *keys, int64_t m, int cb, int32_t *cnt, int32_t *ci) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    atomicAdd(&cnt[k >> cb], 1);
+    ci[i] = (int32_t)(k & (((uint64_t)1 << cb) - 1));
+  }
+}
+
+// transpose of P: keys (col << rb) | row
+__global__ void k_p_transpose_keys(int n, const int32_t *prp, const int32_t *pci, const double *pv,
+                                   int rb, uint64_t *keys, uint64_t *vals) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  for (int k = prp[row]; k < prp[row + 1]; k++) {
+    keys[k] = ((uint64_t)pci[k] << rb) | (uint64_t)row;
+    vals[k] = (uint64_t)__double_as_longlong(pv[k]);
+  }
+}
+
+__global__ void k_unpack_pairs(const uint64_t *keys, const uint64_t *vals, int64_t m, int rb,
+                               int32_t *cnt, int32_t *ci, double *v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    atomicAdd(&cnt[k >> rb], 1);
+    ci[i] = (int32_t)(k & (((uint64_t)1 << rb) - 1));
+    v[i] = __longlong_as_double((long long)vals[i]);
+  }
+}
+
+// ---- a7: test vectors (aggregation.hpp:24-36) ------------------------------
+__global__ void k_tv_sum(int n, uint64_t seed, int level, int T, unsigned long long *sums) {
+  // per-block partial sums in registers, one atomic per block and t
+  for (int t = 0; t < T; t++) {
+    unsigned long long s = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+      s += ligmg_rng::tv_bits(seed, level, t, i);
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&sums[t], s);
+  }
+}
+
+__global__ void k_tv_fill(int n, uint64_t seed, int level, int T, const unsigned long long *sums,
+                          double *X) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int t = 0; t < T; t++) {
+    double mean = ((double)sums[t] / (double)n) * 0x1p-32;
+    X[(int64_t)i * T + t] = (double)ligmg_rng::tv_bits(seed, level, t, i) * 0x1p-32 - mean;
+  }
+}
+
+// ---- a8: Jacobi on the test vectors, IgnoreDiagSR, b = 0
+//      (smoothing.hpp:11-32, 78-98)
+struct TVAcc {
+  double v[MAXT];
+};
+struct OpJacobiTV {
+  using Acc = TVAcc;
+  const int32_t *ci;
+  const double *v, *X, *diag;
+  const uint8_t *hasd;
+  double *Y;
+  int T;
+  __device__ Acc zero() const {
+    Acc a;
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) a.v[t] = 0.0;
+    return a;
+  }
+  __device__ Acc merge(const Acc &a, const Acc &b) const {
+    Acc r;
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) r.v[t] = a.v[t] + b.v[t];
+    return r;
+  }
+  __device__ Acc shfl_down(const Acc &a, int off) const {
+    Acc r;
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) r.v[t] = __shfl_down(a.v[t], off, WAVE);
+    return r;
+  }
+  __device__ Acc shfl0(const Acc &a) const {
+    Acc r;
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) r.v[t] = __shfl(a.v[t], 0, WAVE);
+    return r;
+  }
+  __device__ void add(Acc &acc, int, int k) const {
+    const double a = v[k];
+    const double *xj = X + (int64_t)ci[k] * T;
+#pragma unroll
+    for (int t = 0; t < MAXT; t++)
+      if (t < T) acc.v[t] += (a < 0) ? a * xj[t] : 0.0;
+  }
+  __device__ void finish(int row, const Acc &acc) const {
+    const double w = 2.0 / 3.0;
+    const double d = diag[row];
+    for (int t = 0; t < T; t++) {
+      double x = X[(int64_t)row * T + t];
+      Y[(int64_t)row * T + t] = hasd[row] ? w * (0.0 - acc.v[t]) / d + (1 - w) * x : x;
+    }
+  }
+};
+
+__global__ void k_diag(int n, const int32_t *rp, const int32_t *ci, const double *v, double *diag,
+                       uint8_t *hasd) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  bool f;
+  double d = row_diag(rp, ci, v, row, &f);
+  diag[row] = d;
+  hasd[row] = f ? 1 : 0;
+}
+
+__global__ void k_tv_norm(int n, int T, const double *X, double *nrm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0;
+  for (int t = 0; t < T; t++) s += X[(int64_t)i * T + t] * X[(int64_t)i * T + t];
+  nrm[i] = s;
+}
+
+// ---- a9: affinity_matrix (aggregation.hpp:276-404) -------------------------
+struct OpAffinity : DblMax {
+  const int32_t *ci;
+  const double *v, *X, *nrm;
+  int T, metric;
+  double *raw, *rowmax;
+  __device__ void add(double &acc, int row, int k) const {
+    const int c = ci[k];
+    double r;
+    if (c == row) {
+      r = (metric == LIGMG_STRENGTH_NONE) ? (v[k] < 0 ? -v[k] : 0.0) : 0.0;
+    } else if (metric == LIGMG_STRENGTH_AFFINITY) {
+      const double *xc = X + (int64_t)c * T, *xr = X + (int64_t)row * T;
+      double top = 0;
+      for (int t = 0; t < T; t++) top += xc[t] * xr[t];
+      r = top * top / (nrm[c] * nrm[c] * nrm[row] * nrm[row]);
+    } else if (metric == LIGMG_STRENGTH_ALGEBRAIC_DISTANCE) {
+      const double *xc = X + (int64_t)c * T, *xr = X + (int64_t)row * T;
+      double m = 0;
+      for (int t = 0; t < T; t++) {
+        double d = fabs(xc[t] - xr[t]);
+        m = (m < d) ? d : m;
+      }
+      r = 1 / m;
+    } else {
+      r = v[k] < 0 ? -v[k] : 0.0;
+    }
+    raw[k] = r;
+    acc = (acc < r) ? r : acc;
+  }
+  __device__ void finish(int row, double acc) const { rowmax[row] = acc; }
+};
+
+// position of the transposed entry: tpos[k] for k = (r, c) is the index of
+// (c, r) in row c (the pattern is structurally symmetric)
+struct OpTpos : NoAcc {
+  const int32_t *rp, *ci;
+  int32_t *tpos, *bad;
+  __device__ void add(int &, int row, int k) const {
+    int c = ci[k];
+    int lo = rp[c], hi = rp[c + 1] - 1, f = -1;
+    while (lo <= hi) {
+      int mid = (lo + hi) >> 1;
+      int x = ci[mid];
+      if (x == row) {
+        f = mid;
+        break;
+      }
+      if (x < row) lo = mid + 1;
+      else hi = mid - 1;
+    }
+    tpos[k] = f;
+    if (f < 0) atomicOr(bad, 1);
+  }
+  __device__ void finish(int, int) const {}
+};
+
+struct OpColMax : DblMax {
+  const int32_t *tpos;
+  const double *raw;
+  double *colmax;
+  __device__ void add(double &acc, int, int k) const {
+    double r = raw[tpos[k]];
+    acc = (acc < r) ? r : acc;
+  }
+  __device__ void finish(int row, double acc) const { colmax[row] = acc; }
+};
+
+struct OpScale : NoAcc {
+  const int32_t *ci;
+  const double *raw, *rowmax, *colmax;
+  double thresh;
+  int scaling;
+  double *S;
+  __device__ void add(int &, int row, int k) const {
+    double s = raw[k];
+    if (scaling == LIGMG_AFF_MAX) {
+      double a = rowmax[row], b = colmax[ci[k]];
+      double den = (a < b) ? b : a;
+      s = den > 0 ? s / den : 0.0;
+    }
+    S[k] = (s < thresh || s != s) ? 0.0 : s;
+  }
+  __device__ void finish(int, int) const {}
+};
+
+// ---- a10: col_sum = Reduce(Column) in ascending row order
+struct OpColSum : DblSum {
+  const int32_t *tpos;
+  const double *S;
+  double *cs;
+  __device__ void add(double &acc, int, int k) const { acc += S[tpos[k]]; }
+  __device__ void finish(int row, double acc) const { cs[row] = acc; }
+};
+
+// ---- a11: vote with AggSR (aggregation.hpp:406-518).  The left fold with
+//      ties to the later operand equals the maximum of (priority, weight,
+//      position), which any reduction order reproduces.
+struct VoteAcc {
+  int prio, pos, ind;
+  double w;
+};
+struct OpVote {
+  using Acc = VoteAcc;
+  const int32_t *ci, *state, *ind;
+  const double *S;
+  double thresh;
+  int8_t *vst;
+  int32_t *vind;
+  double *vw;
+  __device__ Acc zero() const { return {-1, -1, -1, -1.0}; }
+  __device__ static bool better(const Acc &b, const Acc &a) {  // b beats a
+    if (b.prio != a.prio) return b.prio > a.prio;
+    if (b.w != a.w) return b.w > a.w;
+    return b.pos > a.pos;
+  }
+  __device__ Acc merge(const Acc &a, const Acc &b) const { return better(b, a) ? b : a; }
+  __device__ Acc shfl_down(const Acc &a, int off) const {
+    return {__shfl_down(a.prio, off, WAVE), __shfl_down(a.pos, off, WAVE),
+            __shfl_down(a.ind, off, WAVE), __shfl_down(a.w, off, WAVE)};
+  }
+  __device__ Acc shfl0(const Acc &a) const {
+    return {__shfl(a.prio, 0, WAVE), __shfl(a.pos, 0, WAVE), __shfl(a.ind, 0, WAVE),
+            __shfl(a.w, 0, WAVE)};
+  }
+  __device__ void add(Acc &acc, int, int k) const {
+    const double s = S[k];
+    if (s == 0.0) return;
+    const int j = ci[k];
+    const int st = state[j];
+    int p;
+    if (st == ST_SEED) p = (s > thresh) ? PRIO_SEED : PRIO_DECIDED;
+    else if (st == ST_UNDECIDED) p = PRIO_UNDECIDED;
+    else p = PRIO_DECIDED;
+    Acc m = {p, k, ind[j], s};
+    if (better(m, acc)) acc = m;
+  }
+  __device__ void finish(int row, const Acc &acc) const {
+    if (acc.prio < 0) {
+      vst[row] = PRIO_DECIDED;
+      vind[row] = -1;
+      vw[row] = -1.0;
+    } else {
+      vst[row] = (int8_t)acc.prio;
+      vind[row] = acc.ind;
+      vw[row] = acc.w;
+    }
+  }
+};
+
+// ---- a12: tally_votes (aggregation.hpp:522-561) ----------------------------
+__global__ void k_voters(int n, const int32_t *state, const int32_t *vind, const double *vw,
+                         double thresh, int32_t *flag, int32_t *vf) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool ok = state[i] == ST_UNDECIDED && vind[i] >= 0 && vw[i] >= thresh && vind[i] != i;
+  flag[i] = ok ? 1 : 0;
+  vf[i] = ok ? vind[i] : -1;
+}
+
+__global__ void k_voter_pairs(int64_t m, const int32_t *idx, const int32_t *vind, const double *vw,
+                              uint64_t *keys, uint64_t *vals) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < m;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int i = idx[t];
+    keys[t] = (uint64_t)vind[i];
+    vals[t] = (uint64_t)__double_as_longlong(vw[i]);
+  }
+}
+
+__global__ void k_scatter_sums(int64_t m, const uint64_t *keys, const double *sums, double *nv) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < m;
+       t += (int64_t)gridDim.x * blockDim.x)
+    nv[keys[t]] = sums[t];
+}
+
+__global__ void k_add_votes(int n, const double *nv, double *votes) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) votes[i] = nv[i] + votes[i];
+}
+
+// ---- a13: mark_nodes (aggregation.hpp:702-750) -----------------------------
+__global__ void k_mark(int n, int first, const int32_t *state, const int32_t *ind, const int8_t *vst,
+                       const int32_t *vind, const int32_t *vf, const double *votes,
+                       const double *cs, double votes_to_seed, double max_agg, int32_t *nstate,
+                       int32_t *nind) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int st = state[i], id = ind[i];
+  if (st == ST_DECIDED || st == ST_DECIDEDROOT || st == ST_SEEDNEIGHBOR) {
+  } else if (st == ST_SEED) {
+    if (votes[i] > max_agg) st = ST_DECIDEDROOT;
+  } else if (vf[i] >= 0 && first && vf[vf[i]] == i) {
+    if (i < vf[i]) {
+      st = ST_SEED;
+      id = i;
+    } else {
+      st = ST_DECIDED;
+      id = vf[i];
+    }
+  } else if (vst[i] == PRIO_SEED) {
+    st = ST_DECIDED;
+    id = vind[i];
+  } else if (votes[i] >= votes_to_seed * cs[i]) {
+    st = ST_SEED;
+    id = i;
+  }
+  nstate[i] = st;
+  nind[i] = id;
+}
+
+// ---- a14: R with zero rows removed -> coarse id = rank of the root
+__global__ void k_mark_roots(int n, const int32_t *ind, int32_t *root) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) root[ind[i]] = 1;
+}
+__global__ void k_agg_ids(int n, const int32_t *ind, const int32_t *cid, int32_t *agg) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) agg[i] = cid[ind[i]];
+}
+
+__global__ void k_init_states(int n, int32_t *state, int32_t *ind) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    state[i] = ST_UNDECIDED;
+    ind[i] = i;
+  }
+}
+
+// ---- aggregation Galerkin entries (agg_i, agg_j, L_ij) in CSR order
+struct OpGalAgg : NoAcc {
+  const int32_t *ci, *agg;
+  const double *v;
+  int cb;
+  uint64_t *keys, *vals;
+  __device__ void add(int &, int row, int k) const {
+    keys[k] = ((uint64_t)agg[row] << cb) | (uint64_t)agg[ci[k]];
+    vals[k] = (uint64_t)__double_as_longlong(v[k]);
+  }
+  __device__ void finish(int, int) const {}
+};
+
+__global__ void k_member_pairs(int n, const int32_t *agg, uint64_t *keys, uint64_t *vals) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    keys[i] = (uint64_t)agg[i];
+    vals[i] = (uint64_t)i;
+  }
+}
+
+__global__ void k_members_unpack(int n, const uint64_t *keys, const uint64_t *vals, int32_t *cnt,
+                                 int32_t *mem) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    atomicAdd(&cnt[keys[i]], 1);
+    mem[i] = (int32_t)vals[i];
+  }
+}
+
+__global__ void k_dinv(int n, const double *diag, const uint8_t *hasd, double *dinv, float *dinv32) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = hasd[i] ? diag[i] : 0.0;
+  double r = d != 0 ? 1.0 / d : 0.0;
+  dinv[i] = r;
+  if (dinv32) dinv32[i] = (float)r;
+}
+
+unsigned g1(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256); }
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// Contraction: (key, value) list -> CSR of nrows rows.
+// ----------------------------------------------------------------------------
+void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, int64_t nrows,
+                     int64_t ncols, DCsr &out, hipStream_t s) {
+  const int cb = std::max(1, bits_for((uint64_t)ncols));
+  const int rb = std::max(1, bits_for((uint64_t)nrows));
+  radix_sort_pairs(keys.p, vals.p, m, cb + rb, s);
+  DevBuf<uint64_t> uk(m);
+  DevBuf<double> us(m);
+  int64_t nu = segmented_canon_sum(keys.p, vals.p, m, uk.p, us.p, s);
+  keys.release();
+  vals.release();
+  out.n = nrows;
+  out.ncols = ncols;
+  out.nnz = nu;
+  out.rp.alloc(nrows + 1);
+  out.ci.alloc(std::max<int64_t>(nu, 1));
+  out.v.alloc(std::max<int64_t>(nu, 1));
+  fill_i32(out.rp.p, 0, nrows + 1, s);
+  if (nu)
+    hipLaunchKernelGGL(k_key_rows, dim3(grid_for(nu, 256, 8192)), dim3(256), 0, s, uk.p, nu, cb,
+                       out.rp.p, out.ci.p);
+  LG_LAUNCH_CHECK();
+  exclusive_scan_i32(out.rp.p, out.rp.p, nrows + 1, s);
+  d2d(out.v.p, us.p, nu, s);
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------------------
+// Elimination level (elimination.hpp:156-345, multigrid.hpp:132-149)
+// ----------------------------------------------------------------------------
+bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &coarse,
+                       hipStream_t s) {
+  const int64_t n = A.n;
+  DevBuf<int32_t> deg(n), elim(n);
+  run_rows(n, A.rp.p, OpDeg{{}, A.v.p, deg.p}, s);
+  run_rows(n, A.rp.p, OpElimMin{A.ci.p, deg.p, A.v.p, o.elim_degree, elim.p}, s);
+  // kept flags -> new ids
+  DevBuf<int32_t> kept(n), newid(n);
+  {
+    // kept = 1 - elim
+    std::vector<int32_t> e = download(elim.p, n, s);
+    for (auto &x : e) x = 1 - x;
+    to_device(kept.p, e.data(), n, s);
+  }
+  int64_t n_kept = exclusive_scan_i32(kept.p, newid.p, n, s);
+  if ((double)n_kept > (double)n * o.elim_factor) return false;  // elimination.hpp:226-228
+  {
+    std::vector<int32_t> k = download(kept.p, n, s), id = download(newid.p, n, s);
+    for (int64_t i = 0; i < n; i++)
+      if (!k[i]) id[i] = -1;
+    to_device(newid.p, id.data(), n, s);
+  }
+  {
+    DevBuf<int32_t> bad(1);
+    fill_i32(bad.p, 0, 1, s);
+    hipLaunchKernelGGL(k_elim_check, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, newid.p,
+                       bad.p);
+    LG_LAUNCH_CHECK();
+    int32_t hb = 0;
+    to_host(&hb, bad.p, 1, s);
+    LG_HIP(hipStreamSynchronize(s));
+    LG_CHECK(hb == 0, LIGMG_ERR_NUMERIC,
+             "eliminated vertices are adjacent (zero-valued stored entries in a row)");
+  }
+  L.kind = LIGMG_LEVEL_ELIM;
+  L.n = n;
+  L.nc = n_kept;
+  L.new_id = std::move(newid);
+  // P and q
+  DevBuf<int32_t> pcnt(n + 1);
+  fill_i32(pcnt.p, 0, n + 1, s);
+  hipLaunchKernelGGL(k_p_count, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, L.new_id.p, pcnt.p);
+  LG_LAUNCH_CHECK();
+  int64_t nnzP = exclusive_scan_i32(pcnt.p, pcnt.p, n + 1, s);
+  L.P.n = n;
+  L.P.ncols = n_kept;
+  L.P.nnz = nnzP;
+  L.P.rp = std::move(pcnt);
+  L.P.ci.alloc(nnzP);
+  L.P.v.alloc(nnzP);
+  L.q.alloc(n);
+  hipLaunchKernelGGL(k_p_fill, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p, L.new_id.p,
+                     L.P.rp.p, L.P.ci.p, L.P.v.p, L.q.p);
+  LG_LAUNCH_CHECK();
+  L.nnzP = nnzP;
+  L.nnzQ = n - n_kept;
+  // R = P^T
+  {
+    DevBuf<uint64_t> keys(nnzP), vals(nnzP);
+    const int rb = std::max(1, bits_for((uint64_t)n));
+    const int cb = std::max(1, bits_for((uint64_t)n_kept));
+    hipLaunchKernelGGL(k_p_transpose_keys, dim3(g1(n)), dim3(256), 0, s, (int)n, L.P.rp.p, L.P.ci.p,
+                       L.P.v.p, rb, keys.p, vals.p);
+    LG_LAUNCH_CHECK();
+    radix_sort_pairs(keys.p, vals.p, nnzP, rb + cb, s);
+    L.R.n = n_kept;
+    L.R.ncols = n;
+    L.R.nnz = nnzP;
+    L.R.rp.alloc(n_kept + 1);
+    L.R.ci.alloc(nnzP);
+    L.R.v.alloc(nnzP);
+    fill_i32(L.R.rp.p, 0, n_kept + 1, s);
+    hipLaunchKernelGGL(k_unpack_pairs, dim3(grid_for(nnzP, 256, 8192)), dim3(256), 0, s, keys.p, vals.p,
+                       nnzP, rb, L.R.rp.p, L.R.ci.p, L.R.v.p);
+    LG_LAUNCH_CHECK();
+    exclusive_scan_i32(L.R.rp.p, L.R.rp.p, n_kept + 1, s);
+  }
+  // Schur complement by contraction
+  DevBuf<int32_t> cnt(n + 1);
+  fill_i32(cnt.p, 0, n + 1, s);
+  run_rows(n, A.rp.p, OpSchurCount{{}, A.rp.p, A.ci.p, L.new_id.p, cnt.p}, s);
+  int64_t m = exclusive_scan_i32(cnt.p, cnt.p, n + 1, s);
+  DevBuf<uint64_t> keys(m), vals(m);
+  const int cb = std::max(1, bits_for((uint64_t)n_kept));
+  hipLaunchKernelGGL(k_schur_fill, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p,
+                     L.new_id.p, cnt.p, L.P.rp.p, L.P.v.p, cb, keys.p, vals.p);
+  LG_LAUNCH_CHECK();
+  contract_to_csr(keys, vals, m, n_kept, n_kept, coarse, s);
+  return true;
+}
+
+// ----------------------------------------------------------------------------
+// Strength of connection (affinity_matrix + col_sum)
+// ----------------------------------------------------------------------------
+void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevBuf<double> &S,
+                      DevBuf<double> &cs, hipStream_t s) {
+  const int64_t n = A.n, nnz = A.nnz;
+  const int T = o.num_tvs;
+  DevBuf<double> X((size_t)n * T), Y((size_t)n * T), nrm(n), diag(n), raw(nnz), rowmax(n), colmax(n);
+  DevBuf<uint8_t> hasd(n);
+  DevBuf<int32_t> tpos(nnz), bad(1);
+  hipLaunchKernelGGL(k_diag, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p, diag.p, hasd.p);
+  LG_LAUNCH_CHECK();
+  if (o.strength_metric != LIGMG_STRENGTH_NONE) {
+    DevBuf<unsigned long long> sums(T);
+    LG_HIP(hipMemsetAsync(sums.p, 0, sizeof(unsigned long long) * T, s));
+    hipLaunchKernelGGL(k_tv_sum, dim3(std::min<unsigned>(g1(n), 1024)), dim3(256), 0, s, (int)n, o.seed,
+                       level_idx, T, sums.p);
+    LG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tv_fill, dim3(g1(n)), dim3(256), 0, s, (int)n, o.seed, level_idx, T, sums.p, X.p);
+    LG_LAUNCH_CHECK();
+    for (int sw = 0; sw < o.tvs_smooth; sw++) {
+      run_rows(n, A.rp.p, OpJacobiTV{A.ci.p, A.v.p, X.p, diag.p, hasd.p, Y.p, T}, s);
+      std::swap(X.p, Y.p);
+    }
+    hipLaunchKernelGGL(k_tv_norm, dim3(g1(n)), dim3(256), 0, s, (int)n, T, X.p, nrm.p);
+    LG_LAUNCH_CHECK();
+    LG_HIP(hipStreamSynchronize(s));
+  }
+  run_rows(n, A.rp.p, OpAffinity{{}, A.ci.p, A.v.p, X.p, nrm.p, T, o.strength_metric, raw.p, rowmax.p}, s);
+  LG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
+  run_rows(n, A.rp.p, OpTpos{{}, A.rp.p, A.ci.p, tpos.p, bad.p}, s);
+  int32_t hb = 0;
+  to_host(&hb, bad.p, 1, s);
+  LG_HIP(hipStreamSynchronize(s));
+  LG_CHECK(hb == 0, LIGMG_ERR_INVALID, "matrix pattern is not structurally symmetric");
+  run_rows(n, A.rp.p, OpColMax{{}, tpos.p, raw.p, colmax.p}, s);
+  S.alloc(nnz);
+  run_rows(n, A.rp.p, OpScale{{}, A.ci.p, raw.p, rowmax.p, colmax.p, o.thresh, o.aff_scaling, S.p}, s);
+  cs.alloc(n);
+  run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S.p, cs.p}, s);
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------------------
+// Voting (aggregation.hpp:838-964)
+// ----------------------------------------------------------------------------
+int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg_options &o,
+                   DevBuf<int32_t> &agg, DevBuf<int32_t> &ind, DevBuf<int32_t> &state, hipStream_t s) {
+  const int64_t n = A.n;
+  state.alloc(n);
+  ind.alloc(n);
+  DevBuf<int32_t> nstate(n), nind(n), vind(n), vf(n), flag(n), idx(n);
+  DevBuf<int8_t> vst(n);
+  DevBuf<double> vw(n), votes(n), nv(n);
+  hipLaunchKernelGGL(k_init_states, dim3(g1(n)), dim3(256), 0, s, (int)n, state.p, ind.p);
+  LG_LAUNCH_CHECK();
+  fill_f64(votes.p, 0.0, n, s);
+  const int kb = std::max(1, bits_for((uint64_t)n));
+  for (int iter = 0; iter < o.agg_iters; iter++) {
+    const double th = o.agg_init_factor * std::pow(o.agg_iter_factor, iter);
+    run_rows(n, A.rp.p, OpVote{A.ci.p, state.p, ind.p, S, th, vst.p, vind.p, vw.p}, s);
+    hipLaunchKernelGGL(k_voters, dim3(g1(n)), dim3(256), 0, s, (int)n, state.p, vind.p, vw.p, th, flag.p,
+                       vf.p);
+    LG_LAUNCH_CHECK();
+    int64_t m = compact_flags(flag.p, n, idx.p, s);
+    fill_f64(nv.p, 0.0, n, s);
+    if (m > 0) {
+      DevBuf<uint64_t> keys(m), vals(m), uk(m);
+      DevBuf<double> us(m);
+      hipLaunchKernelGGL(k_voter_pairs, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, m, idx.p, vind.p,
+                         vw.p, keys.p, vals.p);
+      LG_LAUNCH_CHECK();
+      radix_sort_pairs(keys.p, vals.p, m, kb, s);
+      int64_t nu = segmented_canon_sum(keys.p, vals.p, m, uk.p, us.p, s);
+      hipLaunchKernelGGL(k_scatter_sums, dim3(grid_for(nu, 256, 8192)), dim3(256), 0, s, nu, uk.p, us.p,
+                         nv.p);
+      LG_LAUNCH_CHECK();
+      LG_HIP(hipStreamSynchronize(s));
+    }
+    hipLaunchKernelGGL(k_add_votes, dim3(g1(n)), dim3(256), 0, s, (int)n, nv.p, votes.p);
+    LG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mark, dim3(g1(n)), dim3(256), 0, s, (int)n, iter == 0 ? 1 : 0, state.p, ind.p,
+                       vst.p, vind.p, vf.p, votes.p, cs, (double)o.votes_to_seed,
+                       (double)o.max_agg_size, nstate.p, nind.p);
+    LG_LAUNCH_CHECK();
+    std::swap(state.p, nstate.p);
+    std::swap(ind.p, nind.p);
+  }
+  DevBuf<int32_t> root(n), cid(n);
+  fill_i32(root.p, 0, n, s);
+  hipLaunchKernelGGL(k_mark_roots, dim3(g1(n)), dim3(256), 0, s, (int)n, ind.p, root.p);
+  LG_LAUNCH_CHECK();
+  int64_t nc = exclusive_scan_i32(root.p, cid.p, n, s);
+  agg.alloc(n);
+  hipLaunchKernelGGL(k_agg_ids, dim3(g1(n)), dim3(256), 0, s, (int)n, ind.p, cid.p, agg.p);
+  LG_LAUNCH_CHECK();
+  LG_HIP(hipStreamSynchronize(s));
+  return nc;
+}
+
+void col_sums_of(const DCsr &A, const double *S, DevBuf<double> &cs, hipStream_t s) {
+  const int64_t n = A.n;
+  DevBuf<int32_t> tpos(A.nnz), bad(1);
+  LG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
+  run_rows(n, A.rp.p, OpTpos{{}, A.rp.p, A.ci.p, tpos.p, bad.p}, s);
+  int32_t hb = 0;
+  to_host(&hb, bad.p, 1, s);
+  LG_HIP(hipStreamSynchronize(s));
+  LG_CHECK(hb == 0, LIGMG_ERR_INVALID, "matrix pattern is not structurally symmetric");
+  cs.alloc(n);
+  run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S, cs.p}, s);
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------------------
+// Aggregation level (aggregation.hpp:838-964, multigrid.hpp:151-159)
+// ----------------------------------------------------------------------------
+void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLevel &L, DCsr &coarse,
+                       hipStream_t s) {
+  const int64_t n = A.n;
+  L.kind = LIGMG_LEVEL_AGG;
+  L.n = n;
+  DevBuf<double> cs;
+  compute_strength(A, o, level_idx, L.strength, cs, s);
+  L.nc = run_voting(A, L.strength.p, cs.p, o, L.agg, L.ind, L.state, s);
+  const int64_t nc = L.nc;
+  // Galerkin R A P
+  {
+    DevBuf<uint64_t> keys(A.nnz), vals(A.nnz);
+    const int cb = std::max(1, bits_for((uint64_t)nc));
+    run_rows(n, A.rp.p, OpGalAgg{{}, A.ci.p, L.agg.p, A.v.p, cb, keys.p, vals.p}, s);
+    contract_to_csr(keys, vals, A.nnz, nc, nc, coarse, s);
+  }
+  // member lists (restriction operator, unit values)
+  {
+    DevBuf<uint64_t> keys(n), vals(n);
+    hipLaunchKernelGGL(k_member_pairs, dim3(g1(n)), dim3(256), 0, s, (int)n, L.agg.p, keys.p, vals.p);
+    LG_LAUNCH_CHECK();
+    radix_sort_pairs(keys.p, vals.p, n, std::max(1, bits_for((uint64_t)nc)), s);
+    L.M.n = nc;
+    L.M.ncols = n;
+    L.M.nnz = n;
+    L.M.rp.alloc(nc + 1);
+    L.M.ci.alloc(n);
+    fill_i32(L.M.rp.p, 0, nc + 1, s);
+    hipLaunchKernelGGL(k_members_unpack, dim3(g1(n)), dim3(256), 0, s, (int)n, keys.p, vals.p, L.M.rp.p,
+                       L.M.ci.p);
+    LG_LAUNCH_CHECK();
+    exclusive_scan_i32(L.M.rp.p, L.M.rp.p, nc + 1, s);
+  }
+}
+
+// Inverse diagonal of A (0 where the diagonal is missing or zero).
+void setup_dinv(const DCsr &A, DLevel &L, bool fp32, hipStream_t s) {
+  const int64_t n = A.n;
+  DevBuf<double> diag(n);
+  DevBuf<uint8_t> hasd(n);
+  hipLaunchKernelGGL(k_diag, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p, diag.p, hasd.p);
+  LG_LAUNCH_CHECK();
+  L.dinv.alloc(n);
+  if (fp32) L.dinv32.alloc(n);
+  hipLaunchKernelGGL(k_dinv, dim3(g1(n)), dim3(256), 0, s, (int)n, diag.p, hasd.p, L.dinv.p,
+                     fp32 ? L.dinv32.p : nullptr);
+  LG_LAUNCH_CHECK();
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------------------
+// lambda_max(D^-1 A) by Jacobi-preconditioned CG + Lanczos tridiagonal
+// (the CG eigen-estimator of the reference's Chebyshev, smoothing.hpp:158-165)
+// ----------------------------------------------------------------------------
+namespace {
+__global__ void k_eig_init(int n, uint64_t seed, uint64_t stream, double *r) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = ligmg_rng::u01(seed, stream, i);
+}
+__global__ void k_axpy_dev(int n, const double *alpha, double sign, const double *x, double *y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += sign * alpha[0] * x[i];
+}
+__global__ void k_mul(int n, const double *a, const double *b, double *c) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = a[i] * b[i];
+}
+__global__ void k_xpby(int n, const double *x, double beta, double *p) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = x[i] + beta * p[i];
+}
+}  // namespace
+
+double tridiag_max_eig(const std::vector<double> &d, const std::vector<double> &e) {
+  const int m = (int)d.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < m; i++) {
+    double r = (i > 0 ? std::fabs(e[i - 1]) : 0) + (i + 1 < m ? std::fabs(e[i]) : 0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  auto below = [&](double x) {
+    int c = 0;
+    double q = d[0] - x;
+    if (q < 0) c++;
+    for (int i = 1; i < m; i++) {
+      double qq = (q == 0) ? 1e-300 : q;
+      q = d[i] - x - e[i - 1] * e[i - 1] / qq;
+      if (q < 0) c++;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200; it++) {
+    double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (below(mid) == m) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
+double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_idx, int iters,
+                    hipStream_t s) {
+  const int64_t n = A.n;
+  if (n == 0) return 1.0;
+  RedPlan R;
+  R.build(n, {0, n}, s);
+  DevBuf<double> r(n), z(n), p(n), q(n), sc(4);
+  hipLaunchKernelGGL(k_eig_init, dim3(g1(n)), dim3(256), 0, s, (int)n, seed,
+                     ligmg_rng::STREAM_EIG + (uint64_t)level_idx, r.p);
+  LG_LAUNCH_CHECK();
+  R.sums(r.p, sc.p, s);
+  op_sub_block_mean(r.p, R, sc.p, s);
+  hipLaunchKernelGGL(k_mul, dim3(g1(n)), dim3(256), 0, s, (int)n, dinv, r.p, z.p);
+  d2d(p.p, z.p, n, s);
+  auto dot = [&](const double *a, const double *b) {
+    const double *A_[1] = {a}, *B_[1] = {b};
+    R.dots(1, A_, B_, sc.p, s);
+    double h;
+    to_host(&h, sc.p, 1, s);
+    LG_HIP(hipStreamSynchronize(s));
+    return h;
+  };
+  double gamma = dot(r.p, z.p);
+  std::vector<double> al, be;
+  for (int it = 0; it < iters && gamma > 0; it++) {
+    op_spmv<double>(A, p.p, q.p, s);
+    double delta = dot(p.p, q.p);
+    if (!(delta > 0)) break;
+    double alpha = gamma / delta;
+    to_device(sc.p + 1, &alpha, 1, s);
+    hipLaunchKernelGGL(k_axpy_dev, dim3(g1(n)), dim3(256), 0, s, (int)n, sc.p + 1, -1.0, q.p, r.p);
+    hipLaunchKernelGGL(k_mul, dim3(g1(n)), dim3(256), 0, s, (int)n, dinv, r.p, z.p);
+    double gn = dot(r.p, z.p);
+    double beta = gn / gamma;
+    al.push_back(alpha);
+    be.push_back(beta);
+    hipLaunchKernelGGL(k_xpby, dim3(g1(n)), dim3(256), 0, s, (int)n, z.p, beta, p.p);
+    gamma = gn;
+  }
+  LG_HIP(hipStreamSynchronize(s));
+  const int m = (int)al.size();
+  if (m == 0) return 1.0;
+  std::vector<double> d(m), e(m > 1 ? m - 1 : 0);
+  for (int j = 0; j < m; j++) {
+    d[j] = 1 / al[j] + (j > 0 ? be[j - 1] / al[j - 1] : 0);
+    if (j + 1 < m) e[j] = std::sqrt(be[j]) / al[j];
+  }
+  return tridiag_max_eig(d, e);
+}
+
+// ----------------------------------------------------------------------------
+// Coarse level: dense pseudo-inverse per connected component,
+// L_c^+ = (L_c + 1 1^T / n_c)^-1 - 1 1^T / n_c  (host, FP64 Cholesky).
+// ----------------------------------------------------------------------------
+std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,
+                                    const std::vector<int32_t> &ci, const std::vector<double> &v) {
+  std::vector<int32_t> comp(n, -1);
+  std::vector<int64_t> csize;
+  std::vector<int64_t> stack;
+  for (int64_t s0 = 0; s0 < n; s0++) {
+    if (comp[s0] >= 0) continue;
+    int32_t cid = (int32_t)csize.size();
+    csize.push_back(0);
+    stack.assign(1, s0);
+    comp[s0] = cid;
+    while (!stack.empty()) {
+      int64_t u = stack.back();
+      stack.pop_back();
+      csize[cid]++;
+      for (int32_t k = rp[u]; k < rp[u + 1]; k++) {
+        int32_t w = ci[k];
+        if (v[k] != 0 && comp[w] < 0) {
+          comp[w] = cid;
+          stack.push_back(w);
+        }
+      }
+    }
+  }
+  std::vector<double> M((size_t)n * n, 0.0);
+  for (int64_t i = 0; i < n; i++)
+    for (int32_t k = rp[i]; k < rp[i + 1]; k++) M[i * n + ci[k]] += v[k];
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = 0; j < n; j++)
+      if (comp[i] == comp[j]) M[i * n + j] += 1.0 / (double)csize[comp[i]];
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = i + 1; j < n; j++) {
+      double a = 0.5 * (M[i * n + j] + M[j * n + i]);
+      M[i * n + j] = M[j * n + i] = a;
+    }
+  // in-place Cholesky (lower), then inverse from G^-T G^-1
+  std::vector<double> &G = M;
+  for (int64_t j = 0; j < n; j++) {
+    double sjj = G[j * n + j];
+    for (int64_t k = 0; k < j; k++) sjj -= G[j * n + k] * G[j * n + k];
+    LG_CHECK(sjj > 0, LIGMG_ERR_NUMERIC, "coarse matrix is not positive semidefinite");
+    double g = std::sqrt(sjj);
+    G[j * n + j] = g;
+    for (int64_t i = j + 1; i < n; i++) {
+      double t = G[i * n + j];
+      for (int64_t k = 0; k < j; k++) t -= G[i * n + k] * G[j * n + k];
+      G[i * n + j] = t / g;
+    }
+  }
+  // W = G^-1 (lower triangular)
+  std::vector<double> W((size_t)n * n, 0.0);
+  for (int64_t c = 0; c < n; c++) {
+    for (int64_t i = c; i < n; i++) {
+      double t = (i == c) ? 1.0 : 0.0;
+      for (int64_t k = c; k < i; k++) t -= G[i * n + k] * W[k * n + c];
+      W[i * n + c] = t / G[i * n + i];
+    }
+  }
+  // inv = W^T W
+  std::vector<double> inv((size_t)n * n, 0.0);
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = 0; j <= i; j++) {
+      double t = 0;
+      for (int64_t k = i; k < n; k++) t += W[k * n + i] * W[k * n + j];
+      inv[i * n + j] = inv[j * n + i] = t;
+    }
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = 0; j < n; j++)
+      if (comp[i] == comp[j]) inv[i * n + j] -= 1.0 / (double)csize[comp[i]];
+  return inv;
+}
+
+}  // namespace lg

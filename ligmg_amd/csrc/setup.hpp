@@ -1,0 +1,33 @@
+// Device setup entry points (setup.hip).
+#pragma once
+#include "hierarchy.hpp"
+
+namespace lg {
+
+// Contract (key = row << cb | col, value) pairs into CSR (destroys the inputs).
+void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, int64_t nrows,
+                     int64_t ncols, DCsr &out, hipStream_t s);
+
+// Elimination level from A; false if rejected (kept > elim_factor n).
+bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &coarse, hipStream_t s);
+
+// Strength (per stored entry, 0 where pruned) and column sums.
+void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevBuf<double> &S,
+                      DevBuf<double> &cs, hipStream_t s);
+// Column sums of a strength matrix on A's pattern.
+void col_sums_of(const DCsr &A, const double *S, DevBuf<double> &cs, hipStream_t s);
+// Voting aggregation with given strengths; returns the number of aggregates.
+int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg_options &o,
+                   DevBuf<int32_t> &agg, DevBuf<int32_t> &ind, DevBuf<int32_t> &state, hipStream_t s);
+
+// Aggregation level from A (strength, voting, Galerkin, member lists).
+void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLevel &L, DCsr &coarse,
+                       hipStream_t s);
+
+void setup_dinv(const DCsr &A, DLevel &L, bool fp32, hipStream_t s);
+double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_idx, int iters,
+                    hipStream_t s);
+std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,
+                                    const std::vector<int32_t> &ci, const std::vector<double> &v);
+
+}  // namespace lg

@@ -1,0 +1,439 @@
+// Solve-phase kernels: row-block SpMV family with fused epilogues, transfer
+// operators, dense coarse apply, deterministic blocked reductions.
+#include <algorithm>
+
+#include "ops.hpp"
+
+namespace lg {
+
+namespace {
+
+// ------------------------------------------------------------ value access
+template <class T>
+struct ValOne {
+  __device__ __forceinline__ T operator()(int64_t) const { return (T)1; }
+};
+template <class T>
+struct ValArr {
+  const T *v;
+  __device__ __forceinline__ T operator()(int64_t k) const { return v[k]; }
+};
+
+// --------------------------------------------------------------- epilogues
+template <class T>
+struct EpiStore {
+  T *y;
+  __device__ __forceinline__ void operator()(int64_t r, T s) const { y[r] = s; }
+};
+template <class T>
+struct EpiResid {
+  const T *b;
+  T *r;
+  __device__ __forceinline__ void operator()(int64_t i, T s) const { r[i] = b[i] - s; }
+};
+template <class T>
+struct EpiCheb {
+  const T *dinv, *b, *xc, *xp;
+  T *xn;
+  T alpha, beta;
+  __device__ __forceinline__ void operator()(int64_t i, T s) const {
+    T c = xc[i];
+    T d = xp ? alpha * (c - xp[i]) : (T)0;
+    xn[i] = c + d + beta * dinv[i] * (b[i] - s);
+  }
+};
+template <class T>
+struct EpiInterp {
+  const T *q, *b;
+  T *x;
+  __device__ __forceinline__ void operator()(int64_t i, T s) const { x[i] = s + q[i] * b[i]; }
+};
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T block_sum(T v) {
+  __shared__ T ws[SPMV_BLOCK / WAVE];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  __syncthreads();
+  if (lane == 0) ws[w] = v;
+  __syncthreads();
+  T s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < SPMV_BLOCK / WAVE; i++) s += ws[i];
+  return s;
+}
+
+// One workgroup per unit.  Complete-row units stage every product of the
+// unit in LDS with coalesced loads of colind/values (the x gathers are the
+// only scattered accesses), then reduce rows from LDS: one thread per row
+// when rows average at most 8 entries, one wavefront per row otherwise.
+// Long-row chunks write a partial sum that k_spmv_long combines.
+template <class T, class Val, class Epi>
+__global__ void __launch_bounds__(SPMV_BLOCK)
+    k_spmv_units(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, Val val,
+                 const T *__restrict__ x, const int32_t *__restrict__ urow0,
+                 const int32_t *__restrict__ unrows, const int32_t *__restrict__ ukoff,
+                 T *__restrict__ parts, Epi epi) {
+  __shared__ T prod[UNIT_NNZ];
+  const int u = blockIdx.x;
+  const int row0 = urow0[u], nr = unrows[u], k0 = ukoff[u];
+  if (nr > 0) {
+    const int cnt = rp[row0 + nr] - k0;
+    for (int t = threadIdx.x; t < cnt; t += SPMV_BLOCK) {
+      const int k = k0 + t;
+      prod[t] = val(k) * x[ci[k]];
+    }
+    __syncthreads();
+    if (cnt <= 8 * nr) {
+      for (int r = threadIdx.x; r < nr; r += SPMV_BLOCK) {
+        const int a = rp[row0 + r] - k0, b = rp[row0 + r + 1] - k0;
+        T s = 0;
+        for (int t = a; t < b; t++) s += prod[t];
+        epi(row0 + r, s);
+      }
+    } else {
+      const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+      for (int r = w; r < nr; r += SPMV_BLOCK / WAVE) {
+        const int a = rp[row0 + r] - k0, b = rp[row0 + r + 1] - k0;
+        T s = 0;
+        for (int t = a + lane; t < b; t += WAVE) s += prod[t];
+        s = wave_sum(s);
+        if (lane == 0) epi(row0 + r, s);
+      }
+    }
+  } else {
+    const int row = row0;
+    const int kend = min(k0 + UNIT_NNZ, rp[row + 1]);
+    T s = 0;
+    for (int k = k0 + threadIdx.x; k < kend; k += SPMV_BLOCK) s += val(k) * x[ci[k]];
+    s = block_sum(s);
+    if (threadIdx.x == 0) parts[-nr] = s;
+  }
+}
+
+template <class T, class Epi>
+__global__ void k_spmv_long(const int32_t *lrow, const int32_t *lpart, int64_t nlong,
+                            const T *parts, Epi epi) {
+  int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= nlong) return;
+  T s = 0;
+  for (int p = lpart[j]; p < lpart[j + 1]; p++) s += parts[p];
+  epi(lrow[j], s);
+}
+
+template <class T, class Val, class Epi>
+void launch_spmv(const DCsr &A, Val val, const T *x, Epi epi, hipStream_t s) {
+  if (A.U.nunits == 0) return;
+  T *parts = (T *)A.U.parts.p;
+  hipLaunchKernelGGL((k_spmv_units<T, Val, Epi>), dim3((unsigned)A.U.nunits), dim3(SPMV_BLOCK), 0, s,
+                     A.rp.p, A.ci.p, val, x, A.U.row0.p, A.U.nrows.p, A.U.koff.p, parts, epi);
+  LG_LAUNCH_CHECK();
+  if (A.U.nlong) {
+    hipLaunchKernelGGL((k_spmv_long<T, Epi>), dim3(grid_for(A.U.nlong, 64)), dim3(64), 0, s, A.U.lrow.p,
+                       A.U.lpart.p, A.U.nlong, (const T *)parts, epi);
+    LG_LAUNCH_CHECK();
+  }
+}
+
+template <class T>
+const T *vals_of(const DCsr &A);
+template <>
+const double *vals_of<double>(const DCsr &A) {
+  return A.v.p;
+}
+template <>
+const float *vals_of<float>(const DCsr &A) {
+  return A.v32.p;
+}
+
+template <class T, class Epi>
+void dispatch_spmv(const DCsr &A, const T *x, Epi epi, hipStream_t s) {
+  const T *v = vals_of<T>(A);
+  if (v) launch_spmv<T>(A, ValArr<T>{v}, x, epi, s);
+  else launch_spmv<T>(A, ValOne<T>{}, x, epi, s);
+}
+
+// ------------------------------------------------------------- elementwise
+template <class T>
+__global__ void k_cheb_zero(const T *dinv, const T *b, T *xn, T beta, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    xn[i] = beta * dinv[i] * b[i];
+}
+
+template <class T>
+__global__ void k_prolong_add(const int32_t *agg, int64_t n, const T *xc, T *x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] += xc[agg[i]];
+}
+
+__global__ void k_d2f(const double *a, float *b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+__global__ void k_f2d(const float *a, double *b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+
+// y = Pinv x, one wavefront per row of the dense row-major matrix.
+template <class T>
+__global__ void k_dense_apply(const double *__restrict__ Pm, int64_t n, const T *__restrict__ x,
+                              T *__restrict__ y) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (row >= n) return;
+  double s = 0;
+  for (int64_t j = lane; j < n; j += WAVE) s += Pm[row * n + j] * (double)x[j];
+  s = wave_sum(s);
+  if (lane == 0) y[row] = (T)s;
+}
+
+// ---------------------------------------------------------------- reductions
+constexpr int RED_CHUNK = 16384;
+
+template <class T>
+__global__ void __launch_bounds__(256) k_red_stage1(int np, const T *a0, const T *b0, const T *a1,
+                                                    const T *b1, const T *a2, const T *b2,
+                                                    const T *a3, const T *b3, const int64_t *lo,
+                                                    const int64_t *hi, double *part) {
+  const int s = blockIdx.x;
+  const int64_t l = lo[s], h = hi[s];
+  double acc[4] = {0, 0, 0, 0};
+  const T *A[4] = {a0, a1, a2, a3};
+  const T *B[4] = {b0, b1, b2, b3};
+  for (int64_t i = l + threadIdx.x; i < h; i += 256) {
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (j < np) acc[j] += (double)A[j][i] * (B[j] ? (double)B[j][i] : 1.0);
+  }
+  __shared__ double ws[4][4];
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    double v = wave_sum(acc[j]);
+    if (lane == 0) ws[j][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 && (int)threadIdx.x < np) {
+    int j = threadIdx.x;
+    part[s * 4 + j] = ((ws[j][0] + ws[j][1]) + ws[j][2]) + ws[j][3];
+  }
+}
+
+__global__ void k_red_stage2(int np, int nb, const int32_t *bseg, const double *part, double *out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  for (int j = 0; j < np; j++) {
+    double s = 0;
+    for (int g = bseg[c]; g < bseg[c + 1]; g++) s += part[g * 4 + j];
+    out[c * np + j] = s;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_sub_mean(T *a, const int64_t *lo, const int64_t *hi,
+                                                  const int32_t *segblk, const int64_t *bptr,
+                                                  const double *sums) {
+  const int s = blockIdx.x;
+  const int c = segblk[s];
+  const T m = (T)(sums[c] / (double)(bptr[c + 1] - bptr[c]));
+  for (int64_t i = lo[s] + threadIdx.x; i < hi[s]; i += 256) a[i] -= m;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+
+void build_units(DCsr &A, hipStream_t s) {
+  const int64_t n = A.n;
+  std::vector<int32_t> rp = download(A.rp.p, n + 1, s);
+  std::vector<int32_t> row0, nrows, koff, lrow, lpart;
+  int64_t r = 0, parts = 0;
+  while (r < n) {
+    int64_t len = rp[r + 1] - rp[r];
+    if (len > UNIT_NNZ) {
+      lrow.push_back((int32_t)r);
+      lpart.push_back((int32_t)parts);
+      for (int64_t k = rp[r]; k < rp[r + 1]; k += UNIT_NNZ) {
+        row0.push_back((int32_t)r);
+        nrows.push_back(-(int32_t)parts);
+        koff.push_back((int32_t)k);
+        parts++;
+      }
+      r++;
+      continue;
+    }
+    int64_t r1 = r, acc = 0;
+    while (r1 < n && (rp[r1 + 1] - rp[r1]) <= UNIT_NNZ && acc + (rp[r1 + 1] - rp[r1]) <= UNIT_NNZ) {
+      acc += rp[r1 + 1] - rp[r1];
+      r1++;
+    }
+    row0.push_back((int32_t)r);
+    nrows.push_back((int32_t)(r1 - r));
+    koff.push_back(rp[r]);
+    r = r1;
+  }
+  lpart.push_back((int32_t)parts);
+  Units &U = A.U;
+  U.nunits = (int64_t)row0.size();
+  U.nlong = (int64_t)lrow.size();
+  U.nparts = parts;
+  U.row0.alloc(U.nunits);
+  U.nrows.alloc(U.nunits);
+  U.koff.alloc(U.nunits);
+  to_device(U.row0.p, row0.data(), U.nunits, s);
+  to_device(U.nrows.p, nrows.data(), U.nunits, s);
+  to_device(U.koff.p, koff.data(), U.nunits, s);
+  U.lrow.alloc(std::max<int64_t>(U.nlong, 1));
+  U.lpart.alloc(U.nlong + 1);
+  to_device(U.lrow.p, lrow.data(), U.nlong, s);
+  to_device(U.lpart.p, lpart.data(), U.nlong + 1, s);
+  U.parts.alloc(std::max<int64_t>(parts, 1));
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+void make_v32(DCsr &A, hipStream_t s) {
+  if (!A.v.p) return;
+  A.v32.alloc(A.nnz);
+  op_d2f(A.v.p, A.v32.p, A.nnz, s);
+}
+
+template <class T>
+void op_spmv(const DCsr &A, const T *x, T *y, hipStream_t s) {
+  dispatch_spmv<T>(A, x, EpiStore<T>{y}, s);
+}
+
+template <class T>
+void op_residual(const DCsr &A, const T *x, const T *b, T *r, hipStream_t s) {
+  dispatch_spmv<T>(A, x, EpiResid<T>{b, r}, s);
+}
+
+template <class T>
+void op_cheb_step(const DCsr &A, const T *dinv, const T *b, const T *xc, const T *xp, T *xn,
+                  double alpha, double beta, hipStream_t s) {
+  if (!xc) {
+    hipLaunchKernelGGL(k_cheb_zero<T>, dim3(grid_for(A.n, 256, 8192)), dim3(256), 0, s, dinv, b, xn,
+                       (T)beta, A.n);
+    LG_LAUNCH_CHECK();
+    return;
+  }
+  dispatch_spmv<T>(A, xc, EpiCheb<T>{dinv, b, xc, xp, xn, (T)alpha, (T)beta}, s);
+}
+
+template <class T>
+void op_interp(const DCsr &P, const T *q, const T *xc, const T *b, T *x, hipStream_t s) {
+  dispatch_spmv<T>(P, xc, EpiInterp<T>{q, b, x}, s);
+}
+
+template <class T>
+void op_prolong_add(const int32_t *agg, int64_t n, const T *xc, T *x, hipStream_t s) {
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, agg, n, xc, x);
+  LG_LAUNCH_CHECK();
+}
+
+template <class T>
+void op_dense_apply(const double *Pm, int64_t n, const T *x, T *y, hipStream_t s) {
+  hipLaunchKernelGGL(k_dense_apply<T>, dim3(grid_for(n * WAVE, 256)), dim3(256), 0, s, Pm, n, x, y);
+  LG_LAUNCH_CHECK();
+}
+
+void op_d2f(const double *a, float *b, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_d2f, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, a, b, n);
+  LG_LAUNCH_CHECK();
+}
+void op_f2d(const float *a, double *b, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_f2d, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, a, b, n);
+  LG_LAUNCH_CHECK();
+}
+
+#define LG_INST(T)                                                                                \
+  template void op_spmv<T>(const DCsr &, const T *, T *, hipStream_t);                            \
+  template void op_residual<T>(const DCsr &, const T *, const T *, T *, hipStream_t);             \
+  template void op_cheb_step<T>(const DCsr &, const T *, const T *, const T *, const T *, T *,    \
+                                double, double, hipStream_t);                                     \
+  template void op_interp<T>(const DCsr &, const T *, const T *, const T *, T *, hipStream_t);    \
+  template void op_prolong_add<T>(const int32_t *, int64_t, const T *, T *, hipStream_t);         \
+  template void op_dense_apply<T>(const double *, int64_t, const T *, T *, hipStream_t);
+LG_INST(double)
+LG_INST(float)
+#undef LG_INST
+
+void RedPlan::build(int64_t n_, const std::vector<int64_t> &blocks, hipStream_t s) {
+  n = n_;
+  hbptr = blocks;
+  if (hbptr.size() < 2) hbptr = {0, n};
+  nb = (int)hbptr.size() - 1;
+  std::vector<int64_t> lo, hi;
+  std::vector<int32_t> bs(nb + 1, 0), segblk;
+  for (int c = 0; c < nb; c++) {
+    bs[c] = (int32_t)lo.size();
+    for (int64_t a = hbptr[c]; a < hbptr[c + 1]; a += RED_CHUNK) {
+      lo.push_back(a);
+      hi.push_back(std::min<int64_t>(a + RED_CHUNK, hbptr[c + 1]));
+      segblk.push_back(c);
+    }
+  }
+  bs[nb] = (int32_t)lo.size();
+  nseg = (int)lo.size();
+  seg_lo.alloc(std::max(nseg, 1));
+  seg_hi.alloc(std::max(nseg, 1));
+  bseg.alloc(nb + 1 + std::max(nseg, 1));  // bseg then segment->block map
+  bptr.alloc(nb + 1);
+  part.alloc(std::max(nseg, 1) * 4);
+  to_device(seg_lo.p, lo.data(), nseg, s);
+  to_device(seg_hi.p, hi.data(), nseg, s);
+  to_device(bseg.p, bs.data(), nb + 1, s);
+  to_device(bseg.p + nb + 1, segblk.data(), nseg, s);
+  to_device(bptr.p, hbptr.data(), nb + 1, s);
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+template <class T>
+void RedPlan::dots(int np, const T *const *a, const T *const *b, double *out, hipStream_t s) const {
+  if (nseg == 0) return;
+  const T *A[4] = {nullptr, nullptr, nullptr, nullptr}, *B[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int j = 0; j < np; j++) {
+    A[j] = a[j];
+    B[j] = b[j];
+  }
+  hipLaunchKernelGGL(k_red_stage1<T>, dim3(nseg), dim3(256), 0, s, np, A[0], B[0], A[1], B[1], A[2],
+                     B[2], A[3], B[3], seg_lo.p, seg_hi.p, part.p);
+  LG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_red_stage2, dim3((nb + 63) / 64), dim3(64), 0, s, np, nb, bseg.p, part.p, out);
+  LG_LAUNCH_CHECK();
+}
+
+template <class T>
+void RedPlan::sums(const T *a, double *out, hipStream_t s) const {
+  const T *A[1] = {a}, *B[1] = {nullptr};
+  dots<T>(1, A, B, out, s);
+}
+
+template <class T>
+void op_sub_block_mean(T *a, const RedPlan &P, const double *sums, hipStream_t s) {
+  if (P.nseg == 0) return;
+  hipLaunchKernelGGL(k_sub_mean<T>, dim3(P.nseg), dim3(256), 0, s, a, P.seg_lo.p, P.seg_hi.p,
+                     P.bseg.p + P.nb + 1, P.bptr.p, sums);
+  LG_LAUNCH_CHECK();
+}
+
+template void RedPlan::dots<double>(int, const double *const *, const double *const *, double *,
+                                    hipStream_t) const;
+template void RedPlan::dots<float>(int, const float *const *, const float *const *, double *,
+                                   hipStream_t) const;
+template void RedPlan::sums<double>(const double *, double *, hipStream_t) const;
+template void RedPlan::sums<float>(const float *, double *, hipStream_t) const;
+template void op_sub_block_mean<double>(double *, const RedPlan &, const double *, hipStream_t);
+template void op_sub_block_mean<float>(float *, const RedPlan &, const double *, hipStream_t);
+
+}  // namespace lg

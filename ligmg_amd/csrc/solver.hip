@@ -1,0 +1,985 @@
+// Host orchestration of the device hierarchy and the C ABI
+// (include/ligmg_amd.h).
+//
+//   ligmg_create  <- ligmg::Solver(A, opts)        ligmg.hpp:17-28
+//                     make_hierarchy               multigrid.hpp:161-263
+//   ligmg_solve   <- Solver::solve(b)              ligmg.hpp:31-33
+//                     ligmg::solve                 multigrid.hpp:315-370
+//   cycles        <- vcycle / kcycle / do_level    multigrid.hpp:86-130,
+//                     aggregation.hpp:209-241, elimination.hpp:33-52
+//   Krylov        <- petsc_fcg (PETSc KSPCG/FCG)   cg.hpp:18-116
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+
+#include "../../include/ligmg_amd.h"
+#include "ops.hpp"
+#include "prims.hpp"
+#include "setup.hpp"
+
+namespace lg {
+
+namespace {
+
+thread_local std::string g_last_error;
+
+// ------------------------------------------------------------ Krylov kernels
+enum { SC_BN2 = 0, SC_RN2, SC_GAMMA, SC_DELTA, SC_T1, SC_ALPHA, SC_BETA, SC_ACTIVE, SC_GN, SC_N };
+
+// per-block scalar logic of CG / FCG (one thread per block)
+__global__ void k_kr_alpha(int nb, int fcg, double *sc) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  double *S = sc + c * SC_N;
+  double num = fcg ? S[SC_T1] : S[SC_GAMMA];
+  S[SC_ALPHA] = (S[SC_ACTIVE] != 0 && S[SC_DELTA] != 0) ? num / S[SC_DELTA] : 0.0;
+}
+__global__ void k_kr_check(int nb, double rtol, double *sc) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  double *S = sc + c * SC_N;
+  if (S[SC_ACTIVE] != 0 && !(sqrt(S[SC_RN2]) > rtol * sqrt(S[SC_BN2]))) S[SC_ACTIVE] = 0;
+}
+__global__ void k_kr_beta(int nb, int fcg, double *sc) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  double *S = sc + c * SC_N;
+  if (fcg) {
+    S[SC_BETA] = (S[SC_ACTIVE] != 0 && S[SC_DELTA] != 0) ? -S[SC_T1] / S[SC_DELTA] : 0.0;
+  } else {
+    S[SC_BETA] = (S[SC_ACTIVE] != 0 && S[SC_GAMMA] != 0) ? S[SC_GN] / S[SC_GAMMA] : 0.0;
+    S[SC_GAMMA] = S[SC_GN];
+  }
+}
+__global__ void k_kr_init(int nb, double *sc) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  for (int j = 0; j < SC_N; j++) sc[c * SC_N + j] = 0;
+  sc[c * SC_N + SC_ACTIVE] = 1;
+}
+// scatter a packed dot result out[c*np + j] into scalar slots
+__global__ void k_kr_store(int nb, int np, const double *out, int s0, int s1, double *sc) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  sc[c * SC_N + s0] = out[c * np];
+  if (np > 1) sc[c * SC_N + s1] = out[c * np + 1];
+}
+
+// x += alpha_c p ; r -= alpha_c q   (segment-per-workgroup, block-aware)
+template <class T>
+__global__ void __launch_bounds__(256) k_kr_update(T *x, T *r, const T *p, const T *q,
+                                                   const int64_t *lo, const int64_t *hi,
+                                                   const int32_t *segblk, const double *sc) {
+  const int s = blockIdx.x;
+  const double a = sc[segblk[s] * SC_N + SC_ALPHA];
+  if (a == 0.0) return;
+  const T at = (T)a;
+  for (int64_t i = lo[s] + threadIdx.x; i < hi[s]; i += 256) {
+    x[i] += at * p[i];
+    r[i] -= at * q[i];
+  }
+}
+// p = z + beta_c p
+template <class T>
+__global__ void __launch_bounds__(256) k_kr_pupdate(T *p, const T *z, const int64_t *lo,
+                                                    const int64_t *hi, const int32_t *segblk,
+                                                    const double *sc) {
+  const int s = blockIdx.x;
+  const T b = (T)sc[segblk[s] * SC_N + SC_BETA];
+  for (int64_t i = lo[s] + threadIdx.x; i < hi[s]; i += 256) p[i] = z[i] + b * p[i];
+}
+
+template <class T>
+struct KWork {
+  DevBuf<T> r, z, p, q, x;
+  DevBuf<double> sc, out;
+  void alloc(int64_t n, int nb) {
+    r.alloc(n);
+    z.alloc(n);
+    p.alloc(n);
+    q.alloc(n);
+    x.alloc(n);
+    sc.alloc((size_t)nb * SC_N);
+    out.alloc((size_t)nb * 4);
+  }
+};
+
+std::vector<double> cheb_alpha(double lo, double hi, int steps, std::vector<double> &be) {
+  // Saad, Alg. 12.1, three-term form (see the oracle for the derivation)
+  double theta = (hi + lo) / 2, delta = (hi - lo) / 2, sigma = theta / delta;
+  double rho = 1 / sigma;
+  std::vector<double> al(std::max(steps, 0), 0.0);
+  be.assign(std::max(steps, 0), 0.0);
+  if (steps > 0) be[0] = 1 / theta;
+  for (int k = 1; k < steps; k++) {
+    double rn = 1 / (2 * sigma - rho);
+    al[k] = rn * rho;
+    be[k] = 2 * rn / delta;
+    rho = rn;
+  }
+  return al;
+}
+
+}  // namespace
+
+// Per-level work vectors of one precision.
+template <class T>
+struct LevelWork {
+  DevBuf<T> B[3], R, b;
+  KWork<T> kw;
+};
+
+struct Solver {
+  ligmg_options o;
+  hipStream_t s = nullptr;
+  std::vector<DLevel> H;
+  std::vector<RedPlan> red;
+  std::vector<LevelWork<double>> W64;
+  std::vector<LevelWork<float>> W32;
+  std::vector<std::vector<double>> al_pre, be_pre, al_post, be_post;
+  KWork<double> outer;
+  DevBuf<double> b0, sol, tmp0, dsc;
+  double setup_time = 0;
+  bool mixed = false;
+
+  ~Solver() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+
+  template <class T>
+  LevelWork<T> &work(size_t l);
+
+  // ---------------------------------------------------------------- setup
+  void build(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, int mem,
+             const std::vector<int64_t> &blocks0) {
+    LG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    mixed = o.precision == LIGMG_PREC_MIXED;
+    auto t0 = std::chrono::steady_clock::now();
+    DCsr cur;
+    cur.n = n;
+    cur.ncols = n;
+    int32_t nnz32 = 0;
+    if (mem == LIGMG_MEM_HOST) nnz32 = rp[n] - rp[0];
+    else {
+      int32_t ends[2];
+      to_host(&ends[0], rp, 1, s);
+      to_host(&ends[1], rp + n, 1, s);
+      LG_HIP(hipStreamSynchronize(s));
+      nnz32 = ends[1] - ends[0];
+      LG_CHECK(ends[0] == 0, LIGMG_ERR_INVALID, "rowptr[0] must be 0");
+    }
+    cur.nnz = nnz32;
+    cur.rp.alloc(n + 1);
+    cur.ci.alloc(std::max<int64_t>(cur.nnz, 1));
+    cur.v.alloc(std::max<int64_t>(cur.nnz, 1));
+    if (mem == LIGMG_MEM_HOST) {
+      LG_CHECK(rp[0] == 0, LIGMG_ERR_INVALID, "rowptr[0] must be 0");
+      to_device(cur.rp.p, rp, n + 1, s);
+      to_device(cur.ci.p, ci, cur.nnz, s);
+      to_device(cur.v.p, v, cur.nnz, s);
+    } else {
+      d2d(cur.rp.p, rp, n + 1, s);
+      d2d(cur.ci.p, ci, cur.nnz, s);
+      d2d(cur.v.p, v, cur.nnz, s);
+    }
+    LG_HIP(hipStreamSynchronize(s));
+    auto lap = [&](const char *what, size_t l, int64_t nn, int64_t nz) {
+      if (!o.verbose) return;
+      static auto last = std::chrono::steady_clock::now();
+      auto now = std::chrono::steady_clock::now();
+      fprintf(stderr, "[ligmg] level %zu %-6s n=%lld nnz=%lld  %.3f s\n", l, what, (long long)nn,
+              (long long)nz, std::chrono::duration<double>(now - last).count());
+      last = now;
+    };
+    lap("input", 0, cur.n, cur.nnz);
+    // make_hierarchy (multigrid.hpp:191-213)
+    while ((int)H.size() < o.max_levels && (H.empty() || H.back().A.nnz > o.min_size)) {
+      if (o.always_elim) {
+        DLevel L;
+        DCsr c;
+        if (setup_elimination(cur, o, L, c, s)) {
+          L.A = std::move(cur);
+          H.push_back(std::move(L));
+          cur = std::move(c);
+          lap("elim", H.size() - 1, H.back().A.n, H.back().A.nnz);
+        }
+      }
+      DLevel L;
+      DCsr c;
+      setup_aggregation(cur, o, (int)H.size(), L, c, s);
+      L.A = std::move(cur);
+      H.push_back(std::move(L));
+      cur = std::move(c);
+      lap("agg", H.size() - 1, H.back().A.n, H.back().A.nnz);
+    }
+    {
+      DLevel C;
+      C.kind = LIGMG_LEVEL_COARSE;
+      C.n = cur.n;
+      auto hrp = download(cur.rp.p, cur.n + 1, s);
+      auto hci = download(cur.ci.p, cur.nnz, s);
+      auto hv = download(cur.v.p, cur.nnz, s);
+      auto pinv = dense_pinv_host(cur.n, hrp, hci, hv);
+      C.pinv.alloc(std::max<size_t>(pinv.size(), 1));
+      to_device(C.pinv.p, pinv.data(), pinv.size(), s);
+      C.A = std::move(cur);
+      H.push_back(std::move(C));
+      lap("coarse", H.size() - 1, H.back().A.n, H.back().A.nnz);
+    }
+    // blocks per level: level-0 blocks mapped through kept ids / roots
+    H[0].blocks = blocks0;
+    for (size_t l = 0; l + 1 < H.size(); l++) {
+      const DLevel &L = H[l];
+      std::vector<int64_t> nb(L.blocks.size(), 0);
+      std::vector<int32_t> keep;
+      if (L.kind == LIGMG_LEVEL_ELIM) keep = download(L.new_id.p, L.n, s);
+      else {
+        keep = download(L.ind.p, L.n, s);
+        for (int64_t i = 0; i < L.n; i++) keep[i] = (keep[i] == i) ? 1 : -1;
+      }
+      for (size_t c = 0; c + 1 < L.blocks.size(); c++) {
+        int64_t cnt = 0;
+        for (int64_t i = L.blocks[c]; i < L.blocks[c + 1]; i++) cnt += keep[i] >= 0;
+        nb[c + 1] = nb[c] + cnt;
+      }
+      H[l + 1].blocks = nb;
+    }
+    // smoothers, SpMV units, FP32 copies, work vectors
+    red.resize(H.size());
+    W64.resize(H.size());
+    if (mixed) W32.resize(H.size());
+    al_pre.resize(H.size());
+    be_pre.resize(H.size());
+    al_post.resize(H.size());
+    be_post.resize(H.size());
+    for (size_t l = 0; l < H.size(); l++) {
+      DLevel &L = H[l];
+      red[l].build(L.A.n, L.blocks, s);
+      if (L.kind != LIGMG_LEVEL_ELIM || l == 0) build_units(L.A, s);
+      if (L.kind == LIGMG_LEVEL_AGG) {
+        build_units(L.M, s);
+        setup_dinv(L.A, L, mixed, s);
+        if (o.smoother_type == LIGMG_SMOOTHER_CHEBYSHEV) {
+          L.lam = eig_estimate(L.A, L.dinv.p, o.seed, (int)l, o.cheby_eig_iters, s);
+          L.cheb_lo = o.cheby_lower * L.lam;
+          L.cheb_hi = o.cheby_upper * L.lam;
+          al_pre[l] = cheb_alpha(L.cheb_lo, L.cheb_hi, o.pre_smooth, be_pre[l]);
+          al_post[l] = cheb_alpha(L.cheb_lo, L.cheb_hi, o.post_smooth, be_post[l]);
+        }
+      } else if (L.kind == LIGMG_LEVEL_ELIM) {
+        build_units(L.P, s);
+        build_units(L.R, s);
+      }
+      if (mixed) {
+        make_v32(L.A, s);
+        if (L.kind == LIGMG_LEVEL_ELIM) {
+          make_v32(L.P, s);
+          make_v32(L.R, s);
+          L.q32.alloc(L.n);
+          op_d2f(L.q.p, L.q32.p, L.n, s);
+        }
+      }
+      alloc_work<double>(l);
+      if (mixed) alloc_work<float>(l);
+    }
+    const int64_t n0 = H[0].A.n;
+    b0.alloc(n0);
+    sol.alloc(n0);
+    tmp0.alloc(n0);
+    int maxnb = 1;
+    for (auto &r : red) maxnb = std::max(maxnb, r.nb);
+    dsc.alloc((size_t)8 * maxnb);
+    LG_HIP(hipStreamSynchronize(s));
+    lap("solve-prep", H.size(), 0, 0);
+    setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  template <class T>
+  void alloc_work(size_t l) {
+    const DLevel &L = H[l];
+    LevelWork<T> &w = work<T>(l);
+    const int64_t n = L.A.n;
+    for (auto &B : w.B) B.alloc(std::max<int64_t>(n, 1));
+    if (L.kind == LIGMG_LEVEL_AGG) w.R.alloc(n);
+    w.b.alloc(std::max<int64_t>(n, 1));
+    bool need_kw = o.kcycle && l > 0 && L.kind != LIGMG_LEVEL_ELIM;
+    if (need_kw) w.kw.alloc(n, red[l].nb);
+  }
+
+  // ---------------------------------------------------------------- cycles
+  // Applies level l's cycle to b with a zero initial guess and returns the
+  // buffer holding the result.  The buffer sequence depends only on the
+  // options, so replays (and graph captures) see the same pointers.
+  template <class T>
+  const T *cycle(size_t l, const T *b) {
+    DLevel &L = H[l];
+    LevelWork<T> &w = work<T>(l);
+    if (L.kind == LIGMG_LEVEL_COARSE) {
+      op_dense_apply<T>(L.pinv.p, L.n, b, w.B[0].p, s);
+      return w.B[0].p;
+    }
+    if (L.kind == LIGMG_LEVEL_ELIM) {
+      LevelWork<T> &wn = work<T>(l + 1);
+      op_spmv<T>(L.R, b, wn.b.p, s);
+      const T *xc = recurse<T>(l, wn.b.p);
+      op_interp<T>(L.P, qv<T>(L), xc, b, w.B[0].p, s);
+      return w.B[0].p;
+    }
+    // aggregation level
+    const T *dinv = dinvv<T>(L);
+    int cur = smooth_from_zero<T>(L, w, dinv, b, o.pre_smooth, l);
+    LevelWork<T> &wn = work<T>(l + 1);
+    op_residual<T>(L.A, w.B[cur].p, b, w.R.p, s);
+    op_spmv<T>(L.M, w.R.p, wn.b.p, s);
+    const T *xc = recurse<T>(l, wn.b.p);
+    op_prolong_add<T>(L.agg.p, L.n, xc, w.B[cur].p, s);
+    cur = smooth_from<T>(L, w, dinv, b, cur, o.post_smooth, l);
+    if (o.constant_correction) scale_vec<T>(w.B[cur].p, L.n, 4.0 / 3.0);
+    return w.B[cur].p;
+  }
+
+  template <class T>
+  int smooth_from_zero(DLevel &L, LevelWork<T> &w, const T *dinv, const T *b, int m, size_t l) {
+    if (m <= 0) {
+      fill_zero<T>(w.B[0].p, L.n);
+      return 0;
+    }
+    if (o.smoother_type == LIGMG_SMOOTHER_JACOBI) {
+      fill_zero<T>(w.B[0].p, L.n);
+      return jacobi<T>(L, w, dinv, b, 0);
+    }
+    // x_1 = beta_0 D^-1 b in B[1]; x_{k+1} in B[(k+1) % 3]
+    op_cheb_step<T>(L.A, dinv, b, nullptr, nullptr, w.B[1].p, 0.0, be_pre[l][0], s);
+    for (int k = 1; k < m; k++) {
+      const T *xp = (k == 1) ? nullptr : w.B[(k - 1) % 3].p;
+      op_cheb_step<T>(L.A, dinv, b, w.B[k % 3].p, xp, w.B[(k + 1) % 3].p, al_pre[l][k], be_pre[l][k],
+                      s);
+    }
+    return m % 3;
+  }
+
+  template <class T>
+  int smooth_from(DLevel &L, LevelWork<T> &w, const T *dinv, const T *b, int s0, int m, size_t l) {
+    if (m <= 0) return s0;
+    if (o.smoother_type == LIGMG_SMOOTHER_JACOBI) return jacobi<T>(L, w, dinv, b, s0);
+    for (int k = 0; k < m; k++) {
+      const T *xp = (k == 0) ? nullptr : w.B[(s0 + k - 1) % 3].p;
+      op_cheb_step<T>(L.A, dinv, b, w.B[(s0 + k) % 3].p, xp, w.B[(s0 + k + 1) % 3].p, al_post[l][k],
+                      be_post[l][k], s);
+    }
+    return (s0 + m) % 3;
+  }
+
+  // reference jacobi smoother: one damped sweep (smoothing.hpp:78-98, 206-211)
+  template <class T>
+  int jacobi(DLevel &L, LevelWork<T> &w, const T *dinv, const T *b, int s0) {
+    int s1 = (s0 + 1) % 3;
+    op_cheb_step<T>(L.A, dinv, b, w.B[s0].p, nullptr, w.B[s1].p, 0.0, 2.0 / 3.0, s);
+    return s1;
+  }
+
+  template <class T>
+  const T *recurse(size_t l, const T *bc) {
+    const DLevel &N = H[l + 1];
+    if (!o.kcycle || N.kind == LIGMG_LEVEL_ELIM) return cycle<T>(l + 1, bc);
+    // K-cycle: kcycle_iters steps of FCG on the next level (multigrid.hpp:113-126)
+    LevelWork<T> &wn = work<T>(l + 1);
+    KWork<T> &K = wn.kw;
+    fill_zero<T>(K.x.p, N.A.n);
+    krylov<T>(l + 1, N.A, K.x.p, bc, K, o.kcycle_ksp_type, o.tol, o.kcycle_iters, false, nullptr,
+              nullptr);
+    return K.x.p;
+  }
+
+  // ---------------------------------------------------------------- Krylov
+  // CG / FCG on level lvl's matrix with per-block coefficients.  With
+  // host_check the residual norms come back every iteration (outer loop,
+  // history, stopping test); otherwise exactly maxit iterations run with
+  // device-side convergence flags (K-cycle inner solves, no host sync).
+  template <class T>
+  int krylov(size_t lvl, const DCsr &A, T *x, const T *b, KWork<T> &K, int type, double rtol,
+             int maxit, bool host_check, std::vector<double> *hist, int *status,
+             bool outer_prec = false) {
+    const RedPlan &R = red[lvl];
+    const int nb = R.nb;
+    const int64_t n = A.n;
+    const bool fcg = type == LIGMG_KSP_FCG;
+    hipLaunchKernelGGL(k_kr_init, dim3((nb + 63) / 64), dim3(64), 0, s, nb, K.sc.p);
+    // r = b - A x
+    op_residual<T>(A, x, b, K.r.p, s);
+    dot2<T>(R, b, b, K.r.p, K.r.p, K, SC_BN2, SC_RN2);
+    hipLaunchKernelGGL(k_kr_check, dim3((nb + 63) / 64), dim3(64), 0, s, nb, rtol, K.sc.p);
+    std::vector<double> hsc;
+    if (status) *status = LIGMG_CONVERGED;
+    if (host_check) {
+      hsc = download(K.sc.p, (size_t)nb * SC_N, s);
+      if (hist) hist->push_back(total_norm(hsc, nb));
+      if (!any_active(hsc, nb)) return 0;
+    }
+    apply_prec<T>(lvl, K.r.p, K.z.p, outer_prec);
+    d2d(K.p.p, K.z.p, n, s);
+    dot1<T>(R, K.r.p, K.z.p, K, SC_GAMMA);
+    int it = 0;
+    while (it < maxit) {
+      op_spmv<T>(A, K.p.p, K.q.p, s);
+      if (fcg) dot2<T>(R, K.p.p, K.q.p, K.p.p, K.r.p, K, SC_DELTA, SC_T1);
+      else dot1<T>(R, K.p.p, K.q.p, K, SC_DELTA);
+      hipLaunchKernelGGL(k_kr_alpha, dim3((nb + 63) / 64), dim3(64), 0, s, nb, fcg ? 1 : 0, K.sc.p);
+      hipLaunchKernelGGL(k_kr_update<T>, dim3(R.nseg), dim3(256), 0, s, x, K.r.p, K.p.p, K.q.p,
+                         R.seg_lo.p, R.seg_hi.p, R.bseg.p + R.nb + 1, K.sc.p);
+      LG_LAUNCH_CHECK();
+      it++;
+      dot1<T>(R, K.r.p, K.r.p, K, SC_RN2);
+      hipLaunchKernelGGL(k_kr_check, dim3((nb + 63) / 64), dim3(64), 0, s, nb, rtol, K.sc.p);
+      if (host_check) {
+        hsc = download(K.sc.p, (size_t)nb * SC_N, s);
+        double tn = total_norm(hsc, nb);
+        if (hist) hist->push_back(tn);
+        if (tn != tn) {
+          if (status) *status = LIGMG_DIVERGED_NAN;
+          return it;
+        }
+        if (!any_active(hsc, nb)) return it;
+      }
+      if (it >= maxit) break;
+      apply_prec<T>(lvl, K.r.p, K.z.p, outer_prec);
+      if (fcg) dot1<T>(R, K.z.p, K.q.p, K, SC_T1);
+      else dot1<T>(R, K.r.p, K.z.p, K, SC_GN);
+      hipLaunchKernelGGL(k_kr_beta, dim3((nb + 63) / 64), dim3(64), 0, s, nb, fcg ? 1 : 0, K.sc.p);
+      hipLaunchKernelGGL(k_kr_pupdate<T>, dim3(R.nseg), dim3(256), 0, s, K.p.p, K.z.p, R.seg_lo.p,
+                         R.seg_hi.p, R.bseg.p + R.nb + 1, K.sc.p);
+      LG_LAUNCH_CHECK();
+    }
+    if (host_check && status) *status = LIGMG_DIVERGED_ITS;
+    return it;
+  }
+
+  template <class T>
+  void apply_prec(size_t lvl, const T *r, T *z, bool outer_prec) {
+    if constexpr (std::is_same<T, double>::value) {
+      if (outer_prec) {
+        precond_outer(lvl, r, z);
+        return;
+      }
+    }
+    precond<T>(lvl, r, z);
+  }
+
+  int krylov_outer(size_t base, const double *rhs, std::vector<double> &h, int *status) {
+    return krylov<double>(base, H[base].A, outer.x.p, rhs, outer, o.outer_ksp_type, o.tol, o.iters,
+                          true, &h, status, true);
+  }
+
+  static double total_norm(const std::vector<double> &hsc, int nb) {
+    double t = 0;
+    for (int c = 0; c < nb; c++) t += hsc[c * SC_N + SC_RN2];
+    return std::sqrt(t);
+  }
+  static bool any_active(const std::vector<double> &hsc, int nb) {
+    for (int c = 0; c < nb; c++)
+      if (hsc[c * SC_N + SC_ACTIVE] != 0) return true;
+    return false;
+  }
+
+  template <class T>
+  void dot1(const RedPlan &R, const T *a, const T *b, KWork<T> &K, int slot) {
+    const T *A_[1] = {a}, *B_[1] = {b};
+    R.dots<T>(1, A_, B_, K.out.p, s);
+    hipLaunchKernelGGL(k_kr_store, dim3((R.nb + 63) / 64), dim3(64), 0, s, R.nb, 1, K.out.p, slot, slot,
+                       K.sc.p);
+  }
+  template <class T>
+  void dot2(const RedPlan &R, const T *a0, const T *b0, const T *a1, const T *b1, KWork<T> &K, int s0,
+            int s1) {
+    const T *A_[2] = {a0, a1}, *B_[2] = {b0, b1};
+    R.dots<T>(2, A_, B_, K.out.p, s);
+    hipLaunchKernelGGL(k_kr_store, dim3((R.nb + 63) / 64), dim3(64), 0, s, R.nb, 2, K.out.p, s0, s1,
+                       K.sc.p);
+  }
+
+  // z = M r with the constant removed per block (PETSc removes the attached
+  // nullspace from preconditioned vectors; util.hpp:530-534)
+  template <class T>
+  void precond(size_t lvl, const T *r, T *z) {
+    const int64_t n = H[lvl].A.n;
+    const T *c = cycle<T>(lvl, r);
+    d2d(z, c, n, s);
+    const RedPlan &R = red[lvl];
+    R.sums<T>(z, dsc.p, s);
+    op_sub_block_mean<T>(z, R, dsc.p, s);
+  }
+
+  // outer preconditioner in FP64, cycle in FP32 when mixed
+  void precond_outer(size_t lvl, const double *r, double *z) {
+    const int64_t n = H[lvl].A.n;
+    if (mixed) {
+      LevelWork<float> &w = W32[lvl];
+      op_d2f(r, w.b.p, n, s);
+      const float *c = cycle<float>(lvl, w.b.p);
+      op_f2d(c, z, n, s);
+    } else {
+      const double *c = cycle<double>(lvl, r);
+      d2d(z, c, n, s);
+    }
+    const RedPlan &R = red[lvl];
+    R.sums<double>(z, dsc.p, s);
+    op_sub_block_mean<double>(z, R, dsc.p, s);
+  }
+
+  template <class T>
+  void fill_zero(T *p, int64_t n);
+  template <class T>
+  void scale_vec(T *p, int64_t n, double a);
+  template <class T>
+  const T *dinvv(const DLevel &L);
+  template <class T>
+  const T *qv(const DLevel &L);
+
+  // ---------------------------------------------------------------- solve
+  int solve(const double *b, double *x, int mem, ligmg_info *info, double *hist, int cap);
+};
+
+template <>
+LevelWork<double> &Solver::work<double>(size_t l) {
+  return W64[l];
+}
+template <>
+LevelWork<float> &Solver::work<float>(size_t l) {
+  return W32[l];
+}
+template <>
+const double *Solver::dinvv<double>(const DLevel &L) {
+  return L.dinv.p;
+}
+template <>
+const float *Solver::dinvv<float>(const DLevel &L) {
+  return L.dinv32.p;
+}
+template <>
+const double *Solver::qv<double>(const DLevel &L) {
+  return L.q.p;
+}
+template <>
+const float *Solver::qv<float>(const DLevel &L) {
+  return L.q32.p;
+}
+
+namespace {
+template <class T>
+__global__ void k_scale(T *p, int64_t n, T a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] *= a;
+}
+template <class T>
+__global__ void k_zero(T *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (T)0;
+}
+}  // namespace
+
+template <class T>
+void Solver::fill_zero(T *p, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, n);
+  LG_LAUNCH_CHECK();
+}
+template <class T>
+void Solver::scale_vec(T *p, int64_t n, double a) {
+  hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, n, (T)a);
+  LG_LAUNCH_CHECK();
+}
+
+int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double *hist, int cap) {
+  const DLevel &L0 = H[0];
+  const int64_t n = L0.A.n;
+  if (mem == LIGMG_MEM_HOST) to_device(b0.p, b, n, s);
+  else d2d(b0.p, b, n, s);
+  LG_HIP(hipStreamSynchronize(s));
+  auto t0 = std::chrono::steady_clock::now();
+  const bool do_once = o.top_elim && L0.kind == LIGMG_LEVEL_ELIM;
+  const size_t base = do_once ? 1 : 0;
+  const DLevel &B = H[base];
+  const int64_t nb_ = B.A.n;
+  if (outer.x.n < (size_t)nb_) outer.alloc(nb_, red[base].nb);
+  DevBuf<double> rhsb(nb_);
+  double *rhs = rhsb.p;
+  if (do_once) op_spmv<double>(L0.R, b0.p, rhs, s);
+  else d2d(rhs, b0.p, n, s);
+  red[base].sums<double>(rhs, dsc.p, s);
+  op_sub_block_mean<double>(rhs, red[base], dsc.p, s);
+  fill_zero<double>(outer.x.p, nb_);
+  std::vector<double> h;
+  int status = 0;
+  // the outer loop always runs in FP64 (custom preconditioner bridge)
+  int iters = krylov_outer(base, rhs, h, &status);
+  if (do_once) op_interp<double>(L0.P, L0.q.p, outer.x.p, b0.p, sol.p, s);
+  else d2d(sol.p, outer.x.p, n, s);
+  LG_HIP(hipStreamSynchronize(s));
+  auto t1 = std::chrono::steady_clock::now();
+  // full-system residual (multigrid.hpp:357-366)
+  op_residual<double>(L0.A, sol.p, b0.p, tmp0.p, s);
+  const double *A_[2] = {tmp0.p, b0.p}, *B_[2] = {tmp0.p, b0.p};
+  RedPlan &R0 = red[0];
+  R0.dots<double>(2, A_, B_, dsc.p, s);
+  std::vector<double> d = download(dsc.p, 2 * R0.nb, s);
+  double rr = 0, bb = 0;
+  for (int c = 0; c < R0.nb; c++) {
+    rr += d[c * 2];
+    bb += d[c * 2 + 1];
+  }
+  if (mem == LIGMG_MEM_HOST) to_host(x, sol.p, n, s);
+  else d2d(x, sol.p, n, s);
+  LG_HIP(hipStreamSynchronize(s));
+  if (info) {
+    // cycle_complexity (multigrid.hpp:372-387), ConvergenceInfo (util.hpp:143-170)
+    double sum = 0, mult = 1;
+    for (size_t l = base; l < H.size(); l++) {
+      const DLevel &L = H[l];
+      double wk;
+      if (L.kind == LIGMG_LEVEL_AGG)
+        wk = (double)((o.pre_smooth + o.post_smooth + 1) * L.A.nnz + 3 * L.A.n);
+      else if (L.kind == LIGMG_LEVEL_ELIM)
+        wk = (double)(2 * L.nnzP + L.nnzQ);
+      else
+        wk = (double)L.A.nnz;
+      sum += mult * wk;
+      if (L.kind != LIGMG_LEVEL_ELIM && o.kcycle) mult *= o.kcycle_iters;
+    }
+    double cc = sum / (double)H[0].A.nnz;
+    memset(info, 0, sizeof(*info));
+    info->iters = iters;
+    info->status = status;
+    info->base_level = (int)base;
+    info->history_len = (int)std::min<size_t>(h.size(), cap > 0 ? (size_t)cap : 0);
+    info->rel_residual = h.empty() || h.front() == 0 ? 0 : h.back() / h.front();
+    info->full_rel_residual = bb > 0 ? std::sqrt(rr / bb) : 0;
+    info->time = std::chrono::duration<double>(t1 - t0).count();
+    info->work = iters * (1 + cc);
+    info->cycle_complexity = iters > 0 ? info->work / iters : 0;
+    info->cf = iters > 0 ? std::pow(info->rel_residual, 1.0 / iters) : 0;
+    info->ecf = iters > 0 ? std::pow(info->rel_residual, 1.0 / (iters * info->cycle_complexity)) : 0;
+    info->wda = iters > 0 ? -1.0 / std::log10(info->ecf) : 0;
+    info->setup_time = setup_time;
+  }
+  if (hist)
+    for (int i = 0; i < (int)h.size() && i < cap; i++) hist[i] = h[i];
+  return LIGMG_OK;
+}
+
+}  // namespace lg
+
+// ------------------------------------------------------------------ C ABI
+struct ligmg_solver {
+  lg::Solver S;
+};
+
+namespace {
+int fail(const lg::Error &e) {
+  lg::g_last_error = e.what();
+  return e.code;
+}
+int fail_unknown(const char *what) {
+  lg::g_last_error = what;
+  return LIGMG_ERR_INVALID;
+}
+}  // namespace
+
+extern "C" {
+
+void ligmg_options_default(ligmg_options *o) {
+  memset(o, 0, sizeof(*o));
+  o->verbose = 0;
+  o->always_elim = 1;
+  o->top_elim = 1;
+  o->constant_correction = 0;
+  o->strength_metric = LIGMG_STRENGTH_AFFINITY;
+  o->max_levels = 50;
+  o->min_size = 4000;
+  o->elim_degree = 4;
+  o->tol = 1e-8;
+  o->thresh = 1e-8;
+  o->agg_iters = 10;
+  o->iters = 500;
+  o->pre_smooth = 2;
+  o->post_smooth = 2;
+  o->num_tvs = 10;
+  o->tvs_smooth = 4;
+  o->smoother_type = LIGMG_SMOOTHER_CHEBYSHEV;
+  o->kcycle = 0;
+  o->outer_ksp_type = LIGMG_KSP_CG;
+  o->kcycle_ksp_type = LIGMG_KSP_FCG;
+  o->kcycle_iters = 2;
+  o->votes_to_seed = 1;
+  o->max_agg_size = -1;
+  o->agg_iter_factor = 0.5;
+  o->agg_init_factor = 0.5;
+  o->cheby_lower = 0.3;
+  o->cheby_upper = 1.1;
+  o->aff_scaling = LIGMG_AFF_MAX;
+  o->elim_factor = 0.9;
+  o->elim_iters = 1;
+  o->neighbor_of_neighbor = 0;
+  o->seed = 0;
+  o->cheby_eig_iters = 10;
+  o->precision = LIGMG_PREC_FP64;
+  o->use_graphs = 1;
+  o->reorder = 1;
+}
+
+const char *ligmg_last_error(void) { return lg::g_last_error.c_str(); }
+
+static int check_options(const ligmg_options &o) {
+  if (o.max_agg_size != -1) return LIGMG_ERR_UNSUPPORTED;
+  if (o.elim_iters != 1) return LIGMG_ERR_UNSUPPORTED;
+  if (o.neighbor_of_neighbor) return LIGMG_ERR_UNSUPPORTED;
+  if (o.num_tvs < 1 || o.num_tvs > 16) return LIGMG_ERR_UNSUPPORTED;
+  if (o.aff_scaling != LIGMG_AFF_MAX && o.aff_scaling != LIGMG_AFF_NONE) return LIGMG_ERR_UNSUPPORTED;
+  if (o.pre_smooth < 0 || o.post_smooth < 0 || o.iters < 0 || o.kcycle_iters < 1)
+    return LIGMG_ERR_INVALID;
+  return LIGMG_OK;
+}
+
+int ligmg_create(int64_t n, const int32_t *rowptr, const int32_t *colind, const double *vals,
+                 int mem_kind, const ligmg_options *opts, int32_t ncomp, const int64_t *comp_ptr,
+                 ligmg_handle *out) {
+  lg::g_last_error.clear();
+  if (!out || n <= 0 || n >= INT32_MAX || !rowptr || !colind || !vals) {
+    lg::g_last_error = "ligmg_create: invalid arguments";
+    return LIGMG_ERR_INVALID;
+  }
+  *out = nullptr;
+  try {
+    std::unique_ptr<ligmg_solver> h(new ligmg_solver());
+    if (opts) h->S.o = *opts;
+    else ligmg_options_default(&h->S.o);
+    int rc = check_options(h->S.o);
+    if (rc) {
+      lg::g_last_error = "ligmg_create: unsupported option value";
+      return rc;
+    }
+    std::vector<int64_t> blocks;
+    if (ncomp > 0 && comp_ptr) {
+      blocks.assign(comp_ptr, comp_ptr + ncomp + 1);
+      LG_CHECK(blocks.front() == 0 && blocks.back() == n, LIGMG_ERR_INVALID, "comp_ptr must span [0, n]");
+      for (int c = 0; c < ncomp; c++)
+        LG_CHECK(blocks[c] < blocks[c + 1], LIGMG_ERR_INVALID, "comp_ptr blocks must be non-empty");
+    } else {
+      blocks = {0, n};
+    }
+    h->S.build(n, rowptr, colind, vals, mem_kind, blocks);
+    *out = h.release();
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  } catch (const std::exception &e) {
+    return fail_unknown(e.what());
+  }
+}
+
+int ligmg_solve(ligmg_handle h, const double *b, double *x, int mem_kind, ligmg_info *info,
+                double *history, int32_t cap) {
+  if (!h || !b || !x) return LIGMG_ERR_INVALID;
+  try {
+    return h->S.solve(b, x, mem_kind, info, history, cap);
+  } catch (const lg::Error &e) {
+    return fail(e);
+  } catch (const std::exception &e) {
+    return fail_unknown(e.what());
+  }
+}
+
+int ligmg_destroy(ligmg_handle h) {
+  delete h;
+  return LIGMG_OK;
+}
+
+int ligmg_num_levels(ligmg_handle h, int32_t *out) {
+  if (!h || !out) return LIGMG_ERR_INVALID;
+  *out = (int32_t)h->S.H.size();
+  return LIGMG_OK;
+}
+
+int ligmg_setup_time(ligmg_handle h, double *out) {
+  if (!h || !out) return LIGMG_ERR_INVALID;
+  *out = h->S.setup_time;
+  return LIGMG_OK;
+}
+
+int ligmg_level_info(ligmg_handle h, int32_t l, int32_t *kind, int64_t *n, int64_t *nnz,
+                     int64_t *nc) {
+  if (!h || l < 0 || l >= (int)h->S.H.size()) return LIGMG_ERR_INVALID;
+  const lg::DLevel &L = h->S.H[l];
+  *kind = L.kind;
+  *n = L.A.n;
+  *nnz = L.A.nnz;
+  *nc = L.nc;
+  return LIGMG_OK;
+}
+
+int ligmg_level_matrix(ligmg_handle h, int32_t l, int32_t *rp, int32_t *ci, double *v) {
+  if (!h || l < 0 || l >= (int)h->S.H.size()) return LIGMG_ERR_INVALID;
+  try {
+    const lg::DLevel &L = h->S.H[l];
+    hipStream_t s = h->S.s;
+    lg::to_host(rp, L.A.rp.p, L.A.n + 1, s);
+    lg::to_host(ci, L.A.ci.p, L.A.nnz, s);
+    lg::to_host(v, L.A.v.p, L.A.nnz, s);
+    LG_HIP(hipStreamSynchronize(s));
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_level_map(ligmg_handle h, int32_t l, int32_t *map) {
+  if (!h || l < 0 || l >= (int)h->S.H.size()) return LIGMG_ERR_INVALID;
+  try {
+    const lg::DLevel &L = h->S.H[l];
+    hipStream_t s = h->S.s;
+    if (L.kind == LIGMG_LEVEL_ELIM) lg::to_host(map, L.new_id.p, L.n, s);
+    else if (L.kind == LIGMG_LEVEL_AGG) lg::to_host(map, L.agg.p, L.n, s);
+    else return LIGMG_ERR_INVALID;
+    LG_HIP(hipStreamSynchronize(s));
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_level_states(ligmg_handle h, int32_t l, int32_t *ind, int32_t *state) {
+  if (!h || l < 0 || l >= (int)h->S.H.size() || h->S.H[l].kind != LIGMG_LEVEL_AGG)
+    return LIGMG_ERR_INVALID;
+  try {
+    const lg::DLevel &L = h->S.H[l];
+    hipStream_t s = h->S.s;
+    lg::to_host(ind, L.ind.p, L.n, s);
+    lg::to_host(state, L.state.p, L.n, s);
+    LG_HIP(hipStreamSynchronize(s));
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_level_strength(ligmg_handle h, int32_t l, double *out) {
+  if (!h || l < 0 || l >= (int)h->S.H.size() || h->S.H[l].kind != LIGMG_LEVEL_AGG)
+    return LIGMG_ERR_INVALID;
+  try {
+    const lg::DLevel &L = h->S.H[l];
+    lg::to_host(out, L.strength.p, L.A.nnz, h->S.s);
+    LG_HIP(hipStreamSynchronize(h->S.s));
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_level_cheby(ligmg_handle h, int32_t l, double *lo, double *hi) {
+  if (!h || l < 0 || l >= (int)h->S.H.size() || h->S.H[l].kind != LIGMG_LEVEL_AGG)
+    return LIGMG_ERR_INVALID;
+  *lo = h->S.H[l].cheb_lo;
+  *hi = h->S.H[l].cheb_hi;
+  return LIGMG_OK;
+}
+
+int ligmg_aggregate_with_strength(int64_t n, const int32_t *rowptr, const int32_t *colind,
+                                  const double *sv, const ligmg_options *opts, int32_t *agg,
+                                  int32_t *ind, int32_t *state, int64_t *n_coarse) {
+  if (n <= 0 || !rowptr || !colind || !sv || !agg || !ind || !state || !n_coarse)
+    return LIGMG_ERR_INVALID;
+  hipStream_t s = nullptr;
+  try {
+    ligmg_options o;
+    if (opts) o = *opts;
+    else ligmg_options_default(&o);
+    LG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    lg::DCsr A;
+    A.n = n;
+    A.nnz = rowptr[n] - rowptr[0];
+    A.rp.alloc(n + 1);
+    A.ci.alloc(std::max<int64_t>(A.nnz, 1));
+    lg::DevBuf<double> S(std::max<int64_t>(A.nnz, 1));
+    lg::to_device(A.rp.p, rowptr, n + 1, s);
+    lg::to_device(A.ci.p, colind, A.nnz, s);
+    lg::to_device(S.p, sv, A.nnz, s);
+    lg::DevBuf<double> cs;
+    lg::col_sums_of(A, S.p, cs, s);
+    lg::DevBuf<int32_t> a, i2, st;
+    *n_coarse = lg::run_voting(A, S.p, cs.p, o, a, i2, st, s);
+    lg::to_host(agg, a.p, n, s);
+    lg::to_host(ind, i2.p, n, s);
+    lg::to_host(state, st.p, n, s);
+    LG_HIP(hipStreamSynchronize(s));
+    (void)hipStreamDestroy(s);
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    if (s) (void)hipStreamDestroy(s);
+    return fail(e);
+  }
+}
+
+int ligmg_bench_spmv(ligmg_handle h, int32_t l, int32_t prec, int32_t iters, double *avg_ms,
+                     int64_t *bytes) {
+  if (!h || l < 0 || l >= (int)h->S.H.size() || iters < 1 || !avg_ms || !bytes)
+    return LIGMG_ERR_INVALID;
+  try {
+    lg::Solver &S = h->S;
+    lg::DCsr &A = S.H[l].A;
+    hipStream_t s = S.s;
+    if (A.U.nunits == 0) lg::build_units(A, s);
+    const bool f32 = prec == LIGMG_PREC_MIXED;
+    if (f32 && !A.v32.p) lg::make_v32(A, s);
+    const int64_t n = A.n;
+    lg::DevBuf<double> x(f32 ? 0 : n), y(f32 ? 0 : n);
+    lg::DevBuf<float> x32(f32 ? n : 0), y32(f32 ? n : 0);
+    if (f32) {
+      lg::DevBuf<double> t(n);
+      lg::fill_f64(t.p, 1.0, n, s);
+      lg::op_d2f(t.p, x32.p, n, s);
+      LG_HIP(hipStreamSynchronize(s));
+    } else {
+      lg::fill_f64(x.p, 1.0, n, s);
+    }
+    auto run = [&]() {
+      if (f32) lg::op_spmv<float>(A, x32.p, y32.p, s);
+      else lg::op_spmv<double>(A, x.p, y.p, s);
+    };
+    run();  // warm-up
+    hipEvent_t e0, e1;
+    LG_HIP(hipEventCreate(&e0));
+    LG_HIP(hipEventCreate(&e1));
+    LG_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < iters; i++) run();
+    LG_HIP(hipEventRecord(e1, s));
+    LG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / iters;
+    const int64_t sv = f32 ? 4 : 8, sx = f32 ? 4 : 8;
+    *bytes = A.nnz * (sv + 4) + (n + 1) * 4 + 2 * n * sx;
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_solve_batch(ligmg_handle, int32_t, const double *, double *, int, ligmg_info *, int32_t *) {
+  lg::g_last_error = "ligmg_solve_batch: not available in this build";
+  return LIGMG_ERR_UNSUPPORTED;
+}
+
+int ligmg_edge_grad(ligmg_handle, const double *, const double *, double *) {
+  lg::g_last_error = "ligmg_edge_grad: not available in this build";
+  return LIGMG_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

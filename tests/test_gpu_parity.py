@@ -1,0 +1,218 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Bit-exact: elimination sets, aggregates, AggStates, strengths and every
+coarse matrix (pattern and values).  Solutions: within 1e-6 relative after
+mean removal, 1e-8 relative residual, error against the FP64 sparse-direct
+solve stated."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import oracle_ffi as O
+import ligmg_amd
+from ligmg_amd import _ffi, graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def make_case(name, my_matrix=None):
+    if name == "my_matrix":
+        return my_matrix
+    if name == "grid64":
+        return graphs.grid2d(64, 64)
+    if name == "grid_w":
+        return graphs.grid2d(40, 33, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=2)
+    if name == "rmat12_pend":
+        return graphs.rmat_config(12, "lognormal", pendants=True)
+    if name == "rmat13_unif":
+        return graphs.rmat_config(13, "uniform")
+    if name == "ba":
+        return graphs.barabasi_albert(6000, 5, seed=3)
+    if name == "rgg":
+        n = 6000
+        return graphs.rgg(n, math.sqrt(12 / (math.pi * n)), seed=4)
+    if name == "grid3d":
+        return graphs.grid3d(14, 12, 10, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=5)
+    raise KeyError(name)
+
+
+CASES = ["my_matrix", "grid64", "grid_w", "rmat12_pend", "rmat13_unif", "ba", "rgg", "grid3d"]
+
+
+def opts_pair(**kw):
+    return ligmg_amd.Options(**kw), O.default_options(**kw)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_hierarchy_bit_exact(case, my_matrix):
+    g = make_case(case, my_matrix)
+    print(f"case={case} n={g.n} nnz={g.nnz}", flush=True)
+    do, oo = opts_pair()
+    d = ligmg_amd.Solver(g, do)
+    r = O.OracleSolver(g, oo)
+    assert d.num_levels() == r.num_levels()
+    for l in range(d.num_levels()):
+        assert d.level_info(l) == r.level_info(l), f"level {l}"
+        drp, dci, dv = d.level_matrix(l)
+        rrp, rci, rv = r.level_matrix(l)
+        assert np.array_equal(drp, rrp) and np.array_equal(dci, rci), f"level {l} pattern"
+        assert np.array_equal(dv, rv), f"level {l} values (max diff {np.abs(dv - rv).max()})"
+        kind = d.level_info(l)[0]
+        if kind == _ffi.LEVEL_COARSE:
+            continue
+        assert np.array_equal(d.level_map(l), r.level_map(l)), f"level {l} map"
+        if kind == _ffi.LEVEL_AGG:
+            di, ds = d.level_states(l)
+            ri, rs = r.level_states(l)
+            assert np.array_equal(di, ri) and np.array_equal(ds, rs), f"level {l} states"
+            assert np.array_equal(d.level_strength(l), r.level_strength(l)), f"level {l} strength"
+            dl, dh = d.level_cheby(l)
+            rl, rh = r.level_cheby(l)
+            assert dh == pytest.approx(rh, rel=1e-9)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_voting_with_injected_strength(seed):
+    """Same strengths in, same aggregates out (the north-star bit-exactness
+    condition), including ties: strengths drawn from a few values."""
+    g = graphs.rmat(11, 8, wkind=_ffi.W_UNIT, seed=seed)
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(g.n), np.diff(g.rowptr))
+    vals = rng.choice([0.0, 0.25, 0.5, 0.75, 1.0], size=g.nnz)
+    S = np.where(rows == g.colind, 0.0, vals)
+    da = ligmg_amd.aggregate_with_strength(g, S)
+    rp, ci, _ = O.csr_args(g)
+    agg = np.empty(g.n, np.int32)
+    ind = np.empty(g.n, np.int32)
+    st = np.empty(g.n, np.int32)
+    nc = O.i64()
+    o = O.default_options()
+    assert O.lib().oracle_aggregate_with_strength(
+        g.n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(S, O.f64), O.C.byref(o), O.ptr(agg, O.i32),
+        O.ptr(ind, O.i32), O.ptr(st, O.i32), O.C.byref(nc)) == 0
+    assert da[3] == nc.value
+    assert np.array_equal(da[0], agg) and np.array_equal(da[1], ind) and np.array_equal(da[2], st)
+
+
+def direct_error(g, b, x):
+    A = g.to_scipy().tocsc()
+    xr = np.zeros(g.n)
+    xr[1:] = spla.splu(A[1:, 1:]).solve(b[1:])
+    xr -= xr.mean()
+    return np.linalg.norm((x - x.mean()) - xr) / np.linalg.norm(xr)
+
+
+VARIANTS = {
+    "vcycle_cg": {},
+    "kcycle_fcg": {"kcycle": 1, "outer_ksp_type": _ffi.KSP_FCG},
+    "jacobi": {"smoother_type": _ffi.SMOOTHER_JACOBI},
+    "no_top_elim": {"top_elim": 0},
+    "fcg_outer": {"outer_ksp_type": _ffi.KSP_FCG},
+}
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+@pytest.mark.parametrize("case", ["my_matrix", "rmat12_pend", "grid_w"])
+def test_solve_matches_oracle(case, variant, my_matrix):
+    g = make_case(case, my_matrix)
+    kw = VARIANTS[variant]
+    print(f"case={case} variant={variant}", flush=True)
+    do, oo = opts_pair(**kw)
+    d = ligmg_amd.Solver(g, do)
+    r = O.OracleSolver(g, oo)
+    b = graphs.rhs(g.n, 1, 0)
+    info = d.solve(b)
+    xr, rinfo, rhist = r.solve(b)
+    x = info.solution
+    assert info.status == _ffi.CONVERGED
+    assert info.rel_residual() <= 1e-8
+    assert abs(info.iters() - rinfo.iters) <= 2
+    xm, xrm = x - x.mean(), xr - xr.mean()
+    assert np.linalg.norm(xm - xrm) <= 1e-6 * np.linalg.norm(xrm)
+    err = direct_error(g, b, x)
+    print(f"iters={info.iters()} oracle_iters={rinfo.iters} rel={info.rel_residual():.2e} "
+          f"full={info.full_rel_residual:.2e} err_vs_direct={err:.2e}")
+    assert err <= 1e-6
+    assert len(info.residuals) == info.iters() + 1
+    assert info.cycle_complexity() == pytest.approx(rinfo.cycle_complexity, rel=1e-12)
+
+
+@pytest.mark.parametrize("case", ["rmat12_pend", "grid_w", "ba"])
+def test_mixed_precision(case, my_matrix):
+    """FP32 hierarchy values and cycle inside the FP64 flexible-CG outer
+    loop: reaches 1e-8 and agrees with the FP64 oracle within 1e-6."""
+    g = make_case(case, my_matrix)
+    d = ligmg_amd.Solver(g, ligmg_amd.Options(precision=_ffi.PREC_MIXED,
+                                               outer_ksp_type=_ffi.KSP_FCG))
+    b = graphs.rhs(g.n, 1, 1)
+    info = d.solve(b)
+    xr, rinfo, _ = O.OracleSolver(g).solve(b)
+    assert info.status == _ffi.CONVERGED and info.rel_residual() <= 1e-8
+    x = info.solution
+    assert np.linalg.norm((x - x.mean()) - (xr - xr.mean())) <= 1e-6 * np.linalg.norm(xr - xr.mean())
+    print(f"mixed iters={info.iters()} fp64 oracle iters={rinfo.iters}")
+    assert info.iters() <= rinfo.iters + 5
+
+
+def test_determinism():
+    g = make_case("rmat12_pend")
+    b = graphs.rhs(g.n, 1, 3)
+    x1 = ligmg_amd.Solver(g).solve(b).solution
+    x2 = ligmg_amd.Solver(g).solve(b).solution
+    assert np.array_equal(x1, x2)
+
+
+def test_device_resident_inputs():
+    import torch
+    g = make_case("grid_w")
+    b = graphs.rhs(g.n, 1, 2)
+    dev = torch.device("cuda:0")
+    A = (torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev),
+         torch.from_numpy(g.vals).to(dev))
+    s = ligmg_amd.Solver(A)
+    info = s.solve(torch.from_numpy(b).to(dev))
+    xh = ligmg_amd.Solver(g).solve(b).solution
+    assert np.array_equal(info.solution.cpu().numpy(), xh)
+
+
+def test_blocks_solve_independently():
+    gs = [graphs.barabasi_albert(n, 4, seed=s) for s, n in enumerate([3000, 800, 5000])]
+    u = graphs.union(gs)
+    b = graphs.rhs(u.n, 1, 5, comp_ptr=u.comp_ptr)
+    d = ligmg_amd.Solver(u, comp_ptr=u.comp_ptr)
+    info = d.solve(b)
+    xr, _, _ = O.OracleSolver(u, comp_ptr=u.comp_ptr).solve(b)
+    for c in range(len(gs)):
+        lo, hi = u.comp_ptr[c], u.comp_ptr[c + 1]
+        x, y = info.solution[lo:hi], xr[lo:hi]
+        assert np.linalg.norm((x - x.mean()) - (y - y.mean())) <= 1e-6 * np.linalg.norm(y - y.mean())
+
+
+def test_zero_rhs_and_tiny_graphs():
+    g = graphs.grid2d(8, 8)
+    s = ligmg_amd.Solver(g)
+    info = s.solve(np.zeros(g.n))
+    assert info.iters() == 0 and np.all(info.solution == 0)
+    for n, edges in [(2, [(0, 1)]), (3, [(0, 1), (1, 2)]), (5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)])]:
+        u = np.array([e[0] for e in edges], np.int32)
+        v = np.array([e[1] for e in edges], np.int32)
+        g = graphs.from_edges(n, u, v, np.ones(len(edges)))
+        b = np.arange(n, dtype=float)
+        b -= b.mean()
+        info = ligmg_amd.Solver(g).solve(b)
+        r = b - g.to_scipy() @ info.solution
+        assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b)
+
+
+def test_asymmetric_pattern_is_rejected():
+    g = graphs.grid2d(10, 10)
+    # drop one off-diagonal entry of row 0 -> structurally asymmetric
+    rp, ci, v = g.rowptr.copy(), g.colind.copy(), g.vals.copy()
+    k = rp[0] + 1
+    ci = np.delete(ci, k)
+    v = np.delete(v, k)
+    rp[1:] -= 1
+    with pytest.raises(ligmg_amd.LigmgError):
+        ligmg_amd.Solver((rp, ci, v))
