@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-scale", type=int, default=18)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="setup + SpMV timing + copy calibration only (for counter runs)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -85,11 +87,29 @@ def main():
          torch.from_numpy(g.vals).to(dev))
     torch.cuda.synchronize()
 
+    # copy kernel of known size: calibrates HBM counters (FETCH_SIZE /
+    # WRITE_SIZE) against 2 GiB read + 2 GiB written
+    src = torch.ones(1 << 28, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
     # FP64-only solve (the plain method, the drop-in): V-cycle + CG
     verbose = int(os.environ.get("LIGMG_VERBOSE", "0"))
     s64 = ligmg_amd.Solver(A, ligmg_amd.Options(verbose=verbose))
     print(f"# fp64 setup {s64.setup_time:.2f}s levels {s64.num_levels()}", file=sys.stderr,
           flush=True)
+    if args.spmv_only:
+        lib = _ffi.solver_lib()
+        ms, nbytes = C.c_double(), C.c_int64()
+        for prec in (_ffi.PREC_FP64, _ffi.PREC_MIXED):
+            rc = lib.ligmg_bench_spmv(s64._h, 1 if s64.level_info(0)[0] == _ffi.LEVEL_ELIM else 0,
+                                      prec, 20, C.byref(ms), C.byref(nbytes))
+            assert rc == 0
+            print(f"# spmv prec={prec} {ms.value:.4f} ms {nbytes.value / ms.value / 1e6:.1f} GB/s",
+                  file=sys.stderr, flush=True)
+        return
     info64 = None
     t64 = []
     for _ in range(args.fp64_steps):

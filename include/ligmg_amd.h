@@ -101,7 +101,8 @@ typedef struct ligmg_options {
   int32_t cheby_eig_iters;      /* 10: CG-Lanczos steps for lambda_max       */
   int32_t precision;            /* LIGMG_PREC_FP64                           */
   int32_t use_graphs;           /* 1: replay cycles from captured hipGraphs  */
-  int32_t reorder;              /* 1: solve-phase locality permutation       */
+  int32_t spmv_slice;           /* -1: autotune column slicing of the solve
+                                   SpMVs; 0: never; >0: slice width (cols)  */
 } ligmg_options;
 
 /* ConvergenceInfo (util.hpp:143-170) plus the fields the reference only
@@ -189,6 +190,12 @@ int ligmg_level_cheby(ligmg_handle h, int32_t level, double *lo, double *hi);
  * micro-benchmark (src/bench_spmv.cpp:1-50). */
 int ligmg_bench_spmv(ligmg_handle h, int32_t level, int32_t precision,
                      int32_t iters, double *avg_ms, int64_t *bytes);
+
+/* y = A x with level `level`'s solve-phase operator (the same kernels and
+ * layout the cycles use: column slices, FP32 Laplacian form for
+ * LIGMG_PREC_MIXED).  x and y are host arrays of length n in FP64. */
+int ligmg_level_spmv(ligmg_handle h, int32_t level, int32_t precision,
+                     const double *x, double *y);
 
 /* ---- stage entry points for parity tests (device execution) ---------- */
 /* Voting aggregation (aggregation.hpp:838-964) driven by caller-supplied

@@ -29,7 +29,7 @@ class ligmg_options(C.Structure):
         ("elim_factor", f64), ("elim_iters", i32),
         ("neighbor_of_neighbor", i32), ("seed", u64),
         ("cheby_eig_iters", i32), ("precision", i32), ("use_graphs", i32),
-        ("reorder", i32),
+        ("spmv_slice", i32),
     ]
 
 
@@ -87,6 +87,7 @@ SOLVER_PROTOS = {
     "ligmg_level_states": (C.c_int, [HANDLE, i32, P(i32), P(i32)]),
     "ligmg_level_strength": (C.c_int, [HANDLE, i32, P(f64)]),
     "ligmg_level_cheby": (C.c_int, [HANDLE, i32, P(f64), P(f64)]),
+    "ligmg_level_spmv": (C.c_int, [HANDLE, i32, i32, P(f64), P(f64)]),
     "ligmg_bench_spmv": (C.c_int, [HANDLE, i32, i32, i32, P(f64), P(i64)]),
     "ligmg_aggregate_with_strength": (C.c_int, [i64, P(i32), P(i32), P(f64),
                                                 P(ligmg_options), P(i32), P(i32),

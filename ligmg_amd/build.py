@@ -61,10 +61,30 @@ def build_graphs():
     return target
 
 
+def build_petsc_pc():
+    """The PETSc "ligmg" PC (csrc/petsc/ligmg_pc.cpp), only where PETSc is
+    installed: PETSC_DIR (and PETSC_ARCH) must point at it."""
+    pdir = os.environ.get("PETSC_DIR")
+    if not pdir:
+        return None
+    arch = os.environ.get("PETSC_ARCH", "")
+    inc = [os.path.join(pdir, "include"), os.path.join(pdir, arch, "include")]
+    lib = os.path.join(pdir, arch, "lib")
+    target = os.path.join(HERE, "libligmg_petsc.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", *[f"-I{i}" for i in inc],
+          os.path.join(CSRC, "petsc", "ligmg_pc.cpp"), "-o", target, f"-L{lib}", "-lpetsc",
+          f"-L{HERE}", "-lligmg_amd", f"-Wl,-rpath,{HERE}", f"-Wl,-rpath,{lib}"])
+    return target
+
+
 def build_all(verbose=False):
     g = build_graphs()
     s = build_solver(verbose)
-    return [g, s]
+    out = [g, s]
+    p = build_petsc_pc()
+    if p:
+        out.append(p)
+    return out
 
 
 if __name__ == "__main__":

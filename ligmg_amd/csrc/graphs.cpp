@@ -9,9 +9,12 @@
 #include "rng.hpp"
 
 #include <omp.h>
+#include <sys/mman.h>
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -20,8 +23,44 @@
 
 namespace {
 
+// Phase timing on stderr when LIGMG_GEN_VERBOSE is set.
+struct Phase {
+  bool on = getenv("LIGMG_GEN_VERBOSE") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void operator()(const char *what) {
+    if (!on) return;
+    auto now = std::chrono::steady_clock::now();
+    fprintf(stderr, "[gen] %-14s %.2f s\n", what, std::chrono::duration<double>(now - t).count());
+    t = now;
+  }
+};
+
 using ligmg_rng::ctr_hash;
 using ligmg_rng::u01;
+
+// Large buffers: 2 MiB aligned and advised for transparent huge pages, so
+// that first touch costs one fault per 2 MiB instead of per 4 KiB.  Freed
+// with free().
+void *huge_alloc(size_t bytes) {
+  const size_t align = (size_t)2 << 20;
+  size_t sz = (bytes + align - 1) / align * align;
+  if (sz == 0) sz = align;
+  void *p = aligned_alloc(align, sz);
+  if (p) madvise(p, sz, MADV_HUGEPAGE);
+  return p;
+}
+
+template <class T>
+struct HugeArr {
+  T *p = nullptr;
+  size_t n = 0;
+  explicit HugeArr(size_t count) : p((T *)huge_alloc(sizeof(T) * count)), n(count) {}
+  HugeArr(const HugeArr &) = delete;
+  HugeArr &operator=(const HugeArr &) = delete;
+  ~HugeArr() { free(p); }
+  T &operator[](size_t i) { return p[i]; }
+  T *data() { return p; }
+};
 
 const uint64_t S_RMAT = 0x524d4154ull, S_WEIGHT = 0x57474854ull, S_PEND = 0x50454e44ull,
                S_PLEN = 0x504c454eull, S_BA = 0x42410000ull, S_RGGX = 0x52474758ull,
@@ -58,17 +97,19 @@ struct ColVal {
 int build_laplacian(int64_t n, const Edges &E, ligmg_graph *out) {
   const int64_t m = (int64_t)E.u.size();
   if (n <= 0 || 2 * m + n >= (int64_t)INT32_MAX) return 1;
+  Phase ph;
   std::vector<int64_t> deg(n + 1, 0);
+#pragma omp parallel for
   for (int64_t e = 0; e < m; e++) {
-    deg[E.u[e]]++;
-    deg[E.v[e]]++;
+    __atomic_fetch_add(&deg[E.u[e]], 1, __ATOMIC_RELAXED);
+    __atomic_fetch_add(&deg[E.v[e]], 1, __ATOMIC_RELAXED);
   }
   int64_t nnz = 2 * m + n;
   out->n = n;
   out->nnz = nnz;
-  out->rowptr = (int32_t *)malloc(sizeof(int32_t) * (n + 1));
-  out->colind = (int32_t *)malloc(sizeof(int32_t) * nnz);
-  out->vals = (double *)malloc(sizeof(double) * nnz);
+  out->rowptr = (int32_t *)huge_alloc(sizeof(int32_t) * (n + 1));
+  out->colind = (int32_t *)huge_alloc(sizeof(int32_t) * nnz);
+  out->vals = (double *)huge_alloc(sizeof(double) * nnz);
   out->ncomp = 1;
   out->comp_ptr = (int64_t *)malloc(sizeof(int64_t) * 2);
   if (!out->rowptr || !out->colind || !out->vals || !out->comp_ptr) return 2;
@@ -77,16 +118,32 @@ int build_laplacian(int64_t n, const Edges &E, ligmg_graph *out) {
   std::vector<int64_t> start(n + 1, 0);
   for (int64_t i = 0; i < n; i++) start[i + 1] = start[i] + deg[i] + 1;
   for (int64_t i = 0; i <= n; i++) out->rowptr[i] = (int32_t)start[i];
-  std::vector<int64_t> fill(start.begin(), start.end() - 1);
-  ColVal *cv = (ColVal *)malloc(sizeof(ColVal) * nnz);
+  // the diagonal takes the first slot of every row; edges fill the rest in
+  // any order (rows are sorted afterwards, so the result is deterministic)
+  ph("  lap:deg");
+  std::vector<int64_t> fill(n);
+  ColVal *cv = (ColVal *)huge_alloc(sizeof(ColVal) * nnz);
   if (!cv) return 2;
+  // first touch of the large arrays in parallel, in address order
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < nnz; k++) {
+    cv[k].c = 0;
+    out->colind[k] = 0;
+    out->vals[k] = 0.0;
+  }
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; i++) {
+    cv[start[i]] = {(int32_t)i, 0.0};
+    fill[i] = start[i] + 1;
+  }
+#pragma omp parallel for
   for (int64_t e = 0; e < m; e++) {
     int32_t a = E.u[e], b = E.v[e];
     double w = E.w[e];
-    cv[fill[a]++] = {b, -w};
-    cv[fill[b]++] = {a, -w};
+    cv[__atomic_fetch_add(&fill[a], 1, __ATOMIC_RELAXED)] = {b, -w};
+    cv[__atomic_fetch_add(&fill[b], 1, __ATOMIC_RELAXED)] = {a, -w};
   }
-  for (int64_t i = 0; i < n; i++) cv[fill[i]++] = {(int32_t)i, 0.0};
+  ph("  lap:scatter");
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t i = 0; i < n; i++) {
     ColVal *r = cv + start[i];
@@ -104,6 +161,7 @@ int build_laplacian(int64_t n, const Edges &E, ligmg_graph *out) {
       out->vals[start[i] + k] = r[k].v;
     }
   }
+  ph("  lap:sort");
   free(cv);
   return 0;
 }
@@ -173,7 +231,7 @@ void dedup_pairs(int64_t n, std::vector<int32_t> &us, std::vector<int32_t> &vs) 
   for (int64_t i = 0; i < n; i++) start[i + 1] = start[i] + cnt[i].load();
   std::vector<std::atomic<int64_t>> pos(n);
   for (int64_t i = 0; i < n; i++) pos[i].store(start[i], std::memory_order_relaxed);
-  std::vector<int32_t> col(m);
+  HugeArr<int32_t> col(m);
 #pragma omp parallel for
   for (int64_t e = 0; e < m; e++) col[pos[us[e]].fetch_add(1, std::memory_order_relaxed)] = vs[e];
   std::vector<int64_t> ulen(n, 0);
@@ -260,15 +318,19 @@ int ligmg_gen_rmat(int scale, int edge_factor, double a, double b, double c, int
   if (scale < 1 || scale > 30 || edge_factor < 1 || !out) return 1;
   memset(out, 0, sizeof(*out));
   if (nthreads > 0) omp_set_num_threads(nthreads);
+  Phase ph;
   const int64_t n0 = (int64_t)1 << scale;
   const int64_t E0 = (int64_t)edge_factor << scale;
   std::vector<int32_t> us(E0), vs(E0);
   const double ab = a + b, abc = a + b + c;
+  // u01(seed, S_RMAT, idx) with the per-stream key hoisted out of the loop
+  const uint64_t key = ligmg_rng::mix64(seed ^ ligmg_rng::mix64(S_RMAT));
 #pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < E0; e++) {
     uint64_t x = 0, y = 0;
     for (int l = 0; l < scale; l++) {
-      double r = u01(seed, S_RMAT, (uint64_t)e * 64 + l);
+      double r = (double)(ligmg_rng::mix64(key + (uint64_t)e * 64 + l) >> 11) *
+                 (1.0 / 9007199254740992.0);
       uint64_t bx = 0, by = 0;
       if (r < a) {
       } else if (r < ab) {
@@ -296,12 +358,15 @@ int ligmg_gen_rmat(int scale, int edge_factor, double a, double b, double c, int
     }
   us.resize(k);
   vs.resize(k);
+  ph("edges");
   dedup_pairs(n0, us, vs);
+  ph("dedup");
   Edges E;
   E.u.swap(us);
   E.v.swap(vs);
   E.w.assign(E.u.size(), 0.0);
   int64_t n = keep_lcc(n0, E);
+  ph("lcc");
   // pendant paths
   if (pendant_frac > 0 && pendant_max > 0) {
     int64_t next = n;
@@ -319,8 +384,12 @@ int ligmg_gen_rmat(int scale, int edge_factor, double a, double b, double c, int
     }
     n = next;
   }
+  ph("pendants");
   assign_weights(E, wkind, p0, p1, seed);
-  return build_laplacian(n, E, out);
+  ph("weights");
+  int rc = build_laplacian(n, E, out);
+  ph("laplacian");
+  return rc;
 }
 
 int ligmg_gen_ba(int64_t n, int m, int wkind, double p0, double p1, uint64_t seed,
@@ -443,9 +512,9 @@ int ligmg_graph_union(const ligmg_graph *gs, int count, ligmg_graph *out) {
   if (nnz >= (int64_t)INT32_MAX || n >= (int64_t)INT32_MAX) return 1;
   out->n = n;
   out->nnz = nnz;
-  out->rowptr = (int32_t *)malloc(sizeof(int32_t) * (n + 1));
-  out->colind = (int32_t *)malloc(sizeof(int32_t) * nnz);
-  out->vals = (double *)malloc(sizeof(double) * nnz);
+  out->rowptr = (int32_t *)huge_alloc(sizeof(int32_t) * (n + 1));
+  out->colind = (int32_t *)huge_alloc(sizeof(int32_t) * nnz);
+  out->vals = (double *)huge_alloc(sizeof(double) * nnz);
   out->ncomp = count;
   out->comp_ptr = (int64_t *)malloc(sizeof(int64_t) * (count + 1));
   if (!out->rowptr || !out->colind || !out->vals || !out->comp_ptr) return 2;

@@ -19,6 +19,7 @@ namespace lg {
 // holding at most UNIT_NNZ stored entries, or one UNIT_NNZ-chunk of a longer
 // row whose partial sum is combined afterwards in chunk order.
 constexpr int UNIT_NNZ = 2048;
+constexpr int UNIT_ROWS = 512;
 constexpr int SPMV_BLOCK = 256;
 
 struct Units {
@@ -36,7 +37,17 @@ struct DCsr {
   DevBuf<int32_t> rp, ci;
   DevBuf<double> v;   // empty for pattern-only operators (unit values)
   DevBuf<float> v32;  // FP32 copy (mixed precision)
+  // FP32 Laplacian form: v32 holds the rounded off-diagonals (0 on the
+  // diagonal) and d32 the FP64 diagonal -sum of those rounded entries, so
+  // the FP32 operator keeps the constant nullspace exactly.
+  DevBuf<double> d32;
   Units U;
+  // Column-sliced copies for the solve (index 0: FP64 values, 1: FP32
+  // values): slice k holds the entries with columns in [col0[k], col0[k+1])
+  // as its own CSR over all rows, so one pass gathers from an x window that
+  // stays in L2.  Empty when the operator is used unsliced.
+  std::vector<std::unique_ptr<DCsr>> slices[2];
+  DevBuf<double> acc;  // FP64 row accumulator across slice passes
 };
 
 struct DLevel {

@@ -9,8 +9,15 @@ namespace lg {
 
 // Build the SpMV work units of A from its row pointers (host greedy pass).
 void build_units(DCsr &A, hipStream_t s);
-// FP32 copy of the values.
-void make_v32(DCsr &A, hipStream_t s);
+// Column-sliced copy of A for precision prec (0: FP64, 1: FP32) with
+// slices of `width` columns (width >= ncols: no slicing).
+void build_slices(DCsr &A, int prec, int64_t width, hipStream_t s);
+// Milliseconds per y = A x launch (HIP events, `iters` launches), for tuning.
+double time_spmv(const DCsr &A, int prec, int iters, hipStream_t s);
+
+// FP32 copy of the values.  laplacian: store the operator in Laplacian form
+// (rounded off-diagonals + FP64 diagonal = -their sum).
+void make_v32(DCsr &A, hipStream_t s, bool laplacian = false);
 
 // y = A x
 template <class T> void op_spmv(const DCsr &A, const T *x, T *y, hipStream_t s);

@@ -1,14 +1,18 @@
 // Solve-phase kernels: row-block SpMV family with fused epilogues, transfer
 // operators, dense coarse apply, deterministic blocked reductions.
 #include <algorithm>
+#include <type_traits>
 
 #include "ops.hpp"
+#include "prims.hpp"
 
 namespace lg {
 
 namespace {
 
 // ------------------------------------------------------------ value access
+// Matrix data is streamed once per product: non-temporal loads keep it from
+// displacing the gathered vector in L2.
 template <class T>
 struct ValOne {
   __device__ __forceinline__ T operator()(int64_t) const { return (T)1; }
@@ -16,37 +20,73 @@ struct ValOne {
 template <class T>
 struct ValArr {
   const T *v;
-  __device__ __forceinline__ T operator()(int64_t k) const { return v[k]; }
+  __device__ __forceinline__ T operator()(int64_t k) const { return __builtin_nontemporal_load(v + k); }
 };
 
 // --------------------------------------------------------------- epilogues
+// Row sums are always accumulated in FP64 (a product of two FP32 numbers
+// is exact in FP64), so the FP32 path only rounds storage, not the
+// heavily cancelling Laplacian row sums of hub vertices.
 template <class T>
 struct EpiStore {
   T *y;
-  __device__ __forceinline__ void operator()(int64_t r, T s) const { y[r] = s; }
+  __device__ __forceinline__ void operator()(int64_t r, double s) const { y[r] = (T)s; }
 };
 template <class T>
 struct EpiResid {
   const T *b;
   T *r;
-  __device__ __forceinline__ void operator()(int64_t i, T s) const { r[i] = b[i] - s; }
+  __device__ __forceinline__ void operator()(int64_t i, double s) const {
+    r[i] = (T)((double)b[i] - s);
+  }
 };
 template <class T>
 struct EpiCheb {
   const T *dinv, *b, *xc, *xp;
   T *xn;
-  T alpha, beta;
-  __device__ __forceinline__ void operator()(int64_t i, T s) const {
-    T c = xc[i];
-    T d = xp ? alpha * (c - xp[i]) : (T)0;
-    xn[i] = c + d + beta * dinv[i] * (b[i] - s);
+  double alpha, beta;
+  __device__ __forceinline__ void operator()(int64_t i, double s) const {
+    double c = (double)xc[i];
+    double d = xp ? alpha * (c - (double)xp[i]) : 0.0;
+    xn[i] = (T)(c + d + beta * (double)dinv[i] * ((double)b[i] - s));
   }
 };
 template <class T>
 struct EpiInterp {
   const T *q, *b;
   T *x;
-  __device__ __forceinline__ void operator()(int64_t i, T s) const { x[i] = s + q[i] * b[i]; }
+  __device__ __forceinline__ void operator()(int64_t i, double s) const {
+    x[i] = (T)(s + (double)q[i] * (double)b[i]);
+  }
+};
+
+// slice passes: first writes the FP64 accumulator, middle ones add to it,
+// the last one hands accumulator + its own sum to the real epilogue
+struct EpiAccSet {
+  double *acc;
+  __device__ __forceinline__ void operator()(int64_t i, double s) const { acc[i] = s; }
+};
+struct EpiAccAdd {
+  double *acc;
+  __device__ __forceinline__ void operator()(int64_t i, double s) const { acc[i] += s; }
+};
+template <class Epi>
+struct EpiAccFinal {
+  const double *acc;
+  Epi e;
+  __device__ __forceinline__ void operator()(int64_t i, double s) const { e(i, acc[i] + s); }
+};
+
+// adds d[i] x[i] (FP64) before the wrapped epilogue: the diagonal term of an
+// FP32 operator stored in Laplacian form
+template <class T, class Epi>
+struct EpiDiag {
+  const double *d;
+  const T *x;
+  Epi e;
+  __device__ __forceinline__ void operator()(int64_t i, double s) const {
+    e(i, s + d[i] * (double)x[i]);
+  }
 };
 
 template <class T>
@@ -79,21 +119,48 @@ __global__ void __launch_bounds__(SPMV_BLOCK)
     k_spmv_units(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, Val val,
                  const T *__restrict__ x, const int32_t *__restrict__ urow0,
                  const int32_t *__restrict__ unrows, const int32_t *__restrict__ ukoff,
-                 T *__restrict__ parts, Epi epi) {
-  __shared__ T prod[UNIT_NNZ];
+                 double *__restrict__ parts, Epi epi) {
+  constexpr int IT = UNIT_NNZ / SPMV_BLOCK;
+  __shared__ double prod[UNIT_NNZ];
   const int u = blockIdx.x;
   const int row0 = urow0[u], nr = unrows[u], k0 = ukoff[u];
+  // all column/value loads of a lane first, then all gathers, then products:
+  // IT independent gathers in flight per lane
+  auto stage = [&](int cnt, double *out) {
+    int col[IT];
+    T vv[IT];
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const int t = threadIdx.x + j * SPMV_BLOCK;
+      col[j] = (t < cnt) ? __builtin_nontemporal_load(ci + k0 + t) : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const int t = threadIdx.x + j * SPMV_BLOCK;
+      vv[j] = (t < cnt) ? val(k0 + t) : (T)0;
+    }
+    double xx[IT];
+#pragma unroll
+    for (int j = 0; j < IT; j++) xx[j] = (col[j] >= 0) ? (double)x[col[j]] : 0.0;
+#pragma unroll
+    for (int j = 0; j < IT; j++) out[j] = (double)vv[j] * xx[j];
+  };
   if (nr > 0) {
     const int cnt = rp[row0 + nr] - k0;
-    for (int t = threadIdx.x; t < cnt; t += SPMV_BLOCK) {
-      const int k = k0 + t;
-      prod[t] = val(k) * x[ci[k]];
+    {
+      double pr[IT];
+      stage(cnt, pr);
+#pragma unroll
+      for (int j = 0; j < IT; j++) {
+        const int t = threadIdx.x + j * SPMV_BLOCK;
+        if (t < cnt) prod[t] = pr[j];
+      }
     }
     __syncthreads();
     if (cnt <= 8 * nr) {
       for (int r = threadIdx.x; r < nr; r += SPMV_BLOCK) {
         const int a = rp[row0 + r] - k0, b = rp[row0 + r + 1] - k0;
-        T s = 0;
+        double s = 0;
         for (int t = a; t < b; t++) s += prod[t];
         epi(row0 + r, s);
       }
@@ -101,7 +168,7 @@ __global__ void __launch_bounds__(SPMV_BLOCK)
       const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
       for (int r = w; r < nr; r += SPMV_BLOCK / WAVE) {
         const int a = rp[row0 + r] - k0, b = rp[row0 + r + 1] - k0;
-        T s = 0;
+        double s = 0;
         for (int t = a + lane; t < b; t += WAVE) s += prod[t];
         s = wave_sum(s);
         if (lane == 0) epi(row0 + r, s);
@@ -109,20 +176,23 @@ __global__ void __launch_bounds__(SPMV_BLOCK)
     }
   } else {
     const int row = row0;
-    const int kend = min(k0 + UNIT_NNZ, rp[row + 1]);
-    T s = 0;
-    for (int k = k0 + threadIdx.x; k < kend; k += SPMV_BLOCK) s += val(k) * x[ci[k]];
+    const int cnt = min(k0 + UNIT_NNZ, rp[row + 1]) - k0;
+    double pr[IT];
+    stage(cnt, pr);
+    double s = 0;
+#pragma unroll
+    for (int j = 0; j < IT; j++) s += pr[j];
     s = block_sum(s);
     if (threadIdx.x == 0) parts[-nr] = s;
   }
 }
 
-template <class T, class Epi>
+template <class Epi>
 __global__ void k_spmv_long(const int32_t *lrow, const int32_t *lpart, int64_t nlong,
-                            const T *parts, Epi epi) {
+                            const double *parts, Epi epi) {
   int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j >= nlong) return;
-  T s = 0;
+  double s = 0;
   for (int p = lpart[j]; p < lpart[j + 1]; p++) s += parts[p];
   epi(lrow[j], s);
 }
@@ -130,13 +200,13 @@ __global__ void k_spmv_long(const int32_t *lrow, const int32_t *lpart, int64_t n
 template <class T, class Val, class Epi>
 void launch_spmv(const DCsr &A, Val val, const T *x, Epi epi, hipStream_t s) {
   if (A.U.nunits == 0) return;
-  T *parts = (T *)A.U.parts.p;
+  double *parts = A.U.parts.p;
   hipLaunchKernelGGL((k_spmv_units<T, Val, Epi>), dim3((unsigned)A.U.nunits), dim3(SPMV_BLOCK), 0, s,
                      A.rp.p, A.ci.p, val, x, A.U.row0.p, A.U.nrows.p, A.U.koff.p, parts, epi);
   LG_LAUNCH_CHECK();
   if (A.U.nlong) {
-    hipLaunchKernelGGL((k_spmv_long<T, Epi>), dim3(grid_for(A.U.nlong, 64)), dim3(64), 0, s, A.U.lrow.p,
-                       A.U.lpart.p, A.U.nlong, (const T *)parts, epi);
+    hipLaunchKernelGGL((k_spmv_long<Epi>), dim3(grid_for(A.U.nlong, 64)), dim3(64), 0, s, A.U.lrow.p,
+                       A.U.lpart.p, A.U.nlong, (const double *)parts, epi);
     LG_LAUNCH_CHECK();
   }
 }
@@ -152,19 +222,44 @@ const float *vals_of<float>(const DCsr &A) {
   return A.v32.p;
 }
 
+template <class T>
+constexpr int prec_index() {
+  return std::is_same<T, float>::value ? 1 : 0;
+}
+
 template <class T, class Epi>
 void dispatch_spmv(const DCsr &A, const T *x, Epi epi, hipStream_t s) {
+  const auto &SL = A.slices[prec_index<T>()];
+  if (SL.size() > 1) {
+    const size_t ns = SL.size();
+    for (size_t k = 0; k < ns; k++) {
+      const DCsr &P = *SL[k];
+      ValArr<T> val{vals_of<T>(P)};
+      if (k == 0) {
+        launch_spmv<T>(P, val, x, EpiAccSet{A.acc.p}, s);
+      } else if (k + 1 < ns) {
+        launch_spmv<T>(P, val, x, EpiAccAdd{A.acc.p}, s);
+      } else if (std::is_same<T, float>::value && A.d32.p) {
+        launch_spmv<T>(P, val, x, EpiAccFinal<EpiDiag<T, Epi>>{A.acc.p, {A.d32.p, x, epi}}, s);
+      } else {
+        launch_spmv<T>(P, val, x, EpiAccFinal<Epi>{A.acc.p, epi}, s);
+      }
+    }
+    return;
+  }
   const T *v = vals_of<T>(A);
-  if (v) launch_spmv<T>(A, ValArr<T>{v}, x, epi, s);
+  if (std::is_same<T, float>::value && A.d32.p)
+    launch_spmv<T>(A, ValArr<T>{v}, x, EpiDiag<T, Epi>{A.d32.p, x, epi}, s);
+  else if (v) launch_spmv<T>(A, ValArr<T>{v}, x, epi, s);
   else launch_spmv<T>(A, ValOne<T>{}, x, epi, s);
 }
 
 // ------------------------------------------------------------- elementwise
 template <class T>
-__global__ void k_cheb_zero(const T *dinv, const T *b, T *xn, T beta, int64_t n) {
+__global__ void k_cheb_zero(const T *dinv, const T *b, T *xn, double beta, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    xn[i] = beta * dinv[i] * b[i];
+    xn[i] = (T)(beta * (double)dinv[i] * (double)b[i]);
 }
 
 template <class T>
@@ -172,6 +267,27 @@ __global__ void k_prolong_add(const int32_t *agg, int64_t n, const T *xc, T *x) 
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     x[i] += xc[agg[i]];
+}
+
+// Laplacian-form FP32 copy: off-diagonals rounded, diagonal slot 0, and the
+// FP64 diagonal = -(sum of the rounded off-diagonals); one wavefront per row
+__global__ void k_lap32(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
+                        float *v32, double *d32) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (row >= n) return;
+  double s = 0;
+  for (int k = rp[row] + lane; k < rp[row + 1]; k += WAVE) {
+    if (ci[k] == row) {
+      v32[k] = 0.0f;
+    } else {
+      float f = (float)v[k];
+      v32[k] = f;
+      s += (double)f;
+    }
+  }
+  s = wave_sum(s);
+  if (lane == 0) d32[row] = -s;
 }
 
 __global__ void k_d2f(const double *a, float *b, int64_t n) {
@@ -274,7 +390,8 @@ void build_units(DCsr &A, hipStream_t s) {
       continue;
     }
     int64_t r1 = r, acc = 0;
-    while (r1 < n && (rp[r1 + 1] - rp[r1]) <= UNIT_NNZ && acc + (rp[r1 + 1] - rp[r1]) <= UNIT_NNZ) {
+    while (r1 < n && r1 - r < UNIT_ROWS && (rp[r1 + 1] - rp[r1]) <= UNIT_NNZ &&
+           acc + (rp[r1 + 1] - rp[r1]) <= UNIT_NNZ) {
       acc += rp[r1 + 1] - rp[r1];
       r1++;
     }
@@ -302,10 +419,117 @@ void build_units(DCsr &A, hipStream_t s) {
   LG_HIP(hipStreamSynchronize(s));
 }
 
-void make_v32(DCsr &A, hipStream_t s) {
+namespace {
+__device__ __forceinline__ int lower_bound_row(const int32_t *ci, int lo, int hi, int c) {
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (ci[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__global__ void k_slice_count(int64_t n, const int32_t *rp, const int32_t *ci, int c0, int c1,
+                              int32_t *cnt) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int a = lower_bound_row(ci, rp[i], rp[i + 1], c0);
+    int b = lower_bound_row(ci, a, rp[i + 1], c1);
+    cnt[i] = b - a;
+  }
+}
+template <class V>
+__global__ void k_slice_copy(int64_t n, const int32_t *rp, const int32_t *ci, const V *v, int c0,
+                             const int32_t *srp, int32_t *sci, V *sv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int a = lower_bound_row(ci, rp[i], rp[i + 1], c0);
+    int len = srp[i + 1] - srp[i];
+    for (int t = 0; t < len; t++) {
+      sci[srp[i] + t] = ci[a + t];
+      sv[srp[i] + t] = v[a + t];
+    }
+  }
+}
+}  // namespace
+
+void build_slices(DCsr &A, int prec, int64_t width, hipStream_t s) {
+  auto &SL = A.slices[prec];
+  SL.clear();
+  if (width <= 0 || width >= A.ncols) return;
+  const int64_t n = A.n;
+  const int64_t ns = (A.ncols + width - 1) / width;
+  DevBuf<int32_t> cnt(n + 1);
+  for (int64_t k = 0; k < ns; k++) {
+    const int c0 = (int)(k * width), c1 = (int)std::min<int64_t>((k + 1) * width, A.ncols);
+    std::unique_ptr<DCsr> P(new DCsr());
+    P->n = n;
+    P->ncols = A.ncols;
+    hipLaunchKernelGGL(k_slice_count, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, A.rp.p, A.ci.p,
+                       c0, c1, cnt.p);
+    LG_LAUNCH_CHECK();
+    LG_HIP(hipMemsetAsync(cnt.p + n, 0, sizeof(int32_t), s));
+    P->rp.alloc(n + 1);
+    P->nnz = exclusive_scan_i32(cnt.p, P->rp.p, n + 1, s);
+    P->ci.alloc(std::max<int64_t>(P->nnz, 1));
+    if (prec == 0) {
+      P->v.alloc(std::max<int64_t>(P->nnz, 1));
+      hipLaunchKernelGGL(k_slice_copy<double>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, A.rp.p,
+                         A.ci.p, (const double *)A.v.p, c0, (const int32_t *)P->rp.p, P->ci.p, P->v.p);
+    } else {
+      P->v32.alloc(std::max<int64_t>(P->nnz, 1));
+      hipLaunchKernelGGL(k_slice_copy<float>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, A.rp.p,
+                         A.ci.p, (const float *)A.v32.p, c0, (const int32_t *)P->rp.p, P->ci.p,
+                         P->v32.p);
+    }
+    LG_LAUNCH_CHECK();
+    build_units(*P, s);
+    SL.push_back(std::move(P));
+  }
+  A.acc.ensure(std::max<int64_t>(n, 1));
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+double time_spmv(const DCsr &A, int prec, int iters, hipStream_t s) {
+  const int64_t n = A.n;
+  DevBuf<double> x(prec == 0 ? n : 0), y(prec == 0 ? n : 0);
+  DevBuf<float> x32(prec == 1 ? n : 0), y32(prec == 1 ? n : 0);
+  if (prec == 0) fill_f64(x.p, 1.0, n, s);
+  else {
+    DevBuf<double> t(n);
+    fill_f64(t.p, 1.0, n, s);
+    op_d2f(t.p, x32.p, n, s);
+    LG_HIP(hipStreamSynchronize(s));
+  }
+  auto run = [&]() {
+    if (prec == 0) op_spmv<double>(A, x.p, y.p, s);
+    else op_spmv<float>(A, x32.p, y32.p, s);
+  };
+  run();
+  hipEvent_t e0, e1;
+  LG_HIP(hipEventCreate(&e0));
+  LG_HIP(hipEventCreate(&e1));
+  LG_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < iters; i++) run();
+  LG_HIP(hipEventRecord(e1, s));
+  LG_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  LG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / iters;
+}
+
+void make_v32(DCsr &A, hipStream_t s, bool laplacian) {
   if (!A.v.p) return;
-  A.v32.alloc(A.nnz);
-  op_d2f(A.v.p, A.v32.p, A.nnz, s);
+  A.v32.alloc(std::max<int64_t>(A.nnz, 1));
+  if (laplacian) {
+    A.d32.alloc(std::max<int64_t>(A.n, 1));
+    hipLaunchKernelGGL(k_lap32, dim3(grid_for(A.n * WAVE, 256)), dim3(256), 0, s, A.n, A.rp.p, A.ci.p,
+                       A.v.p, A.v32.p, A.d32.p);
+    LG_LAUNCH_CHECK();
+  } else {
+    op_d2f(A.v.p, A.v32.p, A.nnz, s);
+  }
 }
 
 template <class T>
@@ -323,11 +547,11 @@ void op_cheb_step(const DCsr &A, const T *dinv, const T *b, const T *xc, const T
                   double alpha, double beta, hipStream_t s) {
   if (!xc) {
     hipLaunchKernelGGL(k_cheb_zero<T>, dim3(grid_for(A.n, 256, 8192)), dim3(256), 0, s, dinv, b, xn,
-                       (T)beta, A.n);
+                       beta, A.n);
     LG_LAUNCH_CHECK();
     return;
   }
-  dispatch_spmv<T>(A, xc, EpiCheb<T>{dinv, b, xc, xp, xn, (T)alpha, (T)beta}, s);
+  dispatch_spmv<T>(A, xc, EpiCheb<T>{dinv, b, xc, xp, xn, alpha, beta}, s);
 }
 
 template <class T>

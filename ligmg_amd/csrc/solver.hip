@@ -276,7 +276,8 @@ struct Solver {
         build_units(L.R, s);
       }
       if (mixed) {
-        make_v32(L.A, s);
+        make_v32(L.A, s, true);
+        if (L.kind == LIGMG_LEVEL_AGG) set_dinv32_from_lap(L);
         if (L.kind == LIGMG_LEVEL_ELIM) {
           make_v32(L.P, s);
           make_v32(L.R, s);
@@ -286,6 +287,16 @@ struct Solver {
       }
       alloc_work<double>(l);
       if (mixed) alloc_work<float>(l);
+    }
+    // SpMV layout: operators applied in the solve get column slices when
+    // their vector does not fit in L2 (tuned or fixed by o.spmv_slice)
+    const size_t base = (o.top_elim && H[0].kind == LIGMG_LEVEL_ELIM) ? 1 : 0;
+    for (size_t l = base; l < H.size(); l++) {
+      DLevel &L = H[l];
+      if (L.kind == LIGMG_LEVEL_ELIM) continue;
+      if (L.kind == LIGMG_LEVEL_COARSE && !o.kcycle) continue;
+      if (!mixed || l == base) tune_slices(L.A, 0, l);
+      if (mixed) tune_slices(L.A, 1, l);
     }
     const int64_t n0 = H[0].A.n;
     b0.alloc(n0);
@@ -298,6 +309,35 @@ struct Solver {
     lap("solve-prep", H.size(), 0, 0);
     setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
+
+  void tune_slices(DCsr &A, int prec, size_t l) {
+    const int64_t es = prec ? 4 : 8;
+    if (o.spmv_slice == 0) return;
+    if (o.spmv_slice > 0) {
+      if (A.ncols > o.spmv_slice) build_slices(A, prec, o.spmv_slice, s);
+      return;
+    }
+    if (A.n * es <= ((int64_t)2 << 20)) return;
+    double best_t = time_spmv(A, prec, 3, s);
+    int64_t best_w = 0;
+    for (int64_t mb : {1, 2, 4}) {
+      int64_t w = (mb << 20) / es;
+      if (w >= A.ncols) continue;
+      build_slices(A, prec, w, s);
+      double t = time_spmv(A, prec, 3, s);
+      if (t < best_t) {
+        best_t = t;
+        best_w = w;
+      }
+    }
+    build_slices(A, prec, best_w > 0 ? best_w : A.ncols, s);
+    if (o.verbose)
+      fprintf(stderr, "[ligmg] level %zu fp%d spmv: slice width %lld (%.3f ms)\n", l, prec ? 32 : 64,
+              (long long)best_w, best_t);
+  }
+
+  // the FP32 smoother uses the diagonal of the FP32 Laplacian-form operator
+  void set_dinv32_from_lap(DLevel &L);
 
   template <class T>
   void alloc_work(size_t l) {
@@ -570,6 +610,11 @@ const float *Solver::qv<float>(const DLevel &L) {
 }
 
 namespace {
+__global__ void k_dinv_from_d(int64_t n, const double *d, float *dinv32) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dinv32[i] = d[i] != 0 ? (float)(1.0 / d[i]) : 0.0f;
+}
 template <class T>
 __global__ void k_scale(T *p, int64_t n, T a) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -583,6 +628,13 @@ __global__ void k_zero(T *p, int64_t n) {
     p[i] = (T)0;
 }
 }  // namespace
+
+void Solver::set_dinv32_from_lap(DLevel &L) {
+  if (!L.dinv32.p) L.dinv32.alloc(std::max<int64_t>(L.n, 1));
+  hipLaunchKernelGGL(k_dinv_from_d, dim3(grid_for(L.n, 256, 8192)), dim3(256), 0, s, L.n, L.A.d32.p,
+                     L.dinv32.p);
+  LG_LAUNCH_CHECK();
+}
 
 template <class T>
 void Solver::fill_zero(T *p, int64_t n) {
@@ -730,7 +782,7 @@ void ligmg_options_default(ligmg_options *o) {
   o->cheby_eig_iters = 10;
   o->precision = LIGMG_PREC_FP64;
   o->use_graphs = 1;
-  o->reorder = 1;
+  o->spmv_slice = -1;
 }
 
 const char *ligmg_last_error(void) { return lg::g_last_error.c_str(); }
@@ -935,7 +987,7 @@ int ligmg_bench_spmv(ligmg_handle h, int32_t l, int32_t prec, int32_t iters, dou
     hipStream_t s = S.s;
     if (A.U.nunits == 0) lg::build_units(A, s);
     const bool f32 = prec == LIGMG_PREC_MIXED;
-    if (f32 && !A.v32.p) lg::make_v32(A, s);
+    if (f32 && !A.v32.p) lg::make_v32(A, s, true);
     const int64_t n = A.n;
     lg::DevBuf<double> x(f32 ? 0 : n), y(f32 ? 0 : n);
     lg::DevBuf<float> x32(f32 ? n : 0), y32(f32 ? n : 0);
@@ -966,6 +1018,35 @@ int ligmg_bench_spmv(ligmg_handle h, int32_t l, int32_t prec, int32_t iters, dou
     *avg_ms = ms / iters;
     const int64_t sv = f32 ? 4 : 8, sx = f32 ? 4 : 8;
     *bytes = A.nnz * (sv + 4) + (n + 1) * 4 + 2 * n * sx;
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_level_spmv(ligmg_handle h, int32_t l, int32_t prec, const double *x, double *y) {
+  if (!h || l < 0 || l >= (int)h->S.H.size() || !x || !y) return LIGMG_ERR_INVALID;
+  try {
+    lg::Solver &S = h->S;
+    lg::DCsr &A = S.H[l].A;
+    hipStream_t s = S.s;
+    if (A.U.nunits == 0) lg::build_units(A, s);
+    const int64_t n = A.n;
+    lg::DevBuf<double> dx(n), dy(n);
+    lg::to_device(dx.p, x, n, s);
+    if (prec == LIGMG_PREC_MIXED) {
+      if (!A.v32.p) lg::make_v32(A, s, true);
+      lg::DevBuf<float> fx(n), fy(n);
+      lg::op_d2f(dx.p, fx.p, n, s);
+      lg::op_spmv<float>(A, fx.p, fy.p, s);
+      lg::op_f2d(fy.p, dy.p, n, s);
+      lg::to_host(y, dy.p, n, s);
+      LG_HIP(hipStreamSynchronize(s));
+    } else {
+      lg::op_spmv<double>(A, dx.p, dy.p, s);
+      lg::to_host(y, dy.p, n, s);
+      LG_HIP(hipStreamSynchronize(s));
+    }
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return fail(e);

@@ -40,17 +40,31 @@ class Graph:
         return rows[m], self.colind[m].astype(np.int64), -self.vals[m]
 
 
+class _Owner:
+    """Frees a C-allocated ligmg_graph when the last numpy view of it dies."""
+
+    def __init__(self, g):
+        self.g = g
+
+    def __del__(self):
+        _ffi.graphs_lib().ligmg_graph_free(C.byref(self.g))
+
+
+def _view(owner, addr, ctype, count, dtype):
+    buf = (ctype * max(count, 1)).from_address(addr)
+    buf._owner = owner  # the ctypes buffer keeps the allocation alive
+    return np.frombuffer(buf, dtype=dtype, count=count)
+
+
 def _take(g):
+    """Wraps the generator's arrays without copying them."""
     n, nnz = g.n, g.nnz
-    out = Graph(
-        n=int(n),
-        rowptr=np.ctypeslib.as_array(g.rowptr, shape=(n + 1,)).copy(),
-        colind=np.ctypeslib.as_array(g.colind, shape=(nnz,)).copy(),
-        vals=np.ctypeslib.as_array(g.vals, shape=(nnz,)).copy(),
-        comp_ptr=np.ctypeslib.as_array(g.comp_ptr, shape=(g.ncomp + 1,)).copy(),
-    )
-    _ffi.graphs_lib().ligmg_graph_free(C.byref(g))
-    return out
+    owner = _Owner(g)
+    rp = _view(owner, C.addressof(g.rowptr.contents), C.c_int32, n + 1, np.int32)
+    ci = _view(owner, C.addressof(g.colind.contents), C.c_int32, nnz, np.int32)
+    vv = _view(owner, C.addressof(g.vals.contents), C.c_double, nnz, np.float64)
+    cp = np.ctypeslib.as_array(g.comp_ptr, shape=(g.ncomp + 1,)).copy()
+    return Graph(n=int(n), rowptr=rp, colind=ci, vals=vv, comp_ptr=cp)
 
 
 def _call(fn, *args):

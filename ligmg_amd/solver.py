@@ -244,6 +244,15 @@ class Solver:
         _check(self._lib.ligmg_level_cheby(self._h, l, C.byref(lo), C.byref(hi)), "ligmg_level_cheby")
         return lo.value, hi.value
 
+    def level_spmv(self, l, x, precision=_ffi.PREC_FP64):
+        """y = A_l x with the solve-phase kernel of level l (host arrays)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        P = C.POINTER
+        _check(self._lib.ligmg_level_spmv(self._h, l, precision, x.ctypes.data_as(P(C.c_double)),
+                                          y.ctypes.data_as(P(C.c_double))), "ligmg_level_spmv")
+        return y
+
     def print(self):
         """print_hierarchy (multigrid.hpp:265-292)."""
         print(f"{'Level':>5}  {'Size':>12}  {'NNZ':>14}  {'Type':>6}")

@@ -1050,7 +1050,7 @@ void oracle_default_options(ligmg_options *o) {
   o->cheby_eig_iters = 10;
   o->precision = LIGMG_PREC_FP64;
   o->use_graphs = 1;
-  o->reorder = 1;
+  o->spmv_slice = -1;
 }
 
 int oracle_create(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,

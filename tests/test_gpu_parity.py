@@ -216,3 +216,41 @@ def test_asymmetric_pattern_is_rejected():
     rp[1:] -= 1
     with pytest.raises(ligmg_amd.LigmgError):
         ligmg_amd.Solver((rp, ci, v))
+
+
+@pytest.mark.parametrize("width", [0, 700, 2048])
+@pytest.mark.parametrize("case", ["rmat13_unif", "grid_w", "rmat12_pend"])
+def test_level_spmv_matches_scipy(case, width, my_matrix):
+    """The solve-phase SpMV (row-block units, hub-row chunks, column slices,
+    FP32 Laplacian form) against scipy on every level matrix."""
+    g = make_case(case, my_matrix)
+    d = ligmg_amd.Solver(g, ligmg_amd.Options(spmv_slice=width, precision=_ffi.PREC_MIXED))
+    rng = np.random.default_rng(0)
+    for l in range(d.num_levels()):
+        rp, ci, v = d.level_matrix(l)
+        n = len(rp) - 1
+        import scipy.sparse as sp
+        A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+        x = rng.standard_normal(n)
+        y_ref = A @ x
+        scale = abs(A) @ abs(x) + 1e-300
+        y64 = d.level_spmv(l, x, _ffi.PREC_FP64)
+        assert np.all(np.abs(y64 - y_ref) <= 1e-13 * scale), f"level {l} fp64"
+        if d.level_info(l)[0] != _ffi.LEVEL_ELIM or l == 0:
+            y32 = d.level_spmv(l, x, _ffi.PREC_MIXED)
+            assert np.all(np.abs(y32 - y_ref) <= 2e-6 * scale), f"level {l} fp32"
+            # Laplacian form keeps the constant nullspace to FP64 accuracy
+            c = d.level_spmv(l, np.ones(n), _ffi.PREC_MIXED)
+            assert np.all(np.abs(c) <= 1e-12 * (abs(A) @ np.ones(n)) + 1e-300) or l > 0
+
+
+@pytest.mark.parametrize("prec", [_ffi.PREC_FP64, _ffi.PREC_MIXED])
+def test_sliced_solve_matches_unsliced(prec):
+    g = make_case("rmat13_unif")
+    b = graphs.rhs(g.n, 1, 4)
+    kw = {"precision": prec, "outer_ksp_type": _ffi.KSP_FCG}
+    x0 = ligmg_amd.Solver(g, ligmg_amd.Options(spmv_slice=0, **kw)).solve(b)
+    x1 = ligmg_amd.Solver(g, ligmg_amd.Options(spmv_slice=600, **kw)).solve(b)
+    assert x1.rel_residual() <= 1e-8 and abs(x1.iters() - x0.iters()) <= 1
+    a, c = x0.solution, x1.solution
+    assert np.linalg.norm((a - a.mean()) - (c - c.mean())) <= 1e-7 * np.linalg.norm(a - a.mean())
