@@ -149,11 +149,21 @@ int ligmg_solve(ligmg_handle h, const double *b, double *x, int mem_kind,
 int ligmg_solve_batch(ligmg_handle h, int32_t k, const double *B, double *X,
                       int mem_kind, ligmg_info *info, int32_t *iters_out);
 
-/* Adjoint edge gradient for x = L(w)^+ b: for each stored off-diagonal
- * (i,j) with i<j (upper triangle, row order), g = -(lam_i - lam_j)(x_i - x_j).
- * x, lam and g are device pointers; g has one entry per upper edge. */
-int ligmg_edge_grad(ligmg_handle h, const double *x, const double *lam,
-                    double *g);
+/* Edge-weight gradient of a loss of x = L(w)^+ b given the adjoint
+ * lam = L^+ (dloss/dx): g_e = -(lam_i - lam_j)(x_i - x_j) for the m edges
+ * (ei[e], ej[e]).  All arrays are device pointers.  No counterpart in the
+ * reference (which has no differentiation); used by the torch autograd
+ * binding. */
+int ligmg_edge_grad(int64_t m, const int32_t *ei, const int32_t *ej,
+                    const double *x, const double *lam, double *g);
+
+/* Laplacian values on a fixed CSR pattern from edge weights: vals[k] =
+ * -w[eid[k]] off the diagonal (eid[k] = edge of stored entry k, -1 on the
+ * diagonal) and the diagonal = sum of the row's weights in column order.
+ * Device pointers.  Rebuilds L(w) for the autograd binding. */
+int ligmg_assemble_laplacian(int64_t n, const int32_t *rowptr,
+                             const int32_t *colind, const int32_t *eid,
+                             const double *w, double *vals);
 
 /* Solver destructor / PCDestroy_LIGMG (petsc_shim.hpp:31-38). */
 int ligmg_destroy(ligmg_handle h);

@@ -76,7 +76,10 @@ SOLVER_PROTOS = {
                               P(ligmg_info), P(f64), i32]),
     "ligmg_solve_batch": (C.c_int, [HANDLE, i32, C.c_void_p, C.c_void_p, C.c_int,
                                     P(ligmg_info), P(i32)]),
-    "ligmg_edge_grad": (C.c_int, [HANDLE, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ligmg_edge_grad": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "ligmg_assemble_laplacian": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
     "ligmg_destroy": (C.c_int, [HANDLE]),
     "ligmg_last_error": (C.c_char_p, []),
     "ligmg_num_levels": (C.c_int, [HANDLE, P(i32)]),

@@ -1,0 +1,406 @@
+// Block (multi right-hand side) kernels for batched solves.
+//
+// Block vectors are row-major n x kp (kp = number of right-hand sides
+// rounded up to a power of two, at most 64): the kp values of a row are
+// contiguous, so one gather of a column index fetches a whole row of X.  A
+// group of kp lanes handles one matrix row, lane c handling right-hand
+// side c.  Row sums accumulate in FP64.
+#include <algorithm>
+#include <type_traits>
+
+#include "../../include/ligmg_amd.h"
+#include "ops.hpp"
+#include "prims.hpp"
+
+namespace lg {
+
+namespace {
+
+template <class T>
+struct BValOne {
+  __device__ __forceinline__ T operator()(int64_t) const { return (T)1; }
+};
+template <class T>
+struct BValArr {
+  const T *v;
+  __device__ __forceinline__ T operator()(int64_t k) const { return v[k]; }
+};
+
+// ----------------------------------------------------------- epilogues (i, c)
+template <class T>
+struct BStore {
+  T *y;
+  int kp;
+  __device__ __forceinline__ void operator()(int64_t i, int c, double s) const {
+    y[i * kp + c] = (T)s;
+  }
+};
+template <class T>
+struct BResid {
+  const T *b;
+  T *r;
+  int kp;
+  __device__ __forceinline__ void operator()(int64_t i, int c, double s) const {
+    const int64_t o = i * kp + c;
+    r[o] = (T)((double)b[o] - s);
+  }
+};
+template <class T>
+struct BCheb {
+  const T *dinv, *b, *xc, *xp;
+  T *xn;
+  double alpha, beta;
+  int kp;
+  __device__ __forceinline__ void operator()(int64_t i, int c, double s) const {
+    const int64_t o = i * kp + c;
+    double cur = (double)xc[o];
+    double d = xp ? alpha * (cur - (double)xp[o]) : 0.0;
+    xn[o] = (T)(cur + d + beta * (double)dinv[i] * ((double)b[o] - s));
+  }
+};
+template <class T>
+struct BInterp {
+  const T *q, *b;
+  T *x;
+  int kp;
+  __device__ __forceinline__ void operator()(int64_t i, int c, double s) const {
+    const int64_t o = i * kp + c;
+    x[o] = (T)(s + (double)q[i] * (double)b[o]);
+  }
+};
+template <class T, class Epi>
+struct BDiag {  // FP32 Laplacian form: + d[i] x[i, c]
+  const double *d;
+  const T *x;
+  Epi e;
+  int kp;
+  __device__ __forceinline__ void operator()(int64_t i, int c, double s) const {
+    e(i, c, s + d[i] * (double)x[i * kp + c]);
+  }
+};
+
+// One workgroup per unit (the SpMV units).  Complete-row units: group g of
+// kp lanes handles rows g, g + G, ...; long-row chunks: the groups split the
+// chunk's entries and reduce through LDS into kp partial sums.
+template <class T, class Val, class Epi>
+__global__ void __launch_bounds__(256)
+    k_spmm_units(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, Val val,
+                 const T *__restrict__ X, int kp, int lk, const int32_t *__restrict__ urow0,
+                 const int32_t *__restrict__ unrows, const int32_t *__restrict__ ukoff,
+                 double *__restrict__ parts, Epi epi) {
+  const int G = 256 >> lk;
+  const int g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
+  const int u = blockIdx.x;
+  const int row0 = urow0[u], nr = unrows[u], k0 = ukoff[u];
+  if (nr > 0) {
+    for (int r = g; r < nr; r += G) {
+      const int row = row0 + r;
+      const int a = rp[row], b = rp[row + 1];
+      double s = 0;
+      int j = a;
+      for (; j + 1 < b; j += 2) {  // two independent gathers in flight
+        const int c0 = ci[j], c1 = ci[j + 1];
+        const double v0 = (double)val(j), v1 = (double)val(j + 1);
+        const double x0 = (double)X[(int64_t)c0 * kp + c], x1 = (double)X[(int64_t)c1 * kp + c];
+        s += v0 * x0;
+        s += v1 * x1;
+      }
+      if (j < b) s += (double)val(j) * (double)X[(int64_t)ci[j] * kp + c];
+      epi(row, c, s);
+    }
+  } else {
+    __shared__ double red[256];
+    const int row = row0;
+    const int cnt = min(k0 + UNIT_NNZ, rp[row + 1]) - k0;
+    double s = 0;
+    for (int t = g; t < cnt; t += G) {
+      const int j = k0 + t;
+      s += (double)val(j) * (double)X[(int64_t)ci[j] * kp + c];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (g == 0) {
+      for (int gg = 1; gg < G; gg++) s += red[gg * kp + c];
+      parts[(int64_t)(-nr) * kp + c] = s;
+    }
+  }
+}
+
+template <class Epi>
+__global__ void k_spmm_long(const int32_t *lrow, const int32_t *lpart, int64_t nlong, int kp,
+                            const double *parts, Epi epi) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nlong * kp) return;
+  const int64_t j = t / kp;
+  const int c = (int)(t % kp);
+  double s = 0;
+  for (int p = lpart[j]; p < lpart[j + 1]; p++) s += parts[(int64_t)p * kp + c];
+  epi(lrow[j], c, s);
+}
+
+int log2i(int k) {
+  int l = 0;
+  while ((1 << l) < k) l++;
+  return l;
+}
+
+template <class T>
+const T *bvals(const DCsr &A);
+template <>
+const double *bvals<double>(const DCsr &A) {
+  return A.v.p;
+}
+template <>
+const float *bvals<float>(const DCsr &A) {
+  return A.v32.p;
+}
+
+template <class T, class Val, class Epi>
+void launch_spmm(const DCsr &A, Val val, const T *X, int kp, Epi epi, DevBuf<double> &parts,
+                 hipStream_t s) {
+  if (A.U.nunits == 0) return;
+  LG_CHECK(parts.n >= (size_t)std::max<int64_t>(A.U.nparts, 1) * kp, LIGMG_ERR_INVALID,
+           "spmm partial-sum buffer too small");
+  hipLaunchKernelGGL((k_spmm_units<T, Val, Epi>), dim3((unsigned)A.U.nunits), dim3(256), 0, s, A.rp.p,
+                     A.ci.p, val, X, kp, log2i(kp), A.U.row0.p, A.U.nrows.p, A.U.koff.p, parts.p, epi);
+  LG_LAUNCH_CHECK();
+  if (A.U.nlong) {
+    hipLaunchKernelGGL((k_spmm_long<Epi>), dim3(grid_for(A.U.nlong * kp, 256)), dim3(256), 0, s,
+                       A.U.lrow.p, A.U.lpart.p, A.U.nlong, kp, (const double *)parts.p, epi);
+    LG_LAUNCH_CHECK();
+  }
+}
+
+template <class T, class Epi>
+void dispatch_spmm(const DCsr &A, const T *X, int kp, Epi epi, DevBuf<double> &parts, hipStream_t s) {
+  const T *v = bvals<T>(A);
+  if (std::is_same<T, float>::value && A.d32.p)
+    launch_spmm<T>(A, BValArr<T>{v}, X, kp, BDiag<T, Epi>{A.d32.p, X, epi, kp}, parts, s);
+  else if (v) launch_spmm<T>(A, BValArr<T>{v}, X, kp, epi, parts, s);
+  else launch_spmm<T>(A, BValOne<T>{}, X, kp, epi, parts, s);
+}
+
+// ------------------------------------------------------------- elementwise
+template <class T>
+__global__ void k_bcheb_zero(const T *dinv, const T *b, T *xn, double beta, int64_t n, int lk) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (n << lk);
+       t += (int64_t)gridDim.x * blockDim.x)
+    xn[t] = (T)(beta * (double)dinv[t >> lk] * (double)b[t]);
+}
+
+template <class T>
+__global__ void k_bprolong(const int32_t *agg, int64_t n, int kp, int lk, const T *xc, T *x) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (n << lk);
+       t += (int64_t)gridDim.x * blockDim.x)
+    x[t] += xc[(int64_t)agg[t >> lk] * kp + (t & (kp - 1))];
+}
+
+// column-major (ld n) <-> row-major n x kp (padding columns zero)
+__global__ void k_to_block(const double *B, int64_t n, int k, int kp, double *X) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * kp;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / kp;
+    const int c = (int)(t % kp);
+    X[t] = c < k ? B[(int64_t)c * n + i] : 0.0;
+  }
+}
+__global__ void k_from_block(const double *X, int64_t n, int k, int kp, double *B) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * k;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t / n, i = t % n;
+    B[t] = X[i * kp + c];
+  }
+}
+
+// ------------------------------------------------------- coarse dense apply
+// Y (n x kp) = Pm (n x n, FP64, row-major) * X (n x kp) with FP64 MFMA
+// 16x16x4 tiles: one wavefront per 16 x 16 output tile.  Operand layout of
+// v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16] of
+// the 16x4 / 4x16 slices and holds D[4*(l/16) + v][l%16], v = 0..3.
+template <class T>
+__global__ void __launch_bounds__(256) k_dense_mfma(const double *__restrict__ Pm, int n,
+                                                    const T *__restrict__ X, int kp,
+                                                    T *__restrict__ Y) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const int tiles_j = kp / 16;
+  const int ti = wave / tiles_j, tj = wave % tiles_j;
+  const int i0 = ti * 16, j0 = tj * 16;
+  if (i0 >= n) return;
+  const int li = lane & 15, lk = lane >> 4;
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  v4d acc = {0, 0, 0, 0};
+  const int arow = i0 + li;
+  for (int k0 = 0; k0 < n; k0 += 4) {
+    const int kk = k0 + lk;
+    const double a = (arow < n && kk < n) ? Pm[(int64_t)arow * n + kk] : 0.0;
+    const double b = (kk < n) ? (double)X[(int64_t)kk * kp + j0 + li] : 0.0;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int r = i0 + 4 * lk + v;
+    if (r < n) Y[(int64_t)r * kp + j0 + li] = (T)acc[v];
+  }
+}
+
+template <class T>
+__global__ void k_dense_small(const double *__restrict__ Pm, int64_t n, const T *__restrict__ X,
+                              int kp, T *__restrict__ Y) {
+  // kp < 16: one thread per output entry
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * kp;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / kp;
+    const int c = (int)(t % kp);
+    double s = 0;
+    for (int64_t j = 0; j < n; j++) s += Pm[i * n + j] * (double)X[j * kp + c];
+    Y[t] = (T)s;
+  }
+}
+
+// -------------------------------------------------- per-(block, column) dots
+// part[(s * np + j) * kp + c] for segment s; out[(blk * np + j) * kp + c]
+template <class T>
+__global__ void __launch_bounds__(256) k_bred1(int np, int kp, int lk, const T *a0, const T *b0,
+                                               const T *a1, const T *b1, const int64_t *lo,
+                                               const int64_t *hi, double *part) {
+  const int s = blockIdx.x;
+  const int G = 256 >> lk;
+  const int g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
+  double acc0 = 0, acc1 = 0;
+  for (int64_t i = lo[s] + g; i < hi[s]; i += G) {
+    const int64_t o = i * kp + c;
+    acc0 += (double)a0[o] * (b0 ? (double)b0[o] : 1.0);
+    if (np > 1) acc1 += (double)a1[o] * (b1 ? (double)b1[o] : 1.0);
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = acc0;
+  red[1][threadIdx.x] = acc1;
+  __syncthreads();
+  if (g == 0) {
+    for (int j = 0; j < np; j++) {
+      double t = 0;
+      for (int gg = 0; gg < G; gg++) t += red[j][gg * kp + c];
+      part[((int64_t)s * np + j) * kp + c] = t;
+    }
+  }
+}
+
+__global__ void k_bred2(int np, int nb, int kp, const int32_t *bseg, const double *part,
+                        double *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * kp) return;
+  const int blk = t / kp, c = t % kp;
+  for (int j = 0; j < np; j++) {
+    double s = 0;
+    for (int g = bseg[blk]; g < bseg[blk + 1]; g++) s += part[((int64_t)g * np + j) * kp + c];
+    out[((int64_t)blk * np + j) * kp + c] = s;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_bsub_mean(T *a, int kp, int lk, const int64_t *lo,
+                                                   const int64_t *hi, const int32_t *segblk,
+                                                   const int64_t *bptr, const double *sums) {
+  const int s = blockIdx.x;
+  const int blk = segblk[s];
+  const int G = 256 >> lk;
+  const int g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
+  const T m = (T)(sums[(int64_t)blk * kp + c] / (double)(bptr[blk + 1] - bptr[blk]));
+  for (int64_t i = lo[s] + g; i < hi[s]; i += G) a[i * kp + c] -= m;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+template <class T>
+void op_bspmv(const DCsr &A, const T *X, T *Y, int kp, DevBuf<double> &parts, hipStream_t s) {
+  dispatch_spmm<T>(A, X, kp, BStore<T>{Y, kp}, parts, s);
+}
+template <class T>
+void op_bresidual(const DCsr &A, const T *X, const T *B, T *R, int kp, DevBuf<double> &parts,
+                  hipStream_t s) {
+  dispatch_spmm<T>(A, X, kp, BResid<T>{B, R, kp}, parts, s);
+}
+template <class T>
+void op_bcheb_step(const DCsr &A, const T *dinv, const T *B, const T *Xc, const T *Xp, T *Xn,
+                   double alpha, double beta, int kp, DevBuf<double> &parts, hipStream_t s) {
+  if (!Xc) {
+    hipLaunchKernelGGL(k_bcheb_zero<T>, dim3(grid_for(A.n * kp, 256, 16384)), dim3(256), 0, s, dinv, B,
+                       Xn, beta, A.n, log2i(kp));
+    LG_LAUNCH_CHECK();
+    return;
+  }
+  dispatch_spmm<T>(A, Xc, kp, BCheb<T>{dinv, B, Xc, Xp, Xn, alpha, beta, kp}, parts, s);
+}
+template <class T>
+void op_binterp(const DCsr &P, const T *q, const T *Xc, const T *B, T *X, int kp,
+                DevBuf<double> &parts, hipStream_t s) {
+  dispatch_spmm<T>(P, Xc, kp, BInterp<T>{q, B, X, kp}, parts, s);
+}
+template <class T>
+void op_bprolong_add(const int32_t *agg, int64_t n, const T *Xc, T *X, int kp, hipStream_t s) {
+  hipLaunchKernelGGL(k_bprolong<T>, dim3(grid_for(n * kp, 256, 16384)), dim3(256), 0, s, agg, n, kp,
+                     log2i(kp), Xc, X);
+  LG_LAUNCH_CHECK();
+}
+template <class T>
+void op_bdense_apply(const double *Pm, int64_t n, const T *X, T *Y, int kp, hipStream_t s) {
+  if (kp >= 16) {
+    const int64_t waves = ((n + 15) / 16) * (kp / 16);
+    hipLaunchKernelGGL(k_dense_mfma<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, Pm, (int)n,
+                       X, kp, Y);
+  } else {
+    hipLaunchKernelGGL(k_dense_small<T>, dim3(grid_for(n * kp, 256, 16384)), dim3(256), 0, s, Pm, n, X,
+                       kp, Y);
+  }
+  LG_LAUNCH_CHECK();
+}
+
+void op_to_block(const double *B, int64_t n, int k, int kp, double *X, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_block, dim3(grid_for(n * kp, 256, 16384)), dim3(256), 0, s, B, n, k, kp, X);
+  LG_LAUNCH_CHECK();
+}
+void op_from_block(const double *X, int64_t n, int k, int kp, double *B, hipStream_t s) {
+  hipLaunchKernelGGL(k_from_block, dim3(grid_for(n * k, 256, 16384)), dim3(256), 0, s, X, n, k, kp, B);
+  LG_LAUNCH_CHECK();
+}
+
+template <class T>
+void RedPlan::bdots(int np, int kp, const T *a0, const T *b0, const T *a1, const T *b1, double *out,
+                    DevBuf<double> &bpart, hipStream_t s) const {
+  if (nseg == 0) return;
+  LG_CHECK(bpart.n >= (size_t)nseg * np * kp, LIGMG_ERR_INVALID, "block reduction buffer too small");
+  hipLaunchKernelGGL(k_bred1<T>, dim3(nseg), dim3(256), 0, s, np, kp, log2i(kp), a0, b0, a1, b1, seg_lo.p,
+                     seg_hi.p, bpart.p);
+  LG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bred2, dim3((nb * kp + 63) / 64), dim3(64), 0, s, np, nb, kp, bseg.p, bpart.p, out);
+  LG_LAUNCH_CHECK();
+}
+
+template <class T>
+void op_bsub_block_mean(T *a, int kp, const RedPlan &P, const double *sums, hipStream_t s) {
+  if (P.nseg == 0) return;
+  hipLaunchKernelGGL(k_bsub_mean<T>, dim3(P.nseg), dim3(256), 0, s, a, kp, log2i(kp), P.seg_lo.p,
+                     P.seg_hi.p, P.bseg.p + P.nb + 1, P.bptr.p, sums);
+  LG_LAUNCH_CHECK();
+}
+
+#define LG_BINST(T)                                                                                  \
+  template void op_bspmv<T>(const DCsr &, const T *, T *, int, DevBuf<double> &, hipStream_t);       \
+  template void op_bresidual<T>(const DCsr &, const T *, const T *, T *, int, DevBuf<double> &,     \
+                                hipStream_t);                                                        \
+  template void op_bcheb_step<T>(const DCsr &, const T *, const T *, const T *, const T *, T *,     \
+                                 double, double, int, DevBuf<double> &, hipStream_t);                \
+  template void op_binterp<T>(const DCsr &, const T *, const T *, const T *, T *, int,              \
+                              DevBuf<double> &, hipStream_t);                                        \
+  template void op_bprolong_add<T>(const int32_t *, int64_t, const T *, T *, int, hipStream_t);      \
+  template void op_bdense_apply<T>(const double *, int64_t, const T *, T *, int, hipStream_t);       \
+  template void RedPlan::bdots<T>(int, int, const T *, const T *, const T *, const T *, double *,    \
+                                  DevBuf<double> &, hipStream_t) const;                              \
+  template void op_bsub_block_mean<T>(T *, int, const RedPlan &, const double *, hipStream_t);
+LG_BINST(double)
+LG_BINST(float)
+#undef LG_BINST
+
+}  // namespace lg

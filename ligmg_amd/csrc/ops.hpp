@@ -61,9 +61,37 @@ struct RedPlan {
   void dots(int np, const T *const *a, const T *const *b, double *out, hipStream_t s) const;
   // out[c] = sum over block c of a[i]
   template <class T> void sums(const T *a, double *out, hipStream_t s) const;
+  // block vectors (n x kp row-major): out[(blk * np + j) * kp + c] = sum over
+  // block blk of a_j[i, c] * b_j[i, c]  (np <= 2, b_j == nullptr: 1)
+  template <class T>
+  void bdots(int np, int kp, const T *a0, const T *b0, const T *a1, const T *b1, double *out,
+             DevBuf<double> &bpart, hipStream_t s) const;
 };
 
 // a -= mean of a over each block, means taken from sums[c] (device)
 template <class T> void op_sub_block_mean(T *a, const RedPlan &P, const double *sums, hipStream_t s);
+
+// ---- block (k right-hand sides) operators, batch.hip.  Block vectors are
+// row-major n x kp, kp a power of two <= 64.
+template <class T>
+void op_bspmv(const DCsr &A, const T *X, T *Y, int kp, DevBuf<double> &parts, hipStream_t s);
+template <class T>
+void op_bresidual(const DCsr &A, const T *X, const T *B, T *R, int kp, DevBuf<double> &parts,
+                  hipStream_t s);
+template <class T>
+void op_bcheb_step(const DCsr &A, const T *dinv, const T *B, const T *Xc, const T *Xp, T *Xn,
+                   double alpha, double beta, int kp, DevBuf<double> &parts, hipStream_t s);
+template <class T>
+void op_binterp(const DCsr &P, const T *q, const T *Xc, const T *B, T *X, int kp,
+                DevBuf<double> &parts, hipStream_t s);
+template <class T>
+void op_bprolong_add(const int32_t *agg, int64_t n, const T *Xc, T *X, int kp, hipStream_t s);
+// Y = Pm X with FP64 MFMA tiles (kp >= 16)
+template <class T>
+void op_bdense_apply(const double *Pm, int64_t n, const T *X, T *Y, int kp, hipStream_t s);
+template <class T>
+void op_bsub_block_mean(T *a, int kp, const RedPlan &P, const double *sums, hipStream_t s);
+void op_to_block(const double *B, int64_t n, int k, int kp, double *X, hipStream_t s);
+void op_from_block(const double *X, int64_t n, int k, int kp, double *B, hipStream_t s);
 
 }  // namespace lg

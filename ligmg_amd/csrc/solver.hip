@@ -94,6 +94,42 @@ __global__ void __launch_bounds__(256) k_kr_pupdate(T *p, const T *z, const int6
   for (int64_t i = lo[s] + threadIdx.x; i < hi[s]; i += 256) p[i] = z[i] + b * p[i];
 }
 
+// batch: dot results out[(blk * np + j) * kp + c] -> scalar sets of (blk, c)
+__global__ void k_kr_bstore(int nb, int kp, int np, const double *out, int s0, int s1, double *sc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * kp) return;
+  const int blk = t / kp, c = t % kp;
+  sc[t * SC_N + s0] = out[((int64_t)blk * np) * kp + c];
+  if (np > 1) sc[t * SC_N + s1] = out[((int64_t)blk * np + 1) * kp + c];
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_kr_bupdate(T *x, T *r, const T *p, const T *q, int kp,
+                                                    int lk, const int64_t *lo, const int64_t *hi,
+                                                    const int32_t *segblk, const double *sc) {
+  const int s = blockIdx.x;
+  const int G = 256 >> lk, g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
+  const double a = sc[((int64_t)segblk[s] * kp + c) * SC_N + SC_ALPHA];
+  if (a == 0.0) return;
+  const T at = (T)a;
+  for (int64_t i = lo[s] + g; i < hi[s]; i += G) {
+    const int64_t o = i * kp + c;
+    x[o] += at * p[o];
+    r[o] -= at * q[o];
+  }
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_kr_bpupdate(T *p, const T *z, int kp, int lk,
+                                                     const int64_t *lo, const int64_t *hi,
+                                                     const int32_t *segblk, const double *sc) {
+  const int s = blockIdx.x;
+  const int G = 256 >> lk, g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
+  const T b = (T)sc[((int64_t)segblk[s] * kp + c) * SC_N + SC_BETA];
+  for (int64_t i = lo[s] + g; i < hi[s]; i += G) {
+    const int64_t o = i * kp + c;
+    p[o] = z[o] + b * p[o];
+  }
+}
+
 template <class T>
 struct KWork {
   DevBuf<T> r, z, p, q, x;
@@ -134,6 +170,19 @@ struct LevelWork {
   KWork<T> kw;
 };
 
+// Per-level block work vectors (n x kp) of one precision.
+template <class T>
+struct BLevelWork {
+  DevBuf<T> B[3], R, b;
+  KWork<T> kw;  // K-cycle: vectors n*kp, scalars nb*kp sets
+};
+
+inline int ilog2(int k) {
+  int l = 0;
+  while ((1 << l) < k) l++;
+  return l;
+}
+
 struct Solver {
   ligmg_options o;
   hipStream_t s = nullptr;
@@ -141,13 +190,25 @@ struct Solver {
   std::vector<RedPlan> red;
   std::vector<LevelWork<double>> W64;
   std::vector<LevelWork<float>> W32;
+  std::vector<BLevelWork<double>> BW64;
+  std::vector<BLevelWork<float>> BW32;
+  int bkp = 0;  // block width the batch buffers are sized for
+  DevBuf<double> bparts, bpart_red, bsc_out;
+  KWork<double> bouter;
   std::vector<std::vector<double>> al_pre, be_pre, al_post, be_post;
   KWork<double> outer;
   DevBuf<double> b0, sol, tmp0, dsc;
   double setup_time = 0;
   bool mixed = false;
+  // captured preconditioner applications, keyed by (level, r, z)
+  struct GraphKey {
+    size_t lvl;
+    const void *r, *z;
+  };
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
 
   ~Solver() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
     if (s) (void)hipStreamDestroy(s);
   }
 
@@ -554,8 +615,37 @@ struct Solver {
     op_sub_block_mean<T>(z, R, dsc.p, s);
   }
 
-  // outer preconditioner in FP64, cycle in FP32 when mixed
+  // outer preconditioner in FP64, cycle in FP32 when mixed.  With
+  // o.use_graphs the whole application (hundreds of launches on the coarse
+  // levels) is captured once per (level, r, z) and replayed as a hipGraph;
+  // it contains no host synchronisation and no memset nodes.
   void precond_outer(size_t lvl, const double *r, double *z) {
+    if (o.use_graphs) {
+      for (auto &g : graphs)
+        if (g.first.lvl == lvl && g.first.r == r && g.first.z == z) {
+          LG_HIP(hipGraphLaunch(g.second, s));
+          return;
+        }
+      hipGraph_t graph;
+      LG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        precond_outer_eager(lvl, r, z);
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        throw;
+      }
+      LG_HIP(hipStreamEndCapture(s, &graph));
+      hipGraphExec_t exec;
+      LG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      graphs.push_back({{lvl, r, z}, exec});
+      LG_HIP(hipGraphLaunch(exec, s));
+      return;
+    }
+    precond_outer_eager(lvl, r, z);
+  }
+
+  void precond_outer_eager(size_t lvl, const double *r, double *z) {
     const int64_t n = H[lvl].A.n;
     if (mixed) {
       LevelWork<float> &w = W32[lvl];
@@ -580,9 +670,307 @@ struct Solver {
   template <class T>
   const T *qv(const DLevel &L);
 
+  // ---------------------------------------------------------------- batch
+  template <class T>
+  BLevelWork<T> &bwork(size_t l);
+
+  template <class T>
+  void balloc(int kp) {
+    auto &W = [&]() -> std::vector<BLevelWork<T>> & {
+      if constexpr (std::is_same<T, double>::value) return BW64;
+      else return BW32;
+    }();
+    W.clear();
+    W.resize(H.size());
+    for (size_t l = 0; l < H.size(); l++) {
+      const DLevel &L = H[l];
+      const int64_t n = std::max<int64_t>(L.A.n, 1);
+      for (auto &B : W[l].B) B.alloc(n * kp);
+      if (L.kind == LIGMG_LEVEL_AGG) W[l].R.alloc(n * kp);
+      W[l].b.alloc(n * kp);
+      if (o.kcycle && l > 0 && L.kind != LIGMG_LEVEL_ELIM) W[l].kw.alloc(n * kp, red[l].nb * kp);
+    }
+  }
+
+  void batch_buffers(int kp) {
+    if (bkp == kp) return;
+    balloc<double>(kp);
+    if (mixed) balloc<float>(kp);
+    // scratch sized once for every operator, so no buffer is reallocated
+    // while kernels that use it are queued
+    int64_t maxparts = 1, maxseg = 1;
+    for (size_t l = 0; l < H.size(); l++) {
+      const DLevel &L = H[l];
+      for (const DCsr *M : {&L.A, &L.M, &L.P, &L.R}) maxparts = std::max(maxparts, M->U.nparts);
+      maxseg = std::max<int64_t>(maxseg, red[l].nseg);
+    }
+    bparts.alloc((size_t)maxparts * kp);
+    bpart_red.alloc((size_t)maxseg * 2 * kp);
+    bkp = kp;
+  }
+
+  // block version of cycle(): same buffer discipline
+  template <class T>
+  const T *bcycle(size_t l, const T *b, int kp) {
+    DLevel &L = H[l];
+    BLevelWork<T> &w = bwork<T>(l);
+    if (L.kind == LIGMG_LEVEL_COARSE) {
+      op_bdense_apply<T>(L.pinv.p, L.n, b, w.B[0].p, kp, s);
+      return w.B[0].p;
+    }
+    if (L.kind == LIGMG_LEVEL_ELIM) {
+      BLevelWork<T> &wn = bwork<T>(l + 1);
+      op_bspmv<T>(L.R, b, wn.b.p, kp, bparts, s);
+      const T *xc = brecurse<T>(l, wn.b.p, kp);
+      op_binterp<T>(L.P, qv<T>(L), xc, b, w.B[0].p, kp, bparts, s);
+      return w.B[0].p;
+    }
+    const T *dinv = dinvv<T>(L);
+    int cur;
+    const int m = o.pre_smooth;
+    if (o.smoother_type == LIGMG_SMOOTHER_JACOBI || m <= 0) {
+      fill_zero<T>(w.B[0].p, L.n * kp);
+      cur = 0;
+      if (m > 0) {
+        op_bcheb_step<T>(L.A, dinv, b, w.B[0].p, nullptr, w.B[1].p, 0.0, 2.0 / 3.0, kp, bparts, s);
+        cur = 1;
+      }
+    } else {
+      op_bcheb_step<T>(L.A, dinv, b, nullptr, nullptr, w.B[1].p, 0.0, be_pre[l][0], kp, bparts, s);
+      for (int k = 1; k < m; k++) {
+        const T *xp = (k == 1) ? nullptr : w.B[(k - 1) % 3].p;
+        op_bcheb_step<T>(L.A, dinv, b, w.B[k % 3].p, xp, w.B[(k + 1) % 3].p, al_pre[l][k],
+                         be_pre[l][k], kp, bparts, s);
+      }
+      cur = m % 3;
+    }
+    BLevelWork<T> &wn = bwork<T>(l + 1);
+    op_bresidual<T>(L.A, w.B[cur].p, b, w.R.p, kp, bparts, s);
+    op_bspmv<T>(L.M, w.R.p, wn.b.p, kp, bparts, s);
+    const T *xc = brecurse<T>(l, wn.b.p, kp);
+    op_bprolong_add<T>(L.agg.p, L.n, xc, w.B[cur].p, kp, s);
+    const int mp = o.post_smooth;
+    if (o.smoother_type == LIGMG_SMOOTHER_JACOBI) {
+      if (mp > 0) {
+        int s1 = (cur + 1) % 3;
+        op_bcheb_step<T>(L.A, dinv, b, w.B[cur].p, nullptr, w.B[s1].p, 0.0, 2.0 / 3.0, kp, bparts, s);
+        cur = s1;
+      }
+    } else {
+      for (int k = 0; k < mp; k++) {
+        const T *xp = (k == 0) ? nullptr : w.B[(cur + k - 1) % 3].p;
+        op_bcheb_step<T>(L.A, dinv, b, w.B[(cur + k) % 3].p, xp, w.B[(cur + k + 1) % 3].p,
+                         al_post[l][k], be_post[l][k], kp, bparts, s);
+      }
+      cur = (cur + mp) % 3;
+    }
+    if (o.constant_correction) scale_vec<T>(w.B[cur].p, L.n * kp, 4.0 / 3.0);
+    return w.B[cur].p;
+  }
+
+  template <class T>
+  const T *brecurse(size_t l, const T *bc, int kp) {
+    const DLevel &N = H[l + 1];
+    if (!o.kcycle || N.kind == LIGMG_LEVEL_ELIM) return bcycle<T>(l + 1, bc, kp);
+    BLevelWork<T> &wn = bwork<T>(l + 1);
+    KWork<T> &K = wn.kw;
+    fill_zero<T>(K.x.p, N.A.n * kp);
+    bkrylov<T>(l + 1, N.A, K.x.p, bc, K, kp, o.kcycle_ksp_type, o.tol, o.kcycle_iters, false, nullptr,
+               nullptr, false);
+    return K.x.p;
+  }
+
+  template <class T>
+  void bprec(size_t lvl, const T *r, T *z, int kp, bool outer_prec) {
+    const int64_t n = H[lvl].A.n;
+    if constexpr (std::is_same<T, double>::value) {
+      if (outer_prec && mixed) {
+        BLevelWork<float> &w = BW32[lvl];
+        op_d2f(r, w.b.p, n * kp, s);
+        const float *c = bcycle<float>(lvl, w.b.p, kp);
+        op_f2d(c, z, n * kp, s);
+      } else {
+        d2d(z, bcycle<T>(lvl, r, kp), n * kp, s);
+      }
+    } else {
+      d2d(z, bcycle<T>(lvl, r, kp), n * kp, s);
+    }
+    const RedPlan &R = red[lvl];
+    R.bdots<T>(1, kp, z, nullptr, nullptr, nullptr, bsc_out.p, bpart_red, s);
+    op_bsub_block_mean<T>(z, kp, R, bsc_out.p, s);
+  }
+
+  template <class T>
+  void bdot(const RedPlan &R, int kp, const T *a0, const T *b0, const T *a1, const T *b1,
+            KWork<T> &K, int s0, int s1) {
+    const int np = a1 ? 2 : 1;
+    R.bdots<T>(np, kp, a0, b0, a1, b1, K.out.p, bpart_red, s);
+    hipLaunchKernelGGL(k_kr_bstore, dim3((R.nb * kp + 63) / 64), dim3(64), 0, s, R.nb, kp, np, K.out.p,
+                       s0, s1, K.sc.p);
+  }
+
+  // per-(block, column) CG / FCG on block vectors; iters_col (host, may be
+  // null) receives for every scalar set the iteration it converged at
+  template <class T>
+  int bkrylov(size_t lvl, const DCsr &A, T *x, const T *b, KWork<T> &K, int kp, int type,
+              double rtol, int maxit, bool host_check, std::vector<int> *iters_set, int *status,
+              bool outer_prec) {
+    const RedPlan &R = red[lvl];
+    const int ns = R.nb * kp, lk = ilog2(kp);
+    const bool fcg = type == LIGMG_KSP_FCG;
+    auto grid1 = dim3((ns + 63) / 64);
+    hipLaunchKernelGGL(k_kr_init, grid1, dim3(64), 0, s, ns, K.sc.p);
+    op_bresidual<T>(A, x, b, K.r.p, kp, bparts, s);
+    bdot<T>(R, kp, b, b, K.r.p, K.r.p, K, SC_BN2, SC_RN2);
+    hipLaunchKernelGGL(k_kr_check, grid1, dim3(64), 0, s, ns, rtol, K.sc.p);
+    if (status) *status = LIGMG_CONVERGED;
+    std::vector<double> hsc;
+    std::vector<char> was_active(ns, 0);
+    if (iters_set) iters_set->assign(ns, 0);
+    if (host_check) {
+      hsc = download(K.sc.p, (size_t)ns * SC_N, s);
+      for (int t = 0; t < ns; t++) was_active[t] = hsc[t * SC_N + SC_ACTIVE] != 0;
+      if (!any_active(hsc, ns)) return 0;
+    }
+    const int64_t nk = A.n * kp;
+    bprec<T>(lvl, K.r.p, K.z.p, kp, outer_prec);
+    d2d(K.p.p, K.z.p, nk, s);
+    bdot<T>(R, kp, K.r.p, K.z.p, (const T *)nullptr, (const T *)nullptr, K, SC_GAMMA, SC_GAMMA);
+    int it = 0;
+    while (it < maxit) {
+      op_bspmv<T>(A, K.p.p, K.q.p, kp, bparts, s);
+      if (fcg) bdot<T>(R, kp, K.p.p, K.q.p, K.p.p, K.r.p, K, SC_DELTA, SC_T1);
+      else bdot<T>(R, kp, K.p.p, K.q.p, (const T *)nullptr, (const T *)nullptr, K, SC_DELTA, SC_DELTA);
+      hipLaunchKernelGGL(k_kr_alpha, grid1, dim3(64), 0, s, ns, fcg ? 1 : 0, K.sc.p);
+      hipLaunchKernelGGL(k_kr_bupdate<T>, dim3(R.nseg), dim3(256), 0, s, x, K.r.p, K.p.p, K.q.p, kp, lk,
+                         R.seg_lo.p, R.seg_hi.p, R.bseg.p + R.nb + 1, K.sc.p);
+      LG_LAUNCH_CHECK();
+      it++;
+      bdot<T>(R, kp, K.r.p, K.r.p, (const T *)nullptr, (const T *)nullptr, K, SC_RN2, SC_RN2);
+      hipLaunchKernelGGL(k_kr_check, grid1, dim3(64), 0, s, ns, rtol, K.sc.p);
+      if (host_check) {
+        hsc = download(K.sc.p, (size_t)ns * SC_N, s);
+        bool nan = false;
+        for (int t = 0; t < ns; t++) {
+          double r2 = hsc[t * SC_N + SC_RN2];
+          if (r2 != r2) nan = true;
+          const bool act = hsc[t * SC_N + SC_ACTIVE] != 0;
+          if (iters_set && (was_active[t] || act)) (*iters_set)[t] = it;
+          was_active[t] = act;
+        }
+        if (nan) {
+          if (status) *status = LIGMG_DIVERGED_NAN;
+          return it;
+        }
+        if (!any_active(hsc, ns)) return it;
+      }
+      if (it >= maxit) break;
+      bprec<T>(lvl, K.r.p, K.z.p, kp, outer_prec);
+      if (fcg) bdot<T>(R, kp, K.z.p, K.q.p, (const T *)nullptr, (const T *)nullptr, K, SC_T1, SC_T1);
+      else bdot<T>(R, kp, K.r.p, K.z.p, (const T *)nullptr, (const T *)nullptr, K, SC_GN, SC_GN);
+      hipLaunchKernelGGL(k_kr_beta, grid1, dim3(64), 0, s, ns, fcg ? 1 : 0, K.sc.p);
+      hipLaunchKernelGGL(k_kr_bpupdate<T>, dim3(R.nseg), dim3(256), 0, s, K.p.p, K.z.p, kp, lk,
+                         R.seg_lo.p, R.seg_hi.p, R.bseg.p + R.nb + 1, K.sc.p);
+      LG_LAUNCH_CHECK();
+    }
+    if (host_check && status) *status = LIGMG_DIVERGED_ITS;
+    return it;
+  }
+
+  int solve_batch(int k, const double *B, double *X, int mem, ligmg_info *info, int32_t *iters_out);
+
   // ---------------------------------------------------------------- solve
   int solve(const double *b, double *x, int mem, ligmg_info *info, double *hist, int cap);
 };
+
+template <>
+BLevelWork<double> &Solver::bwork<double>(size_t l) {
+  return BW64[l];
+}
+template <>
+BLevelWork<float> &Solver::bwork<float>(size_t l) {
+  return BW32[l];
+}
+
+int Solver::solve_batch(int k, const double *Bin, double *Xout, int mem, ligmg_info *info,
+                        int32_t *iters_out) {
+  LG_CHECK(k >= 1 && k <= 64, LIGMG_ERR_INVALID, "solve_batch: k must be in [1, 64]");
+  int kp = 1;
+  while (kp < k) kp <<= 1;
+  const DLevel &L0 = H[0];
+  const int64_t n = L0.A.n;
+  batch_buffers(kp);
+  const bool do_once = o.top_elim && L0.kind == LIGMG_LEVEL_ELIM;
+  const size_t base = do_once ? 1 : 0;
+  const int64_t nbse = H[base].A.n;
+  int maxnb = 1;
+  for (auto &r : red) maxnb = std::max(maxnb, r.nb);
+  bsc_out.ensure((size_t)maxnb * kp * 2);
+  if (bouter.x.n < (size_t)(nbse * kp) || bouter.sc.n < (size_t)(red[base].nb * kp * SC_N))
+    bouter.alloc(nbse * kp, red[base].nb * kp);
+  DevBuf<double> bin(n * k), b0b(n * kp), rhs(nbse * kp), solb(n * kp), tmp(n * kp);
+  if (mem == LIGMG_MEM_HOST) to_device(bin.p, Bin, n * k, s);
+  else d2d(bin.p, Bin, n * k, s);
+  LG_HIP(hipStreamSynchronize(s));
+  auto t0 = std::chrono::steady_clock::now();
+  op_to_block(bin.p, n, k, kp, b0b.p, s);
+  if (do_once) op_bspmv<double>(L0.R, b0b.p, rhs.p, kp, bparts, s);
+  else d2d(rhs.p, b0b.p, n * kp, s);
+  red[base].bdots<double>(1, kp, rhs.p, nullptr, nullptr, nullptr, bsc_out.p, bpart_red, s);
+  op_bsub_block_mean<double>(rhs.p, kp, red[base], bsc_out.p, s);
+  fill_zero<double>(bouter.x.p, nbse * kp);
+  std::vector<int> its;
+  int status = 0;
+  int iters = bkrylov<double>(base, H[base].A, bouter.x.p, rhs.p, bouter, kp, o.outer_ksp_type, o.tol,
+                              o.iters, true, &its, &status, true);
+  if (do_once) op_binterp<double>(L0.P, L0.q.p, bouter.x.p, b0b.p, solb.p, kp, bparts, s);
+  else d2d(solb.p, bouter.x.p, n * kp, s);
+  op_from_block(solb.p, n, k, kp, bin.p, s);
+  LG_HIP(hipStreamSynchronize(s));
+  auto t1 = std::chrono::steady_clock::now();
+  // full residual per column (worst column reported)
+  op_bresidual<double>(L0.A, solb.p, b0b.p, tmp.p, kp, bparts, s);
+  red[0].bdots<double>(2, kp, tmp.p, tmp.p, b0b.p, b0b.p, bsc_out.p, bpart_red, s);
+  std::vector<double> d = download(bsc_out.p, (size_t)red[0].nb * 2 * kp, s);
+  double worst = 0;
+  for (int c = 0; c < k; c++) {
+    double rr = 0, bb = 0;
+    for (int blk = 0; blk < red[0].nb; blk++) {
+      rr += d[((size_t)blk * 2) * kp + c];
+      bb += d[((size_t)blk * 2 + 1) * kp + c];
+    }
+    if (bb > 0) worst = std::max(worst, std::sqrt(rr / bb));
+  }
+  if (mem == LIGMG_MEM_HOST) to_host(Xout, bin.p, n * k, s);
+  else d2d(Xout, bin.p, n * k, s);
+  LG_HIP(hipStreamSynchronize(s));
+  if (iters_out) {
+    const int nb = red[base].nb;
+    for (int c = 0; c < k; c++) {
+      int m = 0;
+      for (int blk = 0; blk < nb; blk++) m = std::max(m, its[(size_t)blk * kp + c]);
+      iters_out[c] = m;
+    }
+  }
+  if (info) {
+    memset(info, 0, sizeof(*info));
+    info->iters = iters;
+    info->status = status;
+    info->base_level = (int)base;
+    info->full_rel_residual = worst;
+    info->time = std::chrono::duration<double>(t1 - t0).count();
+    info->setup_time = setup_time;
+    // worst reduced-system relative residual over (block, column) sets
+    std::vector<double> hsc = download(bouter.sc.p, (size_t)red[base].nb * kp * SC_N, s);
+    double rel = 0;
+    for (int t = 0; t < red[base].nb * kp; t++) {
+      double bn = hsc[t * SC_N + SC_BN2];
+      if (bn > 0) rel = std::max(rel, std::sqrt(hsc[t * SC_N + SC_RN2] / bn));
+    }
+    info->rel_residual = rel;
+  }
+  return LIGMG_OK;
+}
 
 template <>
 LevelWork<double> &Solver::work<double>(size_t l) {
@@ -1053,14 +1441,74 @@ int ligmg_level_spmv(ligmg_handle h, int32_t l, int32_t prec, const double *x, d
   }
 }
 
-int ligmg_solve_batch(ligmg_handle, int32_t, const double *, double *, int, ligmg_info *, int32_t *) {
-  lg::g_last_error = "ligmg_solve_batch: not available in this build";
-  return LIGMG_ERR_UNSUPPORTED;
+int ligmg_solve_batch(ligmg_handle h, int32_t k, const double *B, double *X, int mem_kind,
+                      ligmg_info *info, int32_t *iters_out) {
+  if (!h || !B || !X || k < 1 || k > 64) return LIGMG_ERR_INVALID;
+  try {
+    return h->S.solve_batch(k, B, X, mem_kind, info, iters_out);
+  } catch (const lg::Error &e) {
+    return fail(e);
+  } catch (const std::exception &e) {
+    return fail_unknown(e.what());
+  }
 }
 
-int ligmg_edge_grad(ligmg_handle, const double *, const double *, double *) {
-  lg::g_last_error = "ligmg_edge_grad: not available in this build";
-  return LIGMG_ERR_UNSUPPORTED;
+namespace {
+__global__ void k_edge_grad(int64_t m, const int32_t *ei, const int32_t *ej, const double *x,
+                            const double *lam, double *g) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = ei[e], j = ej[e];
+    g[e] = -(lam[i] - lam[j]) * (x[i] - x[j]);
+  }
+}
+__global__ void k_assemble(int64_t n, const int32_t *rp, const int32_t *ci, const int32_t *eid,
+                           const double *w, double *v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double d = 0.0;
+    int dpos = -1;
+    for (int k = rp[i]; k < rp[i + 1]; k++) {
+      const int e = eid[k];
+      if (e < 0) {
+        dpos = k;
+      } else {
+        v[k] = -w[e];
+        d += w[e];
+      }
+    }
+    if (dpos >= 0) v[dpos] = d;
+  }
+}
+}  // namespace
+
+int ligmg_edge_grad(int64_t m, const int32_t *ei, const int32_t *ej, const double *x,
+                    const double *lam, double *g) {
+  if (m < 0 || (m > 0 && (!ei || !ej || !x || !lam || !g))) return LIGMG_ERR_INVALID;
+  if (m == 0) return LIGMG_OK;
+  try {
+    hipLaunchKernelGGL(k_edge_grad, dim3(lg::grid_for(m, 256, 16384)), dim3(256), 0, 0, m, ei, ej, x,
+                       lam, g);
+    LG_LAUNCH_CHECK();
+    LG_HIP(hipStreamSynchronize(0));
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_assemble_laplacian(int64_t n, const int32_t *rp, const int32_t *ci, const int32_t *eid,
+                             const double *w, double *v) {
+  if (n <= 0 || !rp || !ci || !eid || !w || !v) return LIGMG_ERR_INVALID;
+  try {
+    hipLaunchKernelGGL(k_assemble, dim3(lg::grid_for(n, 256, 16384)), dim3(256), 0, 0, n, rp, ci, eid,
+                       w, v);
+    LG_LAUNCH_CHECK();
+    LG_HIP(hipStreamSynchronize(0));
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
 }
 
 }  // extern "C"

@@ -128,6 +128,9 @@ def _csr_args(A):
         ci = ci.to(torch.int32).contiguous()
         v = v.to(torch.float64).contiguous()
         mem = _ffi.MEM_DEVICE if rp.is_cuda else _ffi.MEM_HOST
+        if rp.is_cuda:
+            # the library runs on its own stream: torch's pending writes first
+            torch.cuda.current_stream().synchronize()
         return n, rp.data_ptr(), ci.data_ptr(), v.data_ptr(), mem, (rp, ci, v)
     rp = np.ascontiguousarray(rp, dtype=np.int32)
     ci = np.ascontiguousarray(ci, dtype=np.int32)
@@ -180,6 +183,8 @@ class Solver:
             bb = b.to(torch.float64).contiguous()
             x = torch.empty_like(bb)
             mem = _ffi.MEM_DEVICE if bb.is_cuda else _ffi.MEM_HOST
+            if bb.is_cuda:
+                torch.cuda.current_stream().synchronize()
             rc = self._lib.ligmg_solve(self._h, bb.data_ptr(), x.data_ptr(), mem, C.byref(info),
                                        hist.ctypes.data_as(C.POINTER(C.c_double)), history_cap)
         else:
@@ -192,6 +197,35 @@ class Solver:
                                        history_cap)
         _check(rc, "ligmg_solve")
         return ConvergenceInfo(hist[:info.history_len].copy(), info, x)
+
+    def solve_batch(self, B):
+        """Solve for the k columns of B (n x k, k <= 64) in one hierarchy
+        traversal per cycle.  Returns (X, info, iters_per_column); every
+        (graph block, column) pair has its own Krylov coefficients."""
+        info = _ffi.ligmg_info()
+        if _is_torch(B):
+            import torch
+            k = B.shape[1]
+            Bc = B.to(torch.float64).t().contiguous()  # column-major storage
+            Xc = torch.empty_like(Bc)
+            mem = _ffi.MEM_DEVICE if Bc.is_cuda else _ffi.MEM_HOST
+            if Bc.is_cuda:
+                torch.cuda.current_stream().synchronize()
+            its = np.zeros(k, dtype=np.int32)
+            rc = self._lib.ligmg_solve_batch(self._h, k, Bc.data_ptr(), Xc.data_ptr(), mem,
+                                             C.byref(info), its.ctypes.data_as(C.POINTER(C.c_int32)))
+            X = Xc.t()
+        else:
+            Bc = np.asfortranarray(B, dtype=np.float64)
+            if Bc.ndim != 2 or Bc.shape[0] != self.n:
+                raise ValueError(f"B must have shape ({self.n}, k)")
+            k = Bc.shape[1]
+            X = np.empty_like(Bc, order="F")
+            its = np.zeros(k, dtype=np.int32)
+            rc = self._lib.ligmg_solve_batch(self._h, k, Bc.ctypes.data, X.ctypes.data, _ffi.MEM_HOST,
+                                             C.byref(info), its.ctypes.data_as(C.POINTER(C.c_int32)))
+        _check(rc, "ligmg_solve_batch")
+        return X, info, its
 
     # -------------------------------------------------------- introspection
     def num_levels(self):

@@ -1,0 +1,91 @@
+"""Batched solves (k right-hand sides, many graphs) and hipGraph replays,
+through the C ABI, against single-RHS solves and the CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+import ligmg_amd
+from ligmg_amd import _ffi, graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = a - a.mean(0)
+    b = b - b.mean(0)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("k", [1, 5, 20, 64])
+@pytest.mark.parametrize("prec", [_ffi.PREC_FP64, _ffi.PREC_MIXED])
+def test_batch_matches_single(k, prec):
+    g = graphs.rmat_config(12, "lognormal", pendants=True)
+    print(f"k={k} prec={prec} n={g.n}", flush=True)
+    o = ligmg_amd.Options(precision=prec, outer_ksp_type=_ffi.KSP_FCG)
+    s = ligmg_amd.Solver(g, o)
+    B = graphs.rhs(g.n, k, 11).reshape(g.n, k, order="F")
+    X, info, its = s.solve_batch(B)
+    assert info.status == _ffi.CONVERGED and info.rel_residual <= 1e-8
+    assert info.full_rel_residual <= 1e-7
+    for c in range(k):
+        one = s.solve(np.ascontiguousarray(B[:, c]))
+        assert rel(X[:, c], one.solution) <= 1e-7, c
+        assert abs(int(its[c]) - one.iters()) <= 1, (c, its[c], one.iters())
+
+
+def test_batch_many_graphs_per_block_krylov():
+    gs = [graphs.barabasi_albert(n, 5, seed=i) for i, n in enumerate([2000, 700, 4000, 1500])]
+    u = graphs.union(gs)
+    k = 16
+    B = graphs.rhs(u.n, k, 3, comp_ptr=u.comp_ptr).reshape(u.n, k, order="F")
+    s = ligmg_amd.Solver(u, ligmg_amd.Options(precision=_ffi.PREC_MIXED, outer_ksp_type=_ffi.KSP_FCG),
+                         comp_ptr=u.comp_ptr)
+    X, info, its = s.solve_batch(B)
+    assert info.status == _ffi.CONVERGED
+    r = O.OracleSolver(u, comp_ptr=u.comp_ptr)
+    for c in [0, 7, 15]:
+        xr, _, _ = r.solve(np.ascontiguousarray(B[:, c]))
+        for blk in range(len(gs)):
+            lo, hi = u.comp_ptr[blk], u.comp_ptr[blk + 1]
+            assert rel(X[lo:hi, c], xr[lo:hi]) <= 1e-6
+    # every (graph, column) reached the tolerance on its own block
+    A = u.to_scipy()
+    R = B - A @ X
+    for blk in range(len(gs)):
+        lo, hi = u.comp_ptr[blk], u.comp_ptr[blk + 1]
+        nr = np.linalg.norm(R[lo:hi], axis=0) / np.linalg.norm(B[lo:hi], axis=0)
+        assert nr.max() <= 1e-7
+
+
+def test_batch_kcycle_and_grid3d():
+    g = graphs.grid3d(20, 18, 16, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=1)
+    k = 32
+    B = graphs.rhs(g.n, k, 5).reshape(g.n, k, order="F")
+    s = ligmg_amd.Solver(g, ligmg_amd.Options(kcycle=1, outer_ksp_type=_ffi.KSP_FCG))
+    X, info, its = s.solve_batch(B)
+    assert info.status == _ffi.CONVERGED and info.rel_residual <= 1e-8
+    xr, rinfo, _ = O.OracleSolver(g, O.default_options(kcycle=1, outer_ksp_type=_ffi.KSP_FCG)).solve(
+        np.ascontiguousarray(B[:, 3]))
+    assert rel(X[:, 3], xr) <= 1e-6
+    assert abs(int(its[3]) - rinfo.iters) <= 2
+
+
+def test_graph_replays_match_eager():
+    """Captured preconditioner graphs, replayed many times with eager work
+    and synchronisations in between, give the eager result every time."""
+    g = graphs.rmat_config(12, "uniform", pendants=True)
+    bs = [graphs.rhs(g.n, 1, seed) for seed in range(4)]
+    eager = ligmg_amd.Solver(g, ligmg_amd.Options(use_graphs=0))
+    graphed = ligmg_amd.Solver(g, ligmg_amd.Options(use_graphs=1))
+    ref = [eager.solve(b).solution for b in bs]
+    for rep in range(3):
+        for i, b in enumerate(bs):
+            x = graphed.solve(b).solution
+            assert np.array_equal(x, ref[i]), (rep, i)
+            eager.solve(bs[(i + 1) % 4])  # eager work between replays
+    for prec_opts in [dict(precision=_ffi.PREC_MIXED, outer_ksp_type=_ffi.KSP_FCG),
+                      dict(kcycle=1, outer_ksp_type=_ffi.KSP_FCG)]:
+        e = ligmg_amd.Solver(g, ligmg_amd.Options(use_graphs=0, **prec_opts))
+        gr = ligmg_amd.Solver(g, ligmg_amd.Options(use_graphs=1, **prec_opts))
+        for i, b in enumerate(bs):
+            assert np.array_equal(gr.solve(b).solution, e.solve(b).solution)
