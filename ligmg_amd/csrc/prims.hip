@@ -175,8 +175,10 @@ __global__ void k_seg_heads(const uint64_t *keys, int64_t n, int32_t *flags) {
 }
 
 // 64 segments per wave: a segment of at most 64 terms is summed left to
-// right by its own lane; longer ones by the whole wave, one at a time, with
-// 64 strided partials and the fixed tree (oracle canon_sum).
+// right by its own lane; longer ones by the whole wave, one at a time, in
+// 4096-term chunks (each with 64 strided partials and the fixed tree, a
+// final chunk of <= 64 terms left to right) whose results are added left to
+// right (the oracle's canon_sum).
 __global__ void __launch_bounds__(256) k_seg_sum(const uint64_t *keys, const uint64_t *vals,
                                                  const int32_t *start, int64_t nseg, int64_t n,
                                                  uint64_t *out_keys, double *out_sums) {
@@ -201,11 +203,20 @@ __global__ void __launch_bounds__(256) k_seg_sum(const uint64_t *keys, const uin
     int L = __ffsll((long long)mask) - 1;
     mask &= mask - 1;
     int64_t la = __shfl(a, L, WAVE), lb = __shfl(b, L, WAVE);
-    double p = 0.0;
-    for (int64_t i = la + lane; i < lb; i += WAVE) p += __longlong_as_double((long long)vals[i]);
-    p = wave_tree_sum(p);
-    p = __shfl(p, 0, WAVE);
-    if (lane == L) res = p;
+    double tot = 0.0;
+    for (int64_t ca = la; ca < lb; ca += 4096) {
+      const int64_t cb = min(ca + (int64_t)4096, lb);
+      double p = 0.0;
+      if (cb - ca <= WAVE) {
+        for (int64_t i = ca; i < cb; i++) p += __longlong_as_double((long long)vals[i]);
+      } else {
+        for (int64_t i = ca + lane; i < cb; i += WAVE) p += __longlong_as_double((long long)vals[i]);
+        p = wave_tree_sum(p);
+        p = __shfl(p, 0, WAVE);
+      }
+      tot += p;
+    }
+    if (lane == L) res = tot;
   }
   if (valid) {
     out_keys[seg] = keys[a];

@@ -29,9 +29,12 @@ enum { PRIO_DECIDED = 0, PRIO_UNDECIDED = 1, PRIO_SEED = 2 };
 constexpr int MAXT = 16;
 
 // --------------------------------------------------------------------------
-// Generic canonical row kernel: one lane per row, rows longer than 64
-// entries done cooperatively by the wavefront.
+// Generic canonical row kernel: one lane per row, rows of 65..4096 entries
+// done cooperatively by the wavefront.  Rows longer than CHUNK are left to
+// k_rows_chunks (one wavefront per 4096-entry chunk) and k_rows_huge.
 // --------------------------------------------------------------------------
+constexpr int CHUNK = 4096;
+
 template <class Op>
 __global__ void __launch_bounds__(256) k_rows(int n, const int32_t *__restrict__ rp, Op op) {
   using Acc = typename Op::Acc;
@@ -43,9 +46,10 @@ __global__ void __launch_bounds__(256) k_rows(int n, const int32_t *__restrict__
     a = rp[row];
     b = rp[row + 1];
   }
-  const bool lng = valid && (b - a > WAVE);
+  const bool huge = valid && (b - a > CHUNK);
+  const bool lng = valid && !huge && (b - a > WAVE);
   Acc acc = op.zero();
-  if (valid && !lng)
+  if (valid && !lng && !huge)
     for (int k = a; k < b; k++) op.add(acc, row, k);
   uint64_t mask = __ballot(lng);
   while (mask) {
@@ -61,14 +65,98 @@ __global__ void __launch_bounds__(256) k_rows(int n, const int32_t *__restrict__
     p = op.shfl0(p);
     if (lane == L) acc = p;
   }
-  if (valid) op.finish(row, acc);
+  if (valid && !huge) op.finish(row, acc);
+}
+
+// one wavefront per chunk of a huge row: the chunk's canonical sum
+template <class Op>
+__global__ void __launch_bounds__(256) k_rows_chunks(int64_t nchunks, const int32_t *cstart,
+                                                     const int32_t *cend, const int32_t *crow, Op op,
+                                                     typename Op::Acc *cacc) {
+  using Acc = typename Op::Acc;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t c = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (c >= nchunks) return;
+  const int la = cstart[c], lb = cend[c], lr = crow[c];
+  Acc p = op.zero();
+  if (lb - la <= WAVE) {
+    if (lane == 0)
+      for (int k = la; k < lb; k++) op.add(p, lr, k);
+  } else {
+    for (int k = la + lane; k < lb; k += WAVE) op.add(p, lr, k);
+    for (int off = 32; off >= 1; off >>= 1) {
+      Acc o = op.shfl_down(p, off);
+      p = op.merge(p, o);
+    }
+  }
+  if (lane == 0) cacc[c] = p;
+}
+
+// chunk results of each huge row combined left to right
+template <class Op>
+__global__ void k_rows_huge(int64_t nrows, const int32_t *row, const int32_t *cptr, Op op,
+                            const typename Op::Acc *cacc) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= nrows) return;
+  typename Op::Acc acc = op.zero();
+  for (int c = cptr[j]; c < cptr[j + 1]; c++) acc = op.merge(acc, cacc[c]);
+  op.finish(row[j], acc);
+}
+
+// Rows longer than CHUNK of one matrix and their chunks.
+struct HugeRows {
+  int64_t nrows = 0, nchunks = 0;
+  DevBuf<int32_t> row, cptr, cstart, cend, crow;
+  DevBuf<uint8_t> cacc;
+};
+
+void find_huge(int64_t n, const int32_t *rp, HugeRows &Hr, hipStream_t s) {
+  std::vector<int32_t> h = download(rp, n + 1, s);
+  std::vector<int32_t> row, cptr, cstart, cend, crow;
+  for (int64_t i = 0; i < n; i++) {
+    if (h[i + 1] - h[i] <= CHUNK) continue;
+    row.push_back((int32_t)i);
+    cptr.push_back((int32_t)cstart.size());
+    for (int64_t k = h[i]; k < h[i + 1]; k += CHUNK) {
+      cstart.push_back((int32_t)k);
+      cend.push_back((int32_t)std::min<int64_t>(k + CHUNK, h[i + 1]));
+      crow.push_back((int32_t)i);
+    }
+  }
+  cptr.push_back((int32_t)cstart.size());
+  Hr.nrows = (int64_t)row.size();
+  Hr.nchunks = (int64_t)cstart.size();
+  if (Hr.nrows == 0) return;
+  Hr.row.alloc(Hr.nrows);
+  Hr.cptr.alloc(Hr.nrows + 1);
+  Hr.cstart.alloc(Hr.nchunks);
+  Hr.cend.alloc(Hr.nchunks);
+  Hr.crow.alloc(Hr.nchunks);
+  to_device(Hr.row.p, row.data(), Hr.nrows, s);
+  to_device(Hr.cptr.p, cptr.data(), Hr.nrows + 1, s);
+  to_device(Hr.cstart.p, cstart.data(), Hr.nchunks, s);
+  to_device(Hr.cend.p, cend.data(), Hr.nchunks, s);
+  to_device(Hr.crow.p, crow.data(), Hr.nchunks, s);
+  Hr.cacc.alloc((size_t)Hr.nchunks * 192);  // room for the largest Acc
+  LG_HIP(hipStreamSynchronize(s));
 }
 
 template <class Op>
-void run_rows(int64_t n, const int32_t *rp, const Op &op, hipStream_t s) {
+void run_rows(int64_t n, const int32_t *rp, const Op &op, const HugeRows &Hr, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_rows<Op>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int)n, rp, op);
   LG_LAUNCH_CHECK();
+  if (Hr.nrows) {
+    using Acc = typename Op::Acc;
+    static_assert(sizeof(Acc) <= 192, "Acc too large for the chunk buffer");
+    Acc *cacc = (Acc *)Hr.cacc.p;
+    hipLaunchKernelGGL(k_rows_chunks<Op>, dim3((unsigned)((Hr.nchunks + 3) / 4)), dim3(256), 0, s,
+                       Hr.nchunks, Hr.cstart.p, Hr.cend.p, Hr.crow.p, op, cacc);
+    LG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_rows_huge<Op>, dim3((unsigned)((Hr.nrows + 63) / 64)), dim3(64), 0, s, Hr.nrows,
+                       Hr.row.p, Hr.cptr.p, op, (const Acc *)cacc);
+    LG_LAUNCH_CHECK();
+  }
 }
 
 struct IntSum {
@@ -712,9 +800,11 @@ void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, 
 bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &coarse,
                        hipStream_t s) {
   const int64_t n = A.n;
+  HugeRows Hr;
+  find_huge(n, A.rp.p, Hr, s);
   DevBuf<int32_t> deg(n), elim(n);
-  run_rows(n, A.rp.p, OpDeg{{}, A.v.p, deg.p}, s);
-  run_rows(n, A.rp.p, OpElimMin{A.ci.p, deg.p, A.v.p, o.elim_degree, elim.p}, s);
+  run_rows(n, A.rp.p, OpDeg{{}, A.v.p, deg.p}, Hr, s);
+  run_rows(n, A.rp.p, OpElimMin{A.ci.p, deg.p, A.v.p, o.elim_degree, elim.p}, Hr, s);
   // kept flags -> new ids
   DevBuf<int32_t> kept(n), newid(n);
   {
@@ -789,7 +879,7 @@ bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &c
   // Schur complement by contraction
   DevBuf<int32_t> cnt(n + 1);
   fill_i32(cnt.p, 0, n + 1, s);
-  run_rows(n, A.rp.p, OpSchurCount{{}, A.rp.p, A.ci.p, L.new_id.p, cnt.p}, s);
+  run_rows(n, A.rp.p, OpSchurCount{{}, A.rp.p, A.ci.p, L.new_id.p, cnt.p}, Hr, s);
   int64_t m = exclusive_scan_i32(cnt.p, cnt.p, n + 1, s);
   DevBuf<uint64_t> keys(m), vals(m);
   const int cb = std::max(1, bits_for((uint64_t)n_kept));
@@ -807,6 +897,8 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
                       DevBuf<double> &cs, hipStream_t s) {
   const int64_t n = A.n, nnz = A.nnz;
   const int T = o.num_tvs;
+  HugeRows Hr;
+  find_huge(n, A.rp.p, Hr, s);
   DevBuf<double> X((size_t)n * T), Y((size_t)n * T), nrm(n), diag(n), raw(nnz), rowmax(n), colmax(n);
   DevBuf<uint8_t> hasd(n);
   DevBuf<int32_t> tpos(nnz), bad(1);
@@ -821,25 +913,27 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
     hipLaunchKernelGGL(k_tv_fill, dim3(g1(n)), dim3(256), 0, s, (int)n, o.seed, level_idx, T, sums.p, X.p);
     LG_LAUNCH_CHECK();
     for (int sw = 0; sw < o.tvs_smooth; sw++) {
-      run_rows(n, A.rp.p, OpJacobiTV{A.ci.p, A.v.p, X.p, diag.p, hasd.p, Y.p, T}, s);
+      run_rows(n, A.rp.p, OpJacobiTV{A.ci.p, A.v.p, X.p, diag.p, hasd.p, Y.p, T}, Hr, s);
       std::swap(X.p, Y.p);
     }
     hipLaunchKernelGGL(k_tv_norm, dim3(g1(n)), dim3(256), 0, s, (int)n, T, X.p, nrm.p);
     LG_LAUNCH_CHECK();
     LG_HIP(hipStreamSynchronize(s));
   }
-  run_rows(n, A.rp.p, OpAffinity{{}, A.ci.p, A.v.p, X.p, nrm.p, T, o.strength_metric, raw.p, rowmax.p}, s);
+  run_rows(n, A.rp.p, OpAffinity{{}, A.ci.p, A.v.p, X.p, nrm.p, T, o.strength_metric, raw.p, rowmax.p},
+           Hr, s);
   LG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
-  run_rows(n, A.rp.p, OpTpos{{}, A.rp.p, A.ci.p, tpos.p, bad.p}, s);
+  run_rows(n, A.rp.p, OpTpos{{}, A.rp.p, A.ci.p, tpos.p, bad.p}, Hr, s);
   int32_t hb = 0;
   to_host(&hb, bad.p, 1, s);
   LG_HIP(hipStreamSynchronize(s));
   LG_CHECK(hb == 0, LIGMG_ERR_INVALID, "matrix pattern is not structurally symmetric");
-  run_rows(n, A.rp.p, OpColMax{{}, tpos.p, raw.p, colmax.p}, s);
+  run_rows(n, A.rp.p, OpColMax{{}, tpos.p, raw.p, colmax.p}, Hr, s);
   S.alloc(nnz);
-  run_rows(n, A.rp.p, OpScale{{}, A.ci.p, raw.p, rowmax.p, colmax.p, o.thresh, o.aff_scaling, S.p}, s);
+  run_rows(n, A.rp.p, OpScale{{}, A.ci.p, raw.p, rowmax.p, colmax.p, o.thresh, o.aff_scaling, S.p}, Hr,
+           s);
   cs.alloc(n);
-  run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S.p, cs.p}, s);
+  run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S.p, cs.p}, Hr, s);
   LG_HIP(hipStreamSynchronize(s));
 }
 
@@ -849,6 +943,8 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
 int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg_options &o,
                    DevBuf<int32_t> &agg, DevBuf<int32_t> &ind, DevBuf<int32_t> &state, hipStream_t s) {
   const int64_t n = A.n;
+  HugeRows Hr;
+  find_huge(n, A.rp.p, Hr, s);
   state.alloc(n);
   ind.alloc(n);
   DevBuf<int32_t> nstate(n), nind(n), vind(n), vf(n), flag(n), idx(n);
@@ -860,7 +956,7 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
   const int kb = std::max(1, bits_for((uint64_t)n));
   for (int iter = 0; iter < o.agg_iters; iter++) {
     const double th = o.agg_init_factor * std::pow(o.agg_iter_factor, iter);
-    run_rows(n, A.rp.p, OpVote{A.ci.p, state.p, ind.p, S, th, vst.p, vind.p, vw.p}, s);
+    run_rows(n, A.rp.p, OpVote{A.ci.p, state.p, ind.p, S, th, vst.p, vind.p, vw.p}, Hr, s);
     hipLaunchKernelGGL(k_voters, dim3(g1(n)), dim3(256), 0, s, (int)n, state.p, vind.p, vw.p, th, flag.p,
                        vf.p);
     LG_LAUNCH_CHECK();
@@ -902,15 +998,17 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
 
 void col_sums_of(const DCsr &A, const double *S, DevBuf<double> &cs, hipStream_t s) {
   const int64_t n = A.n;
+  HugeRows Hr;
+  find_huge(n, A.rp.p, Hr, s);
   DevBuf<int32_t> tpos(A.nnz), bad(1);
   LG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
-  run_rows(n, A.rp.p, OpTpos{{}, A.rp.p, A.ci.p, tpos.p, bad.p}, s);
+  run_rows(n, A.rp.p, OpTpos{{}, A.rp.p, A.ci.p, tpos.p, bad.p}, Hr, s);
   int32_t hb = 0;
   to_host(&hb, bad.p, 1, s);
   LG_HIP(hipStreamSynchronize(s));
   LG_CHECK(hb == 0, LIGMG_ERR_INVALID, "matrix pattern is not structurally symmetric");
   cs.alloc(n);
-  run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S, cs.p}, s);
+  run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S, cs.p}, Hr, s);
   LG_HIP(hipStreamSynchronize(s));
 }
 
@@ -930,7 +1028,9 @@ void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLe
   {
     DevBuf<uint64_t> keys(A.nnz), vals(A.nnz);
     const int cb = std::max(1, bits_for((uint64_t)nc));
-    run_rows(n, A.rp.p, OpGalAgg{{}, A.ci.p, L.agg.p, A.v.p, cb, keys.p, vals.p}, s);
+    HugeRows Hr;
+    find_huge(n, A.rp.p, Hr, s);
+    run_rows(n, A.rp.p, OpGalAgg{{}, A.ci.p, L.agg.p, A.v.p, cb, keys.p, vals.p}, Hr, s);
     contract_to_csr(keys, vals, A.nnz, nc, nc, coarse, s);
   }
   // member lists (restriction operator, unit values)

@@ -36,13 +36,15 @@ namespace {
 
 // ---------------------------------------------------------------------------
 // Canonical summation order.  Sequences of at most 64 terms are summed left
-// to right from 0.0; longer ones are split into 64 strided partial sums
-// (term t goes to partial t % 64, each partial summed left to right from
-// 0.0) which are combined by the tree p[l] += p[l + off], off = 32..1.
+// to right from 0.0; up to 4096 terms they are split into 64 strided
+// partial sums (term t goes to partial t % 64, each partial summed left to
+// right from 0.0) combined by the tree p[l] += p[l + off], off = 32..1;
+// longer ones are cut into consecutive 4096-term chunks, each summed by the
+// previous rule, and the chunk results are added left to right from 0.0.
 // Rows are summed over all their stored positions, masked positions
 // contributing +0.0, so the split depends only on the row length.
 // ---------------------------------------------------------------------------
-double canon_sum(const double *a, int64_t m) {
+double canon_sum_chunk(const double *a, int64_t m) {
   if (m <= 64) {
     double s = 0.0;
     for (int64_t i = 0; i < m; i++) s += a[i];
@@ -54,6 +56,13 @@ double canon_sum(const double *a, int64_t m) {
   for (int off = 32; off >= 1; off >>= 1)
     for (int l = 0; l < off; l++) p[l] = p[l] + p[l + off];
   return p[0];
+}
+
+double canon_sum(const double *a, int64_t m) {
+  if (m <= 4096) return canon_sum_chunk(a, m);
+  double s = 0.0;
+  for (int64_t c = 0; c < m; c += 4096) s += canon_sum_chunk(a + c, std::min<int64_t>(4096, m - c));
+  return s;
 }
 
 struct Csr {

@@ -35,10 +35,13 @@ def make_case(name, my_matrix=None):
         return graphs.rgg(n, math.sqrt(12 / (math.pi * n)), seed=4)
     if name == "grid3d":
         return graphs.grid3d(14, 12, 10, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=5)
+    if name == "rmat16_pend":  # 17 hub rows longer than 4096 entries (chunked sums)
+        return graphs.rmat_config(16, "lognormal", pendants=True)
     raise KeyError(name)
 
 
-CASES = ["my_matrix", "grid64", "grid_w", "rmat12_pend", "rmat13_unif", "ba", "rgg", "grid3d"]
+CASES = ["my_matrix", "grid64", "grid_w", "rmat12_pend", "rmat13_unif", "ba", "rgg", "grid3d",
+         "rmat16_pend"]
 
 
 def opts_pair(**kw):

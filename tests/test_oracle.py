@@ -444,3 +444,15 @@ def test_blocks_solve_independently():
         xr = grounded_solve(g, b[lo:hi])
         xm = x[lo:hi] - x[lo:hi].mean()
         assert np.linalg.norm(xm - xr) <= 1e-6 * np.linalg.norm(xr)
+
+
+def test_canon_sum_chunks_long_sequences():
+    """Beyond 4096 terms the sum runs over 4096-term chunks added left to
+    right: a cancellation between terms 0 and 4096 (same strided partial
+    without chunking) no longer happens inside one partial."""
+    a = np.zeros(5000)
+    a[0], a[4096], a[1] = 1e16, -1e16, 1.0
+    assert O.canon_sum(a) == 0.0          # (1e16 + 1) + (-1e16) in chunk order
+    assert O.canon_sum(a[:4096]) == 1e16  # a single chunk
+    b = np.random.default_rng(2).standard_normal(20000)
+    assert abs(O.canon_sum(b) - math.fsum(b)) < 1e-12 * np.abs(b).sum()
