@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-scale", type=int, default=18)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ordering", choices=["degree", "natural"], default="degree",
+                    help="vertex order of the hierarchy (default: degree-sorted)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="setup + SpMV timing + copy calibration only (for counter runs)")
     args = ap.parse_args()
@@ -97,7 +99,8 @@ def main():
 
     # FP64-only solve (the plain method, the drop-in): V-cycle + CG
     verbose = int(os.environ.get("LIGMG_VERBOSE", "0"))
-    s64 = ligmg_amd.Solver(A, ligmg_amd.Options(verbose=verbose))
+    order = _ffi.ORDER_DEGREE if args.ordering == "degree" else _ffi.ORDER_NATURAL
+    s64 = ligmg_amd.Solver(A, ligmg_amd.Options(verbose=verbose, ordering=order))
     print(f"# fp64 setup {s64.setup_time:.2f}s levels {s64.num_levels()}", file=sys.stderr,
           flush=True)
     if args.spmv_only:
@@ -135,8 +138,8 @@ def main():
     torch.cuda.empty_cache()
 
     # FP32 hierarchy + cycle inside the FP64 flexible-CG outer loop
-    s32 = ligmg_amd.Solver(A, ligmg_amd.Options(precision=_ffi.PREC_MIXED,
-                                                 outer_ksp_type=_ffi.KSP_FCG, verbose=verbose))
+    s32 = ligmg_amd.Solver(A, ligmg_amd.Options(precision=_ffi.PREC_MIXED, outer_ksp_type=_ffi.KSP_FCG,
+                                                 verbose=verbose, ordering=order))
     print(f"# mixed setup {s32.setup_time:.2f}s", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         s32.solve(b)
@@ -172,6 +175,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"rmat{args.scale}_lcc_pendants_lognormal1.5_fp32precond_fcg",
+            "ordering": args.ordering,
             "n": g.n, "nnz": g.nnz, "rhs": 1, "rtol": 1e-8,
             "base_level_n": lvl[1], "base_level_nnz": lvl[2],
         },

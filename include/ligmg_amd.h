@@ -61,6 +61,17 @@ enum {
   LIGMG_DIVERGED_NAN = 3
 };
 
+/* Vertex order of the hierarchy (new).  The reference randomly permutes
+ * the input when it runs on more than one rank, for load balance
+ * (Options::rand_perm, options.hpp:119; main.cpp:52-63).  On one GPU the
+ * permutation serves gather locality instead: DEGREE renumbers vertices by
+ * descending stored-entry count (ties by ascending index) inside each
+ * connected block of comp_ptr, so the most-gathered entries of every vector
+ * share a few MiB of L2.  Right-hand sides and solutions stay in the
+ * caller's order; level introspection reports the internal order, and
+ * ligmg_ordering returns the permutation. */
+enum { LIGMG_ORDER_NATURAL = 0, LIGMG_ORDER_DEGREE = 1 };
+
 /* Mirrors ligmg::Options (options.hpp:80-134) field names and defaults for
  * every field that affects setup or solve.  Fields marked (new) have no
  * counterpart in the reference. */
@@ -101,8 +112,10 @@ typedef struct ligmg_options {
   int32_t cheby_eig_iters;      /* 10: CG-Lanczos steps for lambda_max       */
   int32_t precision;            /* LIGMG_PREC_FP64                           */
   int32_t use_graphs;           /* 1: replay cycles from captured hipGraphs  */
-  int32_t spmv_slice;           /* -1: autotune column slicing of the solve
-                                   SpMVs; 0: never; >0: slice width (cols)  */
+  int32_t spmv_slice;           /* 0: no column slicing of the solve SpMVs;
+                                   >0: slice width (cols); -1: autotune     */
+  int32_t ordering;             /* LIGMG_ORDER_DEGREE: vertex order used for
+                                   setup and solve (see enum)               */
 } ligmg_options;
 
 /* ConvergenceInfo (util.hpp:143-170) plus the fields the reference only
@@ -167,6 +180,10 @@ int ligmg_assemble_laplacian(int64_t n, const int32_t *rowptr,
 
 /* Solver destructor / PCDestroy_LIGMG (petsc_shim.hpp:31-38). */
 int ligmg_destroy(ligmg_handle h);
+
+/* The internal vertex order (new; see LIGMG_ORDER_*): perm (host, length n)
+ * receives, for every internal position, the caller's vertex index. */
+int ligmg_ordering(ligmg_handle h, int32_t *perm);
 
 /* Thread-local message of the last failed call ("" if none). */
 const char *ligmg_last_error(void);

@@ -30,6 +30,7 @@ class ligmg_options(C.Structure):
         ("neighbor_of_neighbor", i32), ("seed", u64),
         ("cheby_eig_iters", i32), ("precision", i32), ("use_graphs", i32),
         ("spmv_slice", i32),
+        ("ordering", i32),
     ]
 
 
@@ -57,6 +58,7 @@ SMOOTHER_JACOBI, SMOOTHER_CHEBYSHEV = 0, 1
 AFF_NONE, AFF_ROW, AFF_COLUMN, AFF_ROWCOLUMN, AFF_SYM, AFF_MAX = range(6)
 STRENGTH_AFFINITY, STRENGTH_ALGEBRAIC_DISTANCE, STRENGTH_NONE = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+ORDER_NATURAL, ORDER_DEGREE = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
 LEVEL_ELIM, LEVEL_AGG, LEVEL_COARSE = 0, 1, 2
 LEVEL_NAMES = {LEVEL_ELIM: "elim", LEVEL_AGG: "agg", LEVEL_COARSE: "coarse"}
@@ -81,6 +83,7 @@ SOLVER_PROTOS = {
     "ligmg_assemble_laplacian": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_void_p]),
     "ligmg_destroy": (C.c_int, [HANDLE]),
+    "ligmg_ordering": (C.c_int, [HANDLE, C.c_void_p]),
     "ligmg_last_error": (C.c_char_p, []),
     "ligmg_num_levels": (C.c_int, [HANDLE, P(i32)]),
     "ligmg_setup_time": (C.c_int, [HANDLE, P(f64)]),

@@ -196,19 +196,23 @@ __global__ void k_bprolong(const int32_t *agg, int64_t n, int kp, int lk, const 
 }
 
 // column-major (ld n) <-> row-major n x kp (padding columns zero)
-__global__ void k_to_block(const double *B, int64_t n, int k, int kp, double *X) {
+// column-major caller block (vertex order given by perm, identity when
+// null) <-> row-major internal block
+__global__ void k_to_block(const double *B, int64_t n, int k, int kp, const int32_t *perm, double *X) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * kp;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = t / kp;
     const int c = (int)(t % kp);
-    X[t] = c < k ? B[(int64_t)c * n + i] : 0.0;
+    const int64_t src = perm ? perm[i] : i;
+    X[t] = c < k ? B[(int64_t)c * n + src] : 0.0;
   }
 }
-__global__ void k_from_block(const double *X, int64_t n, int k, int kp, double *B) {
+__global__ void k_from_block(const double *X, int64_t n, int k, int kp, const int32_t *perm, double *B) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * k;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = t / n, i = t % n;
-    B[t] = X[i * kp + c];
+    const int64_t dst = perm ? perm[i] : i;
+    B[c * n + dst] = X[i * kp + c];
   }
 }
 
@@ -357,12 +361,14 @@ void op_bdense_apply(const double *Pm, int64_t n, const T *X, T *Y, int kp, hipS
   LG_LAUNCH_CHECK();
 }
 
-void op_to_block(const double *B, int64_t n, int k, int kp, double *X, hipStream_t s) {
-  hipLaunchKernelGGL(k_to_block, dim3(grid_for(n * kp, 256, 16384)), dim3(256), 0, s, B, n, k, kp, X);
+void op_to_block(const double *B, int64_t n, int k, int kp, double *X, const int32_t *perm, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_block, dim3(grid_for(n * kp, 256, 16384)), dim3(256), 0, s, B, n, k, kp, perm, X);
   LG_LAUNCH_CHECK();
 }
-void op_from_block(const double *X, int64_t n, int k, int kp, double *B, hipStream_t s) {
-  hipLaunchKernelGGL(k_from_block, dim3(grid_for(n * k, 256, 16384)), dim3(256), 0, s, X, n, k, kp, B);
+void op_from_block(const double *X, int64_t n, int k, int kp, double *B, const int32_t *perm,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_from_block, dim3(grid_for(n * k, 256, 16384)), dim3(256), 0, s, X, n, k, kp, perm,
+                     B);
   LG_LAUNCH_CHECK();
 }
 

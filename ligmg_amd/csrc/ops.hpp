@@ -91,7 +91,10 @@ template <class T>
 void op_bdense_apply(const double *Pm, int64_t n, const T *X, T *Y, int kp, hipStream_t s);
 template <class T>
 void op_bsub_block_mean(T *a, int kp, const RedPlan &P, const double *sums, hipStream_t s);
-void op_to_block(const double *B, int64_t n, int k, int kp, double *X, hipStream_t s);
-void op_from_block(const double *X, int64_t n, int k, int kp, double *B, hipStream_t s);
+// column-major n x k caller block <-> row-major n x kp internal block; the
+// caller's vertex of internal row i is perm[i] (identity when perm is null)
+void op_to_block(const double *B, int64_t n, int k, int kp, double *X, const int32_t *perm, hipStream_t s);
+void op_from_block(const double *X, int64_t n, int k, int kp, double *B, const int32_t *perm,
+                   hipStream_t s);
 
 }  // namespace lg

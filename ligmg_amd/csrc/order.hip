@@ -1,0 +1,143 @@
+// Vertex ordering of the input matrix and the vector permutations at the
+// solve boundary.
+//
+// The reference permutes the input randomly, and only when it runs on more
+// than one rank, to balance the ranks (Options::rand_perm, options.hpp:119;
+// main.cpp:52-63).  On one GPU the order serves the gathers of x in every
+// SpMV instead: vertices are renumbered by descending stored-entry count
+// inside each connected block (ties by ascending index).  On a power-law
+// graph the few thousand hubs that most rows point at then sit at the front
+// of every vector, inside one XCD's L2, instead of scattered over the whole
+// index range.  Coarse levels inherit the order (coarse ids follow their
+// roots), so the locality carries down the hierarchy.
+#include "prims.hpp"
+#include "setup.hpp"
+
+namespace lg {
+
+namespace {
+
+// key = block << 32 | (2^32 - 1 - row length): an ascending stable sort
+// gives blocks in order, long rows first, ties by ascending index
+__global__ void k_order_keys(int64_t n, const int32_t *rp, const int64_t *bptr, int nb, uint64_t *keys,
+                             uint64_t *vals) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = nb;  // bptr[lo] <= i < bptr[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (bptr[mid] <= i) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t len = (uint32_t)(rp[i + 1] - rp[i]);
+    keys[i] = ((uint64_t)lo << 32) | (uint64_t)(0xFFFFFFFFu - len);
+    vals[i] = (uint64_t)i;
+  }
+}
+
+__global__ void k_order_perm(int64_t n, const uint64_t *vals, const int32_t *rp, int32_t *perm,
+                             int32_t *iperm, int32_t *len) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t o = (int32_t)vals[i];
+    perm[i] = o;
+    iperm[o] = (int32_t)i;
+    len[i] = rp[o + 1] - rp[o];
+  }
+}
+
+// one wavefront per new row: (new row << cb | new column, old position)
+__global__ void k_order_entries(int64_t n, const int32_t *rp, const int32_t *ci, const int32_t *perm,
+                                const int32_t *iperm, const int32_t *nrp, int cb, uint64_t *keys,
+                                uint64_t *vals) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
+  for (int64_t r = w0; r < n; r += nw) {
+    const int32_t o = perm[r];
+    const int32_t a = rp[o], len = rp[o + 1] - a, d = nrp[r];
+    for (int32_t t = lane; t < len; t += WAVE) {
+      keys[d + t] = ((uint64_t)r << cb) | (uint64_t)iperm[ci[a + t]];
+      vals[d + t] = (uint64_t)(a + t);
+    }
+  }
+}
+
+__global__ void k_order_unpack(int64_t m, const uint64_t *keys, const uint64_t *vals, uint64_t mask,
+                               const double *v, int32_t *nci, double *nv) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    nci[k] = (int32_t)(keys[k] & mask);
+    nv[k] = v[vals[k]];
+  }
+}
+
+__global__ void k_gather(int64_t n, const int32_t *perm, const double *src, double *dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[perm[i]];
+}
+
+__global__ void k_scatter(int64_t n, const int32_t *perm, const double *src, double *dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[perm[i]] = src[i];
+}
+
+}  // namespace
+
+void degree_order(DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &perm, hipStream_t s) {
+  const int64_t n = A.n, m = A.nnz;
+  LG_CHECK(blocks.size() >= 2 && blocks.front() == 0 && blocks.back() == n, LIGMG_ERR_INVALID,
+           "degree_order: blocks must span [0, n]");
+  const int nb = (int)blocks.size() - 1;
+  DevBuf<int32_t> iperm(n), len(n + 1);
+  perm.alloc(n);
+  {
+    DevBuf<int64_t> bptr(blocks.size());
+    to_device(bptr.p, blocks.data(), blocks.size(), s);
+    DevBuf<uint64_t> keys(n), vals(n);
+    hipLaunchKernelGGL(k_order_keys, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, A.rp.p, bptr.p, nb,
+                       keys.p, vals.p);
+    LG_LAUNCH_CHECK();
+    radix_sort_pairs(keys.p, vals.p, n, 32 + bits_for((uint64_t)nb), s);
+    hipLaunchKernelGGL(k_order_perm, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, vals.p, A.rp.p,
+                       perm.p, iperm.p, len.p);
+    LG_LAUNCH_CHECK();
+  }
+  DCsr B;
+  B.n = n;
+  B.ncols = A.ncols;
+  B.rp.alloc(n + 1);
+  fill_i32(len.p + n, 0, 1, s);
+  const int64_t total = exclusive_scan_i32(len.p, B.rp.p, n + 1, s);
+  LG_CHECK(total == m, LIGMG_ERR_INVALID, "degree_order: row pointers inconsistent");
+  B.nnz = m;
+  B.ci.alloc(std::max<int64_t>(m, 1));
+  B.v.alloc(std::max<int64_t>(m, 1));
+  if (m > 0) {
+    const int cb = bits_for((uint64_t)n);
+    DevBuf<uint64_t> keys(m), vals(m);
+    hipLaunchKernelGGL(k_order_entries, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p,
+                       A.ci.p, perm.p, iperm.p, B.rp.p, cb, keys.p, vals.p);
+    LG_LAUNCH_CHECK();
+    radix_sort_pairs(keys.p, vals.p, m, 2 * cb, s);
+    hipLaunchKernelGGL(k_order_unpack, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, m, keys.p, vals.p,
+                       ((uint64_t)1 << cb) - 1, A.v.p, B.ci.p, B.v.p);
+    LG_LAUNCH_CHECK();
+  }
+  LG_HIP(hipStreamSynchronize(s));
+  A = std::move(B);
+}
+
+void gather_vec(int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, perm, src, dst);
+  LG_LAUNCH_CHECK();
+}
+
+void scatter_vec(int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, perm, src, dst);
+  LG_LAUNCH_CHECK();
+}
+
+}  // namespace lg

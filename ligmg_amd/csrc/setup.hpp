@@ -24,6 +24,13 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
 void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLevel &L, DCsr &coarse,
                        hipStream_t s);
 
+// Renumber A's vertices by descending row length inside each block (ties
+// by ascending index); perm receives internal position -> input vertex.
+void degree_order(DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &perm, hipStream_t s);
+// dst[i] = src[perm[i]]  /  dst[perm[i]] = src[i]
+void gather_vec(int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s);
+void scatter_vec(int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s);
+
 void setup_dinv(const DCsr &A, DLevel &L, bool fp32, hipStream_t s);
 double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_idx, int iters,
                     hipStream_t s);

@@ -198,6 +198,9 @@ struct Solver {
   std::vector<std::vector<double>> al_pre, be_pre, al_post, be_post;
   KWork<double> outer;
   DevBuf<double> b0, sol, tmp0, dsc;
+  // internal position -> caller's vertex (empty under LIGMG_ORDER_NATURAL)
+  DevBuf<int32_t> perm;
+  DevBuf<double> rhs_base;  // reduced right-hand side of the outer solve
   double setup_time = 0;
   bool mixed = false;
   // captured preconditioner applications, keyed by (level, r, z)
@@ -249,9 +252,10 @@ struct Solver {
       d2d(cur.v.p, v, cur.nnz, s);
     }
     LG_HIP(hipStreamSynchronize(s));
+    if (o.ordering == LIGMG_ORDER_DEGREE) degree_order(cur, blocks0, perm, s);
+    auto last = t0;
     auto lap = [&](const char *what, size_t l, int64_t nn, int64_t nz) {
       if (!o.verbose) return;
-      static auto last = std::chrono::steady_clock::now();
       auto now = std::chrono::steady_clock::now();
       fprintf(stderr, "[ligmg] level %zu %-6s n=%lld nnz=%lld  %.3f s\n", l, what, (long long)nn,
               (long long)nz, std::chrono::duration<double>(now - last).count());
@@ -913,7 +917,7 @@ int Solver::solve_batch(int k, const double *Bin, double *Xout, int mem, ligmg_i
   else d2d(bin.p, Bin, n * k, s);
   LG_HIP(hipStreamSynchronize(s));
   auto t0 = std::chrono::steady_clock::now();
-  op_to_block(bin.p, n, k, kp, b0b.p, s);
+  op_to_block(bin.p, n, k, kp, b0b.p, perm.p, s);
   if (do_once) op_bspmv<double>(L0.R, b0b.p, rhs.p, kp, bparts, s);
   else d2d(rhs.p, b0b.p, n * kp, s);
   red[base].bdots<double>(1, kp, rhs.p, nullptr, nullptr, nullptr, bsc_out.p, bpart_red, s);
@@ -925,7 +929,7 @@ int Solver::solve_batch(int k, const double *Bin, double *Xout, int mem, ligmg_i
                               o.iters, true, &its, &status, true);
   if (do_once) op_binterp<double>(L0.P, L0.q.p, bouter.x.p, b0b.p, solb.p, kp, bparts, s);
   else d2d(solb.p, bouter.x.p, n * kp, s);
-  op_from_block(solb.p, n, k, kp, bin.p, s);
+  op_from_block(solb.p, n, k, kp, bin.p, perm.p, s);
   LG_HIP(hipStreamSynchronize(s));
   auto t1 = std::chrono::steady_clock::now();
   // full residual per column (worst column reported)
@@ -1039,8 +1043,18 @@ void Solver::scale_vec(T *p, int64_t n, double a) {
 int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double *hist, int cap) {
   const DLevel &L0 = H[0];
   const int64_t n = L0.A.n;
-  if (mem == LIGMG_MEM_HOST) to_device(b0.p, b, n, s);
-  else d2d(b0.p, b, n, s);
+  if (perm.p) {
+    const double *src = b;
+    if (mem == LIGMG_MEM_HOST) {
+      to_device(tmp0.p, b, n, s);
+      src = tmp0.p;
+    }
+    gather_vec(n, perm.p, src, b0.p, s);
+  } else if (mem == LIGMG_MEM_HOST) {
+    to_device(b0.p, b, n, s);
+  } else {
+    d2d(b0.p, b, n, s);
+  }
   LG_HIP(hipStreamSynchronize(s));
   auto t0 = std::chrono::steady_clock::now();
   const bool do_once = o.top_elim && L0.kind == LIGMG_LEVEL_ELIM;
@@ -1048,8 +1062,8 @@ int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double 
   const DLevel &B = H[base];
   const int64_t nb_ = B.A.n;
   if (outer.x.n < (size_t)nb_) outer.alloc(nb_, red[base].nb);
-  DevBuf<double> rhsb(nb_);
-  double *rhs = rhsb.p;
+  rhs_base.ensure(std::max<int64_t>(nb_, 1));
+  double *rhs = rhs_base.p;
   if (do_once) op_spmv<double>(L0.R, b0.p, rhs, s);
   else d2d(rhs, b0.p, n, s);
   red[base].sums<double>(rhs, dsc.p, s);
@@ -1074,8 +1088,18 @@ int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double 
     rr += d[c * 2];
     bb += d[c * 2 + 1];
   }
-  if (mem == LIGMG_MEM_HOST) to_host(x, sol.p, n, s);
-  else d2d(x, sol.p, n, s);
+  if (perm.p) {
+    if (mem == LIGMG_MEM_HOST) {
+      scatter_vec(n, perm.p, sol.p, tmp0.p, s);
+      to_host(x, tmp0.p, n, s);
+    } else {
+      scatter_vec(n, perm.p, sol.p, x, s);
+    }
+  } else if (mem == LIGMG_MEM_HOST) {
+    to_host(x, sol.p, n, s);
+  } else {
+    d2d(x, sol.p, n, s);
+  }
   LG_HIP(hipStreamSynchronize(s));
   if (info) {
     // cycle_complexity (multigrid.hpp:372-387), ConvergenceInfo (util.hpp:143-170)
@@ -1170,7 +1194,8 @@ void ligmg_options_default(ligmg_options *o) {
   o->cheby_eig_iters = 10;
   o->precision = LIGMG_PREC_FP64;
   o->use_graphs = 1;
-  o->spmv_slice = -1;
+  o->spmv_slice = 0;
+  o->ordering = LIGMG_ORDER_DEGREE;
 }
 
 const char *ligmg_last_error(void) { return lg::g_last_error.c_str(); }
@@ -1238,6 +1263,23 @@ int ligmg_solve(ligmg_handle h, const double *b, double *x, int mem_kind, ligmg_
 int ligmg_destroy(ligmg_handle h) {
   delete h;
   return LIGMG_OK;
+}
+
+int ligmg_ordering(ligmg_handle h, int32_t *perm) {
+  if (!h || !perm) return LIGMG_ERR_INVALID;
+  try {
+    lg::Solver &S = h->S;
+    const int64_t n = S.H[0].A.n;
+    if (S.perm.p) {
+      lg::to_host(perm, S.perm.p, n, S.s);
+      LG_HIP(hipStreamSynchronize(S.s));
+    } else {
+      for (int64_t i = 0; i < n; i++) perm[i] = (int32_t)i;
+    }
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
 }
 
 int ligmg_num_levels(ligmg_handle h, int32_t *out) {
@@ -1405,7 +1447,9 @@ int ligmg_bench_spmv(ligmg_handle h, int32_t l, int32_t prec, int32_t iters, dou
     (void)hipEventDestroy(e1);
     *avg_ms = ms / iters;
     const int64_t sv = f32 ? 4 : 8, sx = f32 ? 4 : 8;
-    *bytes = A.nnz * (sv + 4) + (n + 1) * 4 + 2 * n * sx;
+    // values + columns + row pointers + x read once + y written once; the
+    // FP32 Laplacian form also reads its FP64 diagonal
+    *bytes = A.nnz * (sv + 4) + (n + 1) * 4 + 2 * n * sx + (f32 && A.d32.p ? n * 8 : 0);
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return fail(e);

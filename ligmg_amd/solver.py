@@ -228,6 +228,14 @@ class Solver:
         return X, info, its
 
     # -------------------------------------------------------- introspection
+    def ordering(self):
+        """Internal vertex order: entry i is the caller's index of internal
+        vertex i (identity under ORDER_NATURAL).  Level introspection
+        reports matrices and maps in this order."""
+        p = np.empty(self.n, np.int32)
+        _check(self._lib.ligmg_ordering(self._h, p.ctypes.data), "ligmg_ordering")
+        return p
+
     def num_levels(self):
         n = C.c_int32()
         _check(self._lib.ligmg_num_levels(self._h, C.byref(n)), "ligmg_num_levels")

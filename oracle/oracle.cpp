@@ -83,6 +83,46 @@ Csr make_csr(int64_t n, const int32_t *rp, const int32_t *ci, const double *v) {
   return A;
 }
 
+// ---- vertex order (new; the reference only permutes, randomly, across
+// ranks: options.hpp:119, main.cpp:52-63).  Descending stored-entry count
+// inside each block, ties by ascending index; the permuted matrix keeps
+// its columns sorted.
+std::vector<int32_t> degree_order(const Csr &A, const std::vector<int64_t> &blocks) {
+  std::vector<int32_t> perm(A.n);
+  std::iota(perm.begin(), perm.end(), 0);
+  auto len = [&](int32_t i) { return A.rp[i + 1] - A.rp[i]; };
+  for (size_t c = 0; c + 1 < blocks.size(); c++)
+    std::stable_sort(perm.begin() + blocks[c], perm.begin() + blocks[c + 1],
+                     [&](int32_t a, int32_t b) { return len(a) > len(b); });
+  return perm;
+}
+
+Csr permute_csr(const Csr &A, const std::vector<int32_t> &perm) {
+  std::vector<int32_t> inv(A.n);
+  for (int64_t i = 0; i < A.n; i++) inv[perm[i]] = (int32_t)i;
+  Csr B;
+  B.n = A.n;
+  B.rp.assign(A.n + 1, 0);
+  for (int64_t i = 0; i < A.n; i++) B.rp[i + 1] = B.rp[i] + (A.rp[perm[i] + 1] - A.rp[perm[i]]);
+  B.ci.resize(A.ci.size());
+  B.v.resize(A.v.size());
+  std::vector<std::pair<int32_t, double>> row;
+  for (int64_t i = 0; i < A.n; i++) {
+    const int32_t o = perm[i];
+    row.clear();
+    for (int32_t k = A.rp[o]; k < A.rp[o + 1]; k++) row.push_back({inv[A.ci[k]], A.v[k]});
+    std::sort(row.begin(), row.end(),
+              [](const std::pair<int32_t, double> &a, const std::pair<int32_t, double> &b) {
+                return a.first < b.first;
+              });
+    for (size_t t = 0; t < row.size(); t++) {
+      B.ci[B.rp[i] + t] = row[t].first;
+      B.v[B.rp[i] + t] = row[t].second;
+    }
+  }
+  return B;
+}
+
 double diag_of(const Csr &A, int64_t i) {
   for (int32_t k = A.rp[i]; k < A.rp[i + 1]; k++)
     if (A.ci[k] == i) return A.v[k];
@@ -128,6 +168,7 @@ struct Solver {
   ligmg_options opts;
   std::vector<Level> H;
   std::vector<int64_t> comp_ptr;  // blocks of level 0
+  std::vector<int32_t> perm;      // internal position -> caller's vertex (empty: identity)
   double setup_time = 0;
 };
 
@@ -1059,7 +1100,8 @@ void oracle_default_options(ligmg_options *o) {
   o->cheby_eig_iters = 10;
   o->precision = LIGMG_PREC_FP64;
   o->use_graphs = 1;
-  o->spmv_slice = -1;
+  o->spmv_slice = 0;
+  o->ordering = LIGMG_ORDER_DEGREE;
 }
 
 int oracle_create(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
@@ -1076,7 +1118,12 @@ int oracle_create(int64_t n, const int32_t *rp, const int32_t *ci, const double 
     if (ncomp > 0 && comp_ptr) S->comp_ptr.assign(comp_ptr, comp_ptr + ncomp + 1);
     else S->comp_ptr = {0, n};
     auto t0 = std::chrono::steady_clock::now();
-    make_hierarchy(*S, make_csr(n, rp, ci, v));
+    Csr A = make_csr(n, rp, ci, v);
+    if (S->opts.ordering == LIGMG_ORDER_DEGREE) {
+      S->perm = degree_order(A, S->comp_ptr);
+      A = permute_csr(A, S->perm);
+    }
+    make_hierarchy(*S, std::move(A));
     S->setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     *out = S;
     return 0;
@@ -1093,10 +1140,22 @@ int oracle_destroy(oracle_handle h) {
 int oracle_solve(oracle_handle h, const double *b, double *x, ligmg_info *info, double *hist,
                  int32_t cap) {
   try {
-    return solve(*h, b, x, info, hist, cap);
+    if (h->perm.empty()) return solve(*h, b, x, info, hist, cap);
+    const int64_t n = (int64_t)h->perm.size();
+    std::vector<double> bp(n), xp(n);
+    for (int64_t i = 0; i < n; i++) bp[i] = b[h->perm[i]];
+    int rc = solve(*h, bp.data(), xp.data(), info, hist, cap);
+    for (int64_t i = 0; i < n; i++) x[h->perm[i]] = xp[i];
+    return rc;
   } catch (...) {
     return LIGMG_ERR_NUMERIC;
   }
+}
+
+int oracle_ordering(oracle_handle h, int32_t *perm) {
+  const int64_t n = h->H[0].A.n;
+  for (int64_t i = 0; i < n; i++) perm[i] = h->perm.empty() ? (int32_t)i : h->perm[i];
+  return 0;
 }
 
 int oracle_num_levels(oracle_handle h, int32_t *out) {

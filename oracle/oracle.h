@@ -35,6 +35,7 @@ int oracle_create(int64_t n, const int32_t *rowptr, const int32_t *colind,
                   const double *vals, const ligmg_options *opts, int32_t ncomp,
                   const int64_t *comp_ptr, oracle_handle *out);
 int oracle_destroy(oracle_handle h);
+int oracle_ordering(oracle_handle h, int32_t *perm);
 int oracle_solve(oracle_handle h, const double *b, double *x, ligmg_info *info,
                  double *history, int32_t history_cap);
 int oracle_num_levels(oracle_handle h, int32_t *out);

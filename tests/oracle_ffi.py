@@ -20,6 +20,7 @@ PROTOS = {
     "oracle_create": (C.c_int, [i64, P(i32), P(i32), P(f64), P(_ffi.ligmg_options), i32, P(i64),
                                 P(H)]),
     "oracle_destroy": (C.c_int, [H]),
+    "oracle_ordering": (C.c_int, [H, C.c_void_p]),
     "oracle_solve": (C.c_int, [H, P(f64), P(f64), P(_ffi.ligmg_info), P(f64), i32]),
     "oracle_num_levels": (C.c_int, [H, P(i32)]),
     "oracle_level_info": (C.c_int, [H, i32, P(i32), P(i64), P(i64), P(i64)]),
@@ -104,6 +105,11 @@ class OracleSolver:
         n = i32()
         lib().oracle_num_levels(self.h, C.byref(n))
         return n.value
+
+    def ordering(self):
+        p = np.empty(self.g.n, np.int32)
+        assert lib().oracle_ordering(self.h, p.ctypes.data) == 0
+        return p
 
     def level_info(self, l):
         k, n, nnz, nc = i32(), i64(), i64(), i64()

@@ -48,13 +48,15 @@ def opts_pair(**kw):
     return ligmg_amd.Options(**kw), O.default_options(**kw)
 
 
+@pytest.mark.parametrize("ordering", [_ffi.ORDER_DEGREE, _ffi.ORDER_NATURAL])
 @pytest.mark.parametrize("case", CASES)
-def test_hierarchy_bit_exact(case, my_matrix):
+def test_hierarchy_bit_exact(case, ordering, my_matrix):
     g = make_case(case, my_matrix)
-    print(f"case={case} n={g.n} nnz={g.nnz}", flush=True)
-    do, oo = opts_pair()
+    print(f"case={case} ordering={ordering} n={g.n} nnz={g.nnz}", flush=True)
+    do, oo = opts_pair(ordering=ordering)
     d = ligmg_amd.Solver(g, do)
     r = O.OracleSolver(g, oo)
+    assert np.array_equal(d.ordering(), r.ordering())
     assert d.num_levels() == r.num_levels()
     for l in range(d.num_levels()):
         assert d.level_info(l) == r.level_info(l), f"level {l}"

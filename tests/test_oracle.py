@@ -14,6 +14,12 @@ import oracle_ffi as O
 from ligmg_amd import _ffi, graphs
 
 
+def NATURAL(**kw):
+    """Options keeping the input's vertex order (tests that compare level-0
+    maps with the input's indices)."""
+    return O.default_options(ordering=_ffi.ORDER_NATURAL, **kw)
+
+
 def lap_from_edges(n, edges, w=None):
     u = np.array([e[0] for e in edges], np.int32)
     v = np.array([e[1] for e in edges], np.int32)
@@ -124,7 +130,7 @@ def test_grid_eliminates_only_vertex_zero():
     g = graphs.grid2d(16, 16)
     f, nk = elim_select(g)
     assert np.array_equal(np.where(f)[0], [0])
-    s = O.OracleSolver(g)
+    s = O.OracleSolver(g, NATURAL())
     assert s.level_info(0)[0] == _ffi.LEVEL_AGG
 
 
@@ -133,7 +139,7 @@ def test_schur_complement_and_interpolation():
     (pattern and values), P 1 = 1, and x = P x_c + Q b solves L x = b when
     x_c solves the Schur system (elimination.hpp:33-52, 287-340)."""
     g = graphs.rmat(8, 4, wkind=_ffi.W_UNIFORM, p0=0.5, p1=1.5, pendant_frac=0.3, seed=5)
-    s = O.OracleSolver(g)
+    s = O.OracleSolver(g, NATURAL())
     assert s.level_info(0)[0] == _ffi.LEVEL_ELIM
     newid = s.level_map(0)
     kept = np.where(newid >= 0)[0]
@@ -163,7 +169,7 @@ def test_schur_complement_and_interpolation():
 
 def test_aggregation_galerkin_is_RAP():
     g = graphs.grid2d(20, 17, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=3)
-    s = O.OracleSolver(g)
+    s = O.OracleSolver(g, NATURAL())
     assert s.level_info(0)[0] == _ffi.LEVEL_AGG
     agg = s.level_map(0)
     nc = s.level_info(0)[3]
@@ -289,7 +295,7 @@ def test_aggregates_are_stars(seed):
     """Every vertex maps to a root that is itself or a strength neighbour;
     roots map to themselves; coarse ids ascend with the root index."""
     g = graphs.rmat(9, 8, wkind=_ffi.W_LOGNORMAL, p0=0, p1=1.5, seed=seed)
-    s = O.OracleSolver(g, O.default_options(always_elim=0))
+    s = O.OracleSolver(g, NATURAL(always_elim=0))
     agg = s.level_map(0)
     ind, st = s.level_states(0)
     A = g.to_scipy()
@@ -456,3 +462,50 @@ def test_canon_sum_chunks_long_sequences():
     assert O.canon_sum(a[:4096]) == 1e16  # a single chunk
     b = np.random.default_rng(2).standard_normal(20000)
     assert abs(O.canon_sum(b) - math.fsum(b)) < 1e-12 * np.abs(b).sum()
+
+
+# ------------------------------------------------------------ vertex order
+
+
+def test_degree_order_is_stable_descending_within_blocks():
+    """ORDER_DEGREE: inside each block, descending stored-entry count with
+    ties by ascending input index (numpy lexsort restates it); blocks keep
+    their vertices."""
+    gs = [graphs.barabasi_albert(n, 3, seed=s) for s, n in enumerate([500, 120, 900])]
+    u = graphs.union(gs)
+    s = O.OracleSolver(u, comp_ptr=u.comp_ptr)
+    p = s.ordering()
+    ln = np.diff(u.rowptr)
+    cp = u.comp_ptr
+    for c in range(len(gs)):
+        idx = np.arange(cp[c], cp[c + 1])
+        ref = idx[np.lexsort((idx, -ln[idx]))]
+        assert np.array_equal(p[cp[c]:cp[c + 1]], ref)
+    assert np.array_equal(O.OracleSolver(u, NATURAL(), comp_ptr=u.comp_ptr).ordering(), np.arange(u.n))
+
+
+def test_ordered_level0_is_permuted_input():
+    g = graphs.rmat_config(10, "lognormal", pendants=True)
+    s = O.OracleSolver(g)
+    p = s.ordering()
+    A = g.to_scipy()
+    B = A[p][:, p].tocsr()
+    B.sort_indices()
+    rp, ci, v = s.level_matrix(0)
+    assert np.array_equal(rp, B.indptr) and np.array_equal(ci, B.indices)
+    assert np.array_equal(v, B.data)
+
+
+@pytest.mark.parametrize("kcycle", [0, 1])
+def test_ordering_leaves_solution_unchanged(kcycle):
+    """Solutions come back in the caller's order and agree with the
+    natural-order solve (a different hierarchy) within 1e-6."""
+    g = graphs.rmat_config(11, "uniform", pendants=True)
+    b = graphs.rhs(g.n, 1, 2)
+    kw = dict(kcycle=kcycle, outer_ksp_type=_ffi.KSP_FCG if kcycle else _ffi.KSP_CG)
+    x1, i1, _ = O.OracleSolver(g, O.default_options(**kw)).solve(b)
+    x0, i0, _ = O.OracleSolver(g, NATURAL(**kw)).solve(b)
+    assert i1.rel_residual <= 1e-8 and i0.rel_residual <= 1e-8
+    xm, ym = x1 - x1.mean(), x0 - x0.mean()
+    assert np.linalg.norm(xm - ym) <= 1e-6 * np.linalg.norm(ym)
+    assert np.linalg.norm(b - g.to_scipy() @ x1) <= 1e-7 * np.linalg.norm(b)
