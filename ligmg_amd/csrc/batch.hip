@@ -54,7 +54,7 @@ struct BCheb {
   __device__ __forceinline__ void operator()(int64_t i, int c, double s) const {
     const int64_t o = i * kp + c;
     double cur = (double)xc[o];
-    double d = xp ? alpha * (cur - (double)xp[o]) : 0.0;
+    double d = alpha * (cur - (xp ? (double)xp[o] : 0.0));  // null xp: zero previous iterate
     xn[o] = (T)(cur + d + beta * (double)dinv[i] * ((double)b[o] - s));
   }
 };

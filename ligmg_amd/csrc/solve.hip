@@ -47,7 +47,9 @@ struct EpiCheb {
   double alpha, beta;
   __device__ __forceinline__ void operator()(int64_t i, double s) const {
     double c = (double)xc[i];
-    double d = xp ? alpha * (c - (double)xp[i]) : 0.0;
+    // a null xp is a zero previous iterate (second step from a zero guess);
+    // every other caller passes alpha = 0 with it
+    double d = alpha * (c - (xp ? (double)xp[i] : 0.0));
     xn[i] = (T)(c + d + beta * (double)dinv[i] * ((double)b[i] - s));
   }
 };

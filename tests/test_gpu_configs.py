@@ -43,11 +43,15 @@ def test_grid256_vcycle_matches_oracle():
     for l in range(d.num_levels()):
         assert d.level_info(l) == r.level_info(l), f"level {l}"
     info = d.solve(b)
-    xr, rinfo, _ = r.solve(b)
+    xr, rinfo, rhist = r.solve(b)
     print(f"grid256 levels={d.num_levels()} iters={info.iters()} oracle={rinfo.iters} "
           f"rel={info.rel_residual():.2e} setup={d.setup_time:.3f}s solve={info.time():.4f}s")
     assert info.status == _ffi.CONVERGED and info.rel_residual() <= 1e-8
-    assert abs(info.iters() - rinfo.iters) <= 1
+    # same method: the residual histories coincide early on; rounding in
+    # the (differently ordered) inner products moves the last iterations
+    h = np.asarray(info.residuals)
+    assert np.allclose(h[:10], rhist[:10], rtol=1e-6)
+    assert abs(info.iters() - rinfo.iters) <= 2
     assert centered_rel(info.solution, xr) <= 1e-6
     assert host_residual(g, b, info.solution) <= 1e-7
 
