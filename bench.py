@@ -26,6 +26,22 @@ METRIC = "time to 1e-8 rel. residual: FP32-precond vs FP64 on GPU; SpMV HBM GB/s
 HBM_PEAK_GBS = 8000.0
 
 
+def traffic_of(kernel):
+    """HBM bytes per launch of a kernel, from the committed counter summary
+    (FETCH_SIZE x read calibration + WRITE_SIZE x write calibration, both
+    calibrated on a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_rmat23.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 2 and parts[0] == kernel + ":":
+                    return float(parts[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(scale, nnz_full):
     """The CPU oracle (single thread) on a bounded sample of the same
     construction, scaled to the full workload by stored entries."""
@@ -100,19 +116,24 @@ def main():
     # FP64-only solve (the plain method, the drop-in): V-cycle + CG
     verbose = int(os.environ.get("LIGMG_VERBOSE", "0"))
     order = _ffi.ORDER_DEGREE if args.ordering == "degree" else _ffi.ORDER_NATURAL
+    if args.spmv_only:
+        # base-level kernels only (counter runs): plain SpMV and fused
+        # Chebyshev step, FP64 and FP32, 20 launches each
+        sm = ligmg_amd.Solver(A, ligmg_amd.Options(precision=_ffi.PREC_MIXED, verbose=verbose,
+                                                    ordering=order))
+        lib = _ffi.solver_lib()
+        ms, nbytes = C.c_double(), C.c_int64()
+        base = 1 if sm.level_info(0)[0] == _ffi.LEVEL_ELIM else 0
+        for fn in ("ligmg_bench_spmv", "ligmg_bench_smoother"):
+            for prec in (_ffi.PREC_FP64, _ffi.PREC_MIXED):
+                rc = getattr(lib, fn)(sm._h, base, prec, 20, C.byref(ms), C.byref(nbytes))
+                assert rc == 0
+                print(f"# {fn} prec={prec} {ms.value:.4f} ms {nbytes.value} B "
+                      f"{nbytes.value / ms.value / 1e6:.1f} GB/s", file=sys.stderr, flush=True)
+        return
     s64 = ligmg_amd.Solver(A, ligmg_amd.Options(verbose=verbose, ordering=order))
     print(f"# fp64 setup {s64.setup_time:.2f}s levels {s64.num_levels()}", file=sys.stderr,
           flush=True)
-    if args.spmv_only:
-        lib = _ffi.solver_lib()
-        ms, nbytes = C.c_double(), C.c_int64()
-        for prec in (_ffi.PREC_FP64, _ffi.PREC_MIXED):
-            rc = lib.ligmg_bench_spmv(s64._h, 1 if s64.level_info(0)[0] == _ffi.LEVEL_ELIM else 0,
-                                      prec, 20, C.byref(ms), C.byref(nbytes))
-            assert rc == 0
-            print(f"# spmv prec={prec} {ms.value:.4f} ms {nbytes.value / ms.value / 1e6:.1f} GB/s",
-                  file=sys.stderr, flush=True)
-        return
     info64 = None
     t64 = []
     for _ in range(args.fp64_steps):
@@ -130,6 +151,9 @@ def main():
     rc = lib.ligmg_bench_spmv(s64._h, base, _ffi.PREC_MIXED, 20, C.byref(ms), C.byref(nbytes))
     assert rc == 0
     spmv32 = {"ms": ms.value, "bytes": nbytes.value, "gbs": nbytes.value / ms.value / 1e6}
+    rc = lib.ligmg_bench_smoother(s64._h, base, _ffi.PREC_FP64, 20, C.byref(ms), C.byref(nbytes))
+    assert rc == 0
+    cheb64 = {"ms": ms.value, "bytes": nbytes.value, "gbs": nbytes.value / ms.value / 1e6}
     lvl = s64.level_info(base)
     setup64 = s64.setup_time
     x64 = info64.solution
@@ -158,6 +182,15 @@ def main():
           f"full {info32.full_rel_residual:.2e} diff_vs_fp64 {err:.2e}", file=sys.stderr,
           flush=True)
     assert info32.rel_residual() <= 1e-8 and info64.rel_residual() <= 1e-8
+    # roofline of the dominant kernel of the timed solve: the FP32 fused
+    # Chebyshev-step SpMV on the base level, HIP events on the solver's stream
+    rc = lib.ligmg_bench_smoother(s32._h, base, _ffi.PREC_MIXED, 20, C.byref(ms), C.byref(nbytes))
+    assert rc == 0
+    cheb32 = {"ms": ms.value, "bytes": nbytes.value, "gbs": nbytes.value / ms.value / 1e6}
+    print(f"# base-level smoother step fp32 {cheb32['ms']:.4f} ms {cheb32['gbs']:.0f} GB/s; "
+          f"fp64 {cheb64['ms']:.4f} ms {cheb64['gbs']:.0f} GB/s; spmv fp64 {spmv64['ms']:.4f} ms "
+          f"{spmv64['gbs']:.0f} GB/s, fp32 {spmv32['ms']:.4f} ms {spmv32['gbs']:.0f} GB/s",
+          file=sys.stderr, flush=True)
 
     ms_step = 1e3 * wall / args.steps
     out = {
@@ -188,15 +221,16 @@ def main():
         "solution_diff_fp32_vs_fp64": err,
         "setup_s_fp64": setup64, "setup_s_mixed": s32.setup_time,
         "graph_gen_s": t_gen,
-        "spmv_fp64": spmv64, "spmv_fp32": spmv32,
+        "spmv_fp64": spmv64, "spmv_fp32": spmv32, "smoother_fp64": cheb64, "smoother_fp32": cheb32,
         "roofline": {
             "bound": "hbm",
-            "achieved": spmv64["gbs"],
+            "achieved": cheb32["gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": spmv64["gbs"] / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_spmv_units (FP64 base-level Laplacian SpMV)",
+            "frac": cheb32["gbs"] / HBM_PEAK_GBS,
+            "traffic": traffic_of("smoother_fp32"),
+            "kernel": ("k_spmv_units<float, ValArr<float>, EpiDiag<float, EpiCheb<float>>> "
+                       "(base-level FP32 Chebyshev step)"),
         },
     }
     if not args.no_cpu:

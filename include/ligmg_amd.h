@@ -218,6 +218,14 @@ int ligmg_level_cheby(ligmg_handle h, int32_t level, double *lo, double *hi);
 int ligmg_bench_spmv(ligmg_handle h, int32_t level, int32_t precision,
                      int32_t iters, double *avg_ms, int64_t *bytes);
 
+/* (new) Times `iters` launches of one fused Chebyshev smoothing step on
+ * aggregation level `level`, x_new = x + a (x - x_prev) + b D^-1 (r - A x),
+ * in the given precision (the dominant kernel of a V-cycle).  Algorithmic
+ * bytes of one launch: nnz*(s_v + 4) + (n+1)*4 + 5*n*s_x, plus n*8 for the
+ * FP64 diagonal of the FP32 Laplacian form. */
+int ligmg_bench_smoother(ligmg_handle h, int32_t level, int32_t precision,
+                         int32_t iters, double *avg_ms, int64_t *bytes);
+
 /* y = A x with level `level`'s solve-phase operator (the same kernels and
  * layout the cycles use: column slices, FP32 Laplacian form for
  * LIGMG_PREC_MIXED).  x and y are host arrays of length n in FP64. */

@@ -1456,6 +1456,66 @@ int ligmg_bench_spmv(ligmg_handle h, int32_t l, int32_t prec, int32_t iters, dou
   }
 }
 
+extern "C++" {
+namespace {
+template <class T>
+double time_cheb(lg::Solver &S, lg::DLevel &L, const T *dinv, int iters, hipStream_t s) {
+  const int64_t n = L.A.n;
+  lg::DevBuf<T> b(n), x0(n), x1(n), x2(n);
+  {
+    lg::DevBuf<double> t(n);
+    lg::fill_f64(t.p, 1.0, n, s);
+    for (T *dst : {b.p, x0.p, x1.p}) {
+      if constexpr (std::is_same<T, float>::value) lg::op_d2f(t.p, dst, n, s);
+      else lg::d2d(dst, t.p, n, s);
+    }
+    LG_HIP(hipStreamSynchronize(s));
+  }
+  const double al = 0.5, be = 0.5 / std::max(L.cheb_hi, 1e-300);
+  // rotate three buffers as the smoother does
+  T *B[3] = {x0.p, x1.p, x2.p};
+  auto step = [&](int k) {
+    lg::op_cheb_step<T>(L.A, dinv, b.p, B[(k + 1) % 3], B[k % 3], B[(k + 2) % 3], al, be, s);
+  };
+  step(0);
+  hipEvent_t e0, e1;
+  LG_HIP(hipEventCreate(&e0));
+  LG_HIP(hipEventCreate(&e1));
+  LG_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < iters; i++) step(i + 1);
+  LG_HIP(hipEventRecord(e1, s));
+  LG_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  LG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / iters;
+}
+}  // namespace
+}
+
+int ligmg_bench_smoother(ligmg_handle h, int32_t l, int32_t prec, int32_t iters, double *avg_ms,
+                         int64_t *bytes) {
+  if (!h || l < 0 || l >= (int)h->S.H.size() || iters < 1 || !avg_ms || !bytes)
+    return LIGMG_ERR_INVALID;
+  try {
+    lg::Solver &S = h->S;
+    lg::DLevel &L = S.H[l];
+    LG_CHECK(L.kind == LIGMG_LEVEL_AGG, LIGMG_ERR_INVALID, "ligmg_bench_smoother: not an aggregation level");
+    const bool f32 = prec == LIGMG_PREC_MIXED;
+    LG_CHECK(!f32 || (L.A.v32.p && L.dinv32.p), LIGMG_ERR_INVALID,
+             "ligmg_bench_smoother: FP32 operators exist only with precision = LIGMG_PREC_MIXED");
+    const int64_t n = L.A.n;
+    if (f32) *avg_ms = time_cheb<float>(S, L, L.dinv32.p, iters, S.s);
+    else *avg_ms = time_cheb<double>(S, L, L.dinv.p, iters, S.s);
+    const int64_t sv = f32 ? 4 : 8, sx = f32 ? 4 : 8;
+    *bytes = L.A.nnz * (sv + 4) + (n + 1) * 4 + 5 * n * sx + (f32 && L.A.d32.p ? n * 8 : 0);
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
 int ligmg_level_spmv(ligmg_handle h, int32_t l, int32_t prec, const double *x, double *y) {
   if (!h || l < 0 || l >= (int)h->S.H.size() || !x || !y) return LIGMG_ERR_INVALID;
   try {

@@ -264,3 +264,28 @@ def test_sliced_solve_matches_unsliced(prec):
     assert x1.rel_residual() <= 1e-8 and abs(x1.iters() - x0.iters()) <= 1
     a, c = x0.solution, x1.solution
     assert np.linalg.norm((a - a.mean()) - (c - c.mean())) <= 1e-7 * np.linalg.norm(a - a.mean())
+
+
+def test_bench_hooks_report_algorithmic_bytes():
+    """The SpMV / smoother timing hooks run on the solver's stream and
+    return the algorithmic bytes of one launch."""
+    import ctypes as C
+    g = make_case("rmat12_pend")
+    s = ligmg_amd.Solver(g, ligmg_amd.Options(precision=_ffi.PREC_MIXED, outer_ksp_type=_ffi.KSP_FCG))
+    lib = _ffi.solver_lib()
+    base = 1 if s.level_info(0)[0] == _ffi.LEVEL_ELIM else 0
+    k, n, nnz, _ = s.level_info(base)
+    assert k == _ffi.LEVEL_AGG
+    ms, nb = C.c_double(), C.c_int64()
+    for prec, sv in ((_ffi.PREC_FP64, 8), (_ffi.PREC_MIXED, 4)):
+        assert lib.ligmg_bench_spmv(s._h, base, prec, 3, C.byref(ms), C.byref(nb)) == 0
+        assert ms.value > 0
+        assert nb.value == nnz * (sv + 4) + (n + 1) * 4 + 2 * n * sv + (8 * n if sv == 4 else 0)
+        assert lib.ligmg_bench_smoother(s._h, base, prec, 3, C.byref(ms), C.byref(nb)) == 0
+        assert nb.value == nnz * (sv + 4) + (n + 1) * 4 + 5 * n * sv + (8 * n if sv == 4 else 0)
+    # an elimination level has no smoother
+    if base == 1:
+        assert lib.ligmg_bench_smoother(s._h, 0, _ffi.PREC_FP64, 3, C.byref(ms), C.byref(nb)) != 0
+    # FP32 operators only exist for a mixed-precision handle
+    s64 = ligmg_amd.Solver(g)
+    assert lib.ligmg_bench_smoother(s64._h, base, _ffi.PREC_MIXED, 3, C.byref(ms), C.byref(nb)) != 0
