@@ -226,6 +226,13 @@ int ligmg_bench_spmv(ligmg_handle h, int32_t level, int32_t precision,
 int ligmg_bench_smoother(ligmg_handle h, int32_t level, int32_t precision,
                          int32_t iters, double *avg_ms, int64_t *bytes);
 
+/* z = M r: one cycle of the preconditioner from level `level` down (the
+ * V- or K-cycle of multigrid.hpp:86-130 with a zero initial guess) in the
+ * given precision.  r and z are host arrays of length n in FP64.  Mirrors
+ * the oracle's apply_cycle for parity tests. */
+int ligmg_apply_cycle(ligmg_handle h, int32_t level, int32_t precision,
+                      const double *r, double *z);
+
 /* y = A x with level `level`'s solve-phase operator (the same kernels and
  * layout the cycles use: column slices, FP32 Laplacian form for
  * LIGMG_PREC_MIXED).  x and y are host arrays of length n in FP64. */

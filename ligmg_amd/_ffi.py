@@ -94,6 +94,7 @@ SOLVER_PROTOS = {
     "ligmg_level_strength": (C.c_int, [HANDLE, i32, P(f64)]),
     "ligmg_level_cheby": (C.c_int, [HANDLE, i32, P(f64), P(f64)]),
     "ligmg_level_spmv": (C.c_int, [HANDLE, i32, i32, P(f64), P(f64)]),
+    "ligmg_apply_cycle": (C.c_int, [HANDLE, i32, i32, P(f64), P(f64)]),
     "ligmg_bench_spmv": (C.c_int, [HANDLE, i32, i32, i32, P(f64), P(i64)]),
     "ligmg_bench_smoother": (C.c_int, [HANDLE, i32, i32, i32, P(f64), P(i64)]),
     "ligmg_aggregate_with_strength": (C.c_int, [i64, P(i32), P(i32), P(f64),

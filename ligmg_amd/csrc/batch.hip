@@ -129,13 +129,14 @@ __global__ void __launch_bounds__(256)
 template <class Epi>
 __global__ void k_spmm_long(const int32_t *lrow, const int32_t *lpart, int64_t nlong, int kp,
                             const double *parts, Epi epi) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= nlong * kp) return;
-  const int64_t j = t / kp;
-  const int c = (int)(t % kp);
-  double s = 0;
-  for (int p = lpart[j]; p < lpart[j + 1]; p++) s += parts[(int64_t)p * kp + c];
-  epi(lrow[j], c, s);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nlong * kp;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = t / kp;
+    const int c = (int)(t % kp);
+    double s = 0;
+    for (int p = lpart[j]; p < lpart[j + 1]; p++) s += parts[(int64_t)p * kp + c];
+    epi(lrow[j], c, s);
+  }
 }
 
 int log2i(int k) {

@@ -286,15 +286,76 @@ __global__ void k_p_fill(int n, const int32_t *rp, const int32_t *ci, const doub
     return;
   }
   bool f;
-  double d = row_diag(rp, ci, v, row, &f);
-  q[row] = 1 / d;
+  const double d = row_diag(rp, ci, v, row, &f);
+  // an isolated vertex (no nonzero off-diagonal: a block reduced to one
+  // vertex) gets 0: its value is a nullspace component
+  bool iso = true;
+  for (int k = rp[row]; k < rp[row + 1]; k++)
+    if (ci[k] != row && v[k] != 0) iso = false;
+  const double inv = iso ? 0.0 : 1 / d;
+  q[row] = inv;
   for (int k = rp[row]; k < rp[row + 1]; k++) {
     int i = ci[k];
     if (i == row) continue;
     pci[o] = newid[i];
-    pv[o] = 1 / d * -v[k];
+    pv[o] = inv * -v[k];
     o++;
   }
+}
+
+// FP32 elimination operators from the FP32 Laplacian form of the level
+// matrix (off-diagonals v32, diagonal d32 = -sum of them): q32 = 1 / d32,
+// P32(e, b) = -v32(e, b) / d32(e), so every P32 row sums to 1 again and the
+// FP32 cycle eliminates exactly the operator it smooths with.
+__global__ void k_p_fill32(int n, const int32_t *rp, const int32_t *ci, const float *v32, const double *d32,
+                           const int32_t *newid, const int32_t *prp, float *pv32, float *q32) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  int o = prp[row];
+  if (newid[row] >= 0) {
+    pv32[o] = 1.0f;
+    q32[row] = 0.0f;
+    return;
+  }
+  const double d = d32[row];
+  const double inv = d != 0 ? 1 / d : 0.0;
+  q32[row] = (float)inv;
+  for (int k = rp[row]; k < rp[row + 1]; k++) {
+    if (ci[k] == row) continue;
+    pv32[o++] = (float)(inv * -(double)v32[k]);
+  }
+}
+
+__global__ void k_p_transpose_keys32(int n, const int32_t *prp, const int32_t *pci, const float *pv32,
+                                     int rb, uint64_t *keys, uint64_t *vals) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  for (int k = prp[row]; k < prp[row + 1]; k++) {
+    keys[k] = ((uint64_t)pci[k] << rb) | (uint64_t)row;
+    vals[k] = (uint64_t)__float_as_uint(pv32[k]);
+  }
+}
+
+__global__ void k_unpack_vals32(const uint64_t *vals, int64_t m, float *v32) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v32[i] = __uint_as_float((uint32_t)vals[i]);
+}
+
+// inverse diagonal for the smoother; isolated rows get 0 (see k_p_fill)
+__global__ void k_dinv_rows(int n, const int32_t *rp, const int32_t *ci, const double *v, double *dinv,
+                            float *dinv32) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  double d = 0;
+  bool iso = true;
+  for (int k = rp[row]; k < rp[row + 1]; k++) {
+    if (ci[k] == row) d = v[k];
+    else if (v[k] != 0) iso = false;
+  }
+  const double r = (d != 0 && !iso) ? 1.0 / d : 0.0;
+  dinv[row] = r;
+  if (dinv32) dinv32[row] = (float)r;
 }
 
 // ---- a6: Schur contraction entries, in the oracle's list order: rows in
@@ -752,15 +813,6 @@ __global__ void k_members_unpack(int n, const uint64_t *keys, const uint64_t *va
   }
 }
 
-__global__ void k_dinv(int n, const double *diag, const uint8_t *hasd, double *dinv, float *dinv32) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double d = hasd[i] ? diag[i] : 0.0;
-  double r = d != 0 ? 1.0 / d : 0.0;
-  dinv[i] = r;
-  if (dinv32) dinv32[i] = (float)r;
-}
-
 unsigned g1(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256); }
 
 }  // namespace
@@ -888,6 +940,30 @@ bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &c
   LG_LAUNCH_CHECK();
   contract_to_csr(keys, vals, m, n_kept, n_kept, coarse, s);
   return true;
+}
+
+void make_elim32(DLevel &L, hipStream_t s) {
+  const DCsr &A = L.A;
+  LG_CHECK(A.v32.p && A.d32.p, LIGMG_ERR_INVALID, "make_elim32: level matrix has no FP32 Laplacian form");
+  const int64_t n = A.n, nnzP = L.P.nnz, n_kept = L.nc;
+  L.P.v32.alloc(std::max<int64_t>(nnzP, 1));
+  L.q32.alloc(std::max<int64_t>(n, 1));
+  hipLaunchKernelGGL(k_p_fill32, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v32.p, A.d32.p,
+                     L.new_id.p, L.P.rp.p, L.P.v32.p, L.q32.p);
+  LG_LAUNCH_CHECK();
+  // R32 = P32^T with the same (stable) transposition that built R
+  DevBuf<uint64_t> keys(nnzP), vals(nnzP);
+  const int rb = std::max(1, bits_for((uint64_t)n));
+  const int cb = std::max(1, bits_for((uint64_t)n_kept));
+  hipLaunchKernelGGL(k_p_transpose_keys32, dim3(g1(n)), dim3(256), 0, s, (int)n, L.P.rp.p, L.P.ci.p,
+                     L.P.v32.p, rb, keys.p, vals.p);
+  LG_LAUNCH_CHECK();
+  radix_sort_pairs(keys.p, vals.p, nnzP, rb + cb, s);
+  L.R.v32.alloc(std::max<int64_t>(nnzP, 1));
+  hipLaunchKernelGGL(k_unpack_vals32, dim3(grid_for(nnzP, 256, 8192)), dim3(256), 0, s, vals.p, nnzP,
+                     L.R.v32.p);
+  LG_LAUNCH_CHECK();
+  LG_HIP(hipStreamSynchronize(s));
 }
 
 // ----------------------------------------------------------------------------
@@ -1055,13 +1131,9 @@ void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLe
 // Inverse diagonal of A (0 where the diagonal is missing or zero).
 void setup_dinv(const DCsr &A, DLevel &L, bool fp32, hipStream_t s) {
   const int64_t n = A.n;
-  DevBuf<double> diag(n);
-  DevBuf<uint8_t> hasd(n);
-  hipLaunchKernelGGL(k_diag, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p, diag.p, hasd.p);
-  LG_LAUNCH_CHECK();
   L.dinv.alloc(n);
   if (fp32) L.dinv32.alloc(n);
-  hipLaunchKernelGGL(k_dinv, dim3(g1(n)), dim3(256), 0, s, (int)n, diag.p, hasd.p, L.dinv.p,
+  hipLaunchKernelGGL(k_dinv_rows, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p, L.dinv.p,
                      fp32 ? L.dinv32.p : nullptr);
   LG_LAUNCH_CHECK();
   LG_HIP(hipStreamSynchronize(s));

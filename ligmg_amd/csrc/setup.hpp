@@ -32,6 +32,9 @@ void gather_vec(int64_t n, const int32_t *perm, const double *src, double *dst, 
 void scatter_vec(int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s);
 
 void setup_dinv(const DCsr &A, DLevel &L, bool fp32, hipStream_t s);
+// FP32 P, R = P^T and Q of an elimination level from the FP32 Laplacian form
+// of its matrix (L.A.v32 / L.A.d32 must exist).
+void make_elim32(DLevel &L, hipStream_t s);
 double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_idx, int iters,
                     hipStream_t s);
 std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,

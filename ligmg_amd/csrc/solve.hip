@@ -192,11 +192,12 @@ __global__ void __launch_bounds__(SPMV_BLOCK)
 template <class Epi>
 __global__ void k_spmv_long(const int32_t *lrow, const int32_t *lpart, int64_t nlong,
                             const double *parts, Epi epi) {
-  int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (j >= nlong) return;
-  double s = 0;
-  for (int p = lpart[j]; p < lpart[j + 1]; p++) s += parts[p];
-  epi(lrow[j], s);
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nlong;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0;
+    for (int p = lpart[j]; p < lpart[j + 1]; p++) s += parts[p];
+    epi(lrow[j], s);
+  }
 }
 
 template <class T, class Val, class Epi>
@@ -276,20 +277,21 @@ __global__ void k_prolong_add(const int32_t *agg, int64_t n, const T *xc, T *x) 
 __global__ void k_lap32(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
                         float *v32, double *d32) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
-  if (row >= n) return;
-  double s = 0;
-  for (int k = rp[row] + lane; k < rp[row + 1]; k += WAVE) {
-    if (ci[k] == row) {
-      v32[k] = 0.0f;
-    } else {
-      float f = (float)v[k];
-      v32[k] = f;
-      s += (double)f;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
+  for (int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; row < n; row += nw) {
+    double s = 0;
+    for (int k = rp[row] + lane; k < rp[row + 1]; k += WAVE) {
+      if (ci[k] == row) {
+        v32[k] = 0.0f;
+      } else {
+        float f = (float)v[k];
+        v32[k] = f;
+        s += (double)f;
+      }
     }
+    s = wave_sum(s);
+    if (lane == 0) d32[row] = -s;
   }
-  s = wave_sum(s);
-  if (lane == 0) d32[row] = -s;
 }
 
 __global__ void k_d2f(const double *a, float *b, int64_t n) {
@@ -308,12 +310,13 @@ template <class T>
 __global__ void k_dense_apply(const double *__restrict__ Pm, int64_t n, const T *__restrict__ x,
                               T *__restrict__ y) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
-  if (row >= n) return;
-  double s = 0;
-  for (int64_t j = lane; j < n; j += WAVE) s += Pm[row * n + j] * (double)x[j];
-  s = wave_sum(s);
-  if (lane == 0) y[row] = (T)s;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
+  for (int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; row < n; row += nw) {
+    double s = 0;
+    for (int64_t j = lane; j < n; j += WAVE) s += Pm[row * n + j] * (double)x[j];
+    s = wave_sum(s);
+    if (lane == 0) y[row] = (T)s;
+  }
 }
 
 // ---------------------------------------------------------------- reductions

@@ -343,12 +343,7 @@ struct Solver {
       if (mixed) {
         make_v32(L.A, s, true);
         if (L.kind == LIGMG_LEVEL_AGG) set_dinv32_from_lap(L);
-        if (L.kind == LIGMG_LEVEL_ELIM) {
-          make_v32(L.P, s);
-          make_v32(L.R, s);
-          L.q32.alloc(L.n);
-          op_d2f(L.q.p, L.q32.p, L.n, s);
-        }
+        if (L.kind == LIGMG_LEVEL_ELIM) make_elim32(L, s);
       }
       alloc_work<double>(l);
       if (mixed) alloc_work<float>(l);
@@ -1510,6 +1505,33 @@ int ligmg_bench_smoother(ligmg_handle h, int32_t l, int32_t prec, int32_t iters,
     else *avg_ms = time_cheb<double>(S, L, L.dinv.p, iters, S.s);
     const int64_t sv = f32 ? 4 : 8, sx = f32 ? 4 : 8;
     *bytes = L.A.nnz * (sv + 4) + (n + 1) * 4 + 5 * n * sx + (f32 && L.A.d32.p ? n * 8 : 0);
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_apply_cycle(ligmg_handle h, int32_t l, int32_t prec, const double *r, double *z) {
+  if (!h || l < 0 || l >= (int)h->S.H.size() || !r || !z) return LIGMG_ERR_INVALID;
+  try {
+    lg::Solver &S = h->S;
+    const bool f32 = prec == LIGMG_PREC_MIXED;
+    LG_CHECK(!f32 || S.mixed, LIGMG_ERR_INVALID,
+             "ligmg_apply_cycle: FP32 cycles exist only with precision = LIGMG_PREC_MIXED");
+    const int64_t n = S.H[l].A.n;
+    lg::DevBuf<double> dr(n), dz(n);
+    lg::to_device(dr.p, r, n, S.s);
+    if (f32) {
+      lg::DevBuf<float> fr(n);
+      lg::op_d2f(dr.p, fr.p, n, S.s);
+      const float *out = S.cycle<float>((size_t)l, fr.p);
+      lg::op_f2d(out, dz.p, n, S.s);
+    } else {
+      const double *out = S.cycle<double>((size_t)l, dr.p);
+      lg::d2d(dz.p, out, n, S.s);
+    }
+    lg::to_host(z, dz.p, n, S.s);
+    LG_HIP(hipStreamSynchronize(S.s));
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return fail(e);

@@ -295,6 +295,15 @@ class Solver:
                                           y.ctypes.data_as(P(C.c_double))), "ligmg_level_spmv")
         return y
 
+    def apply_cycle(self, l, r, precision=_ffi.PREC_FP64):
+        """z = M r: one preconditioner cycle from level l (host arrays)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty_like(r)
+        P = C.POINTER
+        _check(self._lib.ligmg_apply_cycle(self._h, l, precision, r.ctypes.data_as(P(C.c_double)),
+                                           z.ctypes.data_as(P(C.c_double))), "ligmg_apply_cycle")
+        return z
+
     def print(self):
         """print_hierarchy (multigrid.hpp:265-292)."""
         print(f"{'Level':>5}  {'Size':>12}  {'NNZ':>14}  {'Type':>6}")

@@ -123,6 +123,16 @@ Csr permute_csr(const Csr &A, const std::vector<int32_t> &perm) {
   return B;
 }
 
+// A row with no nonzero off-diagonal is an isolated vertex: a connected
+// block reduced to one vertex, whose diagonal is the round-off (or exact 0)
+// of a Galerkin sum that cancels.  Its value is a nullspace component, so
+// the smoother and the elimination give it 0 instead of 1 / (round-off).
+bool isolated(const Csr &A, int64_t i) {
+  for (int32_t k = A.rp[i]; k < A.rp[i + 1]; k++)
+    if (A.ci[k] != i && A.v[k] != 0) return false;
+  return true;
+}
+
 double diag_of(const Csr &A, int64_t i) {
   for (int32_t k = A.rp[i]; k < A.rp[i + 1]; k++)
     if (A.ci[k] == i) return A.v[k];
@@ -257,15 +267,16 @@ bool single_elimination(const Csr &A, const ligmg_options &o, Level &L, Csr &coa
         if (!elim[i]) E.push_back({L.new_id[j], L.new_id[i], A.v[k]});
       }
     } else {
-      double d = diag_of(A, j);
-      L.q[j] = 1 / d;
+      const double d = diag_of(A, j);
+      const double inv = isolated(A, j) ? 0.0 : 1 / d;
+      L.q[j] = inv;
       size_t first = L.P.ci.size();
       for (int32_t k = A.rp[j]; k < A.rp[j + 1]; k++) {
         int32_t i = A.ci[k];
         if (i == j) continue;
         if (L.new_id[i] < 0) throw std::runtime_error("adjacent eliminated vertices");
         L.P.ci.push_back(L.new_id[i]);
-        L.P.v.push_back(1 / d * -A.v[k]);
+        L.P.v.push_back(inv * -A.v[k]);
       }
       // Schur contributions: (a, b) += L_ea * P(e, b)
       size_t last = L.P.ci.size();
@@ -725,7 +736,7 @@ void aggregation_level(const Csr &A, const ligmg_options &o, int level_idx, Leve
   L.dinv.resize(A.n);
   for (int64_t i = 0; i < A.n; i++) {
     double d = diag_of(A, i);
-    L.dinv[i] = d != 0 ? 1.0 / d : 0.0;
+    L.dinv[i] = (d != 0 && !isolated(A, i)) ? 1.0 / d : 0.0;
   }
   if (o.smoother_type == LIGMG_SMOOTHER_CHEBYSHEV) {
     L.lam = eig_estimate(A, L.dinv, o.seed, level_idx, o.cheby_eig_iters);
@@ -1296,7 +1307,7 @@ int oracle_cheby_apply(int64_t n, const int32_t *rp, const int32_t *ci, const do
   std::vector<double> dinv(n);
   for (int64_t i = 0; i < n; i++) {
     double d = diag_of(A, i);
-    dinv[i] = d != 0 ? 1.0 / d : 0.0;
+    dinv[i] = (d != 0 && !isolated(A, i)) ? 1.0 / d : 0.0;
   }
   cheby_apply(A, dinv, lo, hi, steps, x, b);
   return 0;
@@ -1308,7 +1319,7 @@ int oracle_eig_estimate(int64_t n, const int32_t *rp, const int32_t *ci, const d
   std::vector<double> dinv(n);
   for (int64_t i = 0; i < n; i++) {
     double d = diag_of(A, i);
-    dinv[i] = d != 0 ? 1.0 / d : 0.0;
+    dinv[i] = (d != 0 && !isolated(A, i)) ? 1.0 / d : 0.0;
   }
   *lam = eig_estimate(A, dinv, seed, level, iters);
   return 0;

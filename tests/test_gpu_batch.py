@@ -89,3 +89,30 @@ def test_graph_replays_match_eager():
         gr = ligmg_amd.Solver(g, ligmg_amd.Options(use_graphs=1, **prec_opts))
         for i, b in enumerate(bs):
             assert np.array_equal(gr.solve(b).solution, e.solve(b).solution)
+
+
+@pytest.mark.parametrize("prec", [_ffi.PREC_FP64, _ffi.PREC_MIXED])
+def test_union_with_collapsing_blocks(prec):
+    """Small blocks collapse to isolated vertices mid-hierarchy (and get
+    eliminated): FP64 and FP32-preconditioned solves, single and batched,
+    still converge on every (block, column)."""
+    sizes = [40000, 30, 7, 2500, 3, 90000, 12, 600]
+    gs = [graphs.barabasi_albert(n, 5 if n > 10 else 1, seed=i) for i, n in enumerate(sizes)]
+    u = graphs.union(gs)
+    k = 8
+    B = graphs.rhs(u.n, k, 2, comp_ptr=u.comp_ptr).reshape(u.n, k, order="F")
+    s = ligmg_amd.Solver(u, ligmg_amd.Options(precision=prec, outer_ksp_type=_ffi.KSP_FCG, iters=200),
+                         comp_ptr=u.comp_ptr)
+    print("levels", [s.level_info(l) for l in range(s.num_levels())])
+    one = s.solve(np.ascontiguousarray(B[:, 0]))
+    assert one.status == _ffi.CONVERGED, one.iters()
+    X, info, its = s.solve_batch(B)
+    assert info.status == _ffi.CONVERGED, its
+    A = u.to_scipy()
+    R = B - A @ X
+    cp = u.comp_ptr
+    for blk in range(len(gs)):
+        lo, hi = cp[blk], cp[blk + 1]
+        nb = np.linalg.norm(B[lo:hi], axis=0)
+        nr = np.linalg.norm(R[lo:hi], axis=0)
+        assert np.all(nr <= 1e-7 * nb + 1e-300), (blk, nr / nb)

@@ -289,3 +289,20 @@ def test_bench_hooks_report_algorithmic_bytes():
     # FP32 operators only exist for a mixed-precision handle
     s64 = ligmg_amd.Solver(g)
     assert lib.ligmg_bench_smoother(s64._h, base, _ffi.PREC_MIXED, 3, C.byref(ms), C.byref(nb)) != 0
+
+
+def test_fp32_operator_beyond_four_million_rows():
+    """The FP32 Laplacian form covers every row of a level larger than
+    2^22 rows (a wavefront-per-row kernel must stride over the grid)."""
+    import scipy.sparse as sp
+    g = graphs.grid2d(2100, 2100, wkind=_ffi.W_UNIFORM, p0=0.5, p1=2.0, seed=9)
+    assert g.n > (1 << 22)
+    d = ligmg_amd.Solver(g, ligmg_amd.Options(precision=_ffi.PREC_MIXED))
+    assert d.level_info(0)[0] == _ffi.LEVEL_AGG
+    rp, ci, v = d.level_matrix(0)
+    A = sp.csr_matrix((v, ci, rp), shape=(g.n, g.n))
+    x = np.random.default_rng(1).standard_normal(g.n)
+    y = d.level_spmv(0, x, _ffi.PREC_MIXED)
+    err = np.abs(y - A @ x) / (abs(A) @ abs(x))
+    print(f"max rel err first 4M rows {err[:1 << 22].max():.2e}, beyond {err[1 << 22:].max():.2e}")
+    assert err.max() <= 2e-6

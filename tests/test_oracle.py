@@ -509,3 +509,37 @@ def test_ordering_leaves_solution_unchanged(kcycle):
     xm, ym = x1 - x1.mean(), x0 - x0.mean()
     assert np.linalg.norm(xm - ym) <= 1e-6 * np.linalg.norm(ym)
     assert np.linalg.norm(b - g.to_scipy() @ x1) <= 1e-7 * np.linalg.norm(b)
+
+
+def collapsing_union():
+    """A union whose small blocks collapse to single (isolated) vertices
+    while the large ones still coarsen: the Galerkin diagonal of such a
+    vertex is an exact cancellation (0 with unit weights)."""
+    tiny = [graphs.from_edges(3, np.array([0, 1]), np.array([1, 2]), np.ones(2)),
+            graphs.from_edges(2, np.array([0]), np.array([1]), np.ones(1)),
+            graphs.from_edges(4, np.array([0, 1, 2, 0]), np.array([1, 2, 3, 2]), np.ones(4))]
+    gs = [graphs.barabasi_albert(3000, 5, seed=1)] + tiny + [graphs.barabasi_albert(500, 2, seed=2)]
+    return graphs.union(gs)
+
+
+def test_isolated_vertices_get_zero_smoother_and_solve_converges():
+    u = collapsing_union()
+    r = O.OracleSolver(u, comp_ptr=u.comp_ptr)
+    seen = 0
+    for l in range(r.num_levels()):
+        rp, ci, v = r.level_matrix(l)
+        n = len(rp) - 1
+        for i in range(n):
+            seg = slice(rp[i], rp[i + 1])
+            if np.all((ci[seg] == i) | (v[seg] == 0)):
+                seen += 1
+    assert seen > 0  # the case is exercised
+    b = graphs.rhs(u.n, 1, 3, comp_ptr=u.comp_ptr)
+    x, info, _ = r.solve(b)
+    assert np.all(np.isfinite(x)) and info.rel_residual <= 1e-8
+    A = u.to_scipy()
+    cp = u.comp_ptr
+    for c in range(len(cp) - 1):
+        lo, hi = cp[c], cp[c + 1]
+        res = b[lo:hi] - (A @ x)[lo:hi]
+        assert np.linalg.norm(res) <= 1e-7 * np.linalg.norm(b[lo:hi])
