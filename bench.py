@@ -78,8 +78,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-scale", type=int, default=18)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--ordering", choices=["degree", "natural"], default="degree",
-                    help="vertex order of the hierarchy (default: degree-sorted)")
+    ap.add_argument("--ordering", choices=["degree", "natural"], default="natural",
+                    help="vertex order of the hierarchy (default: the input's)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="setup + SpMV timing + copy calibration only (for counter runs)")
     args = ap.parse_args()

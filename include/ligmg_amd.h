@@ -63,11 +63,13 @@ enum {
 
 /* Vertex order of the hierarchy (new).  The reference randomly permutes
  * the input when it runs on more than one rank, for load balance
- * (Options::rand_perm, options.hpp:119; main.cpp:52-63).  On one GPU the
- * permutation serves gather locality instead: DEGREE renumbers vertices by
- * descending stored-entry count (ties by ascending index) inside each
- * connected block of comp_ptr, so the most-gathered entries of every vector
- * share a few MiB of L2.  Right-hand sides and solutions stay in the
+ * (Options::rand_perm, options.hpp:119; main.cpp:52-63), and keeps the
+ * input's order on one rank: NATURAL, the default, does the same.  DEGREE
+ * renumbers vertices by descending stored-entry count (ties by ascending
+ * index) inside each connected block of comp_ptr, so the most-gathered
+ * entries of every vector share a few MiB of L2 (faster SpMVs); it also
+ * changes the aggregation's tie-breaks and therefore the hierarchy and
+ * the iteration count.  Right-hand sides and solutions stay in the
  * caller's order; level introspection reports the internal order, and
  * ligmg_ordering returns the permutation. */
 enum { LIGMG_ORDER_NATURAL = 0, LIGMG_ORDER_DEGREE = 1 };
@@ -114,7 +116,7 @@ typedef struct ligmg_options {
   int32_t use_graphs;           /* 1: replay cycles from captured hipGraphs  */
   int32_t spmv_slice;           /* 0: no column slicing of the solve SpMVs;
                                    >0: slice width (cols); -1: autotune     */
-  int32_t ordering;             /* LIGMG_ORDER_DEGREE: vertex order used for
+  int32_t ordering;             /* LIGMG_ORDER_NATURAL: vertex order used for
                                    setup and solve (see enum)               */
 } ligmg_options;
 

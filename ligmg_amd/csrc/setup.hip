@@ -342,20 +342,28 @@ __global__ void k_unpack_vals32(const uint64_t *vals, int64_t m, float *v32) {
     v32[i] = __uint_as_float((uint32_t)vals[i]);
 }
 
-// inverse diagonal for the smoother; isolated rows get 0 (see k_p_fill)
-__global__ void k_dinv_rows(int n, const int32_t *rp, const int32_t *ci, const double *v, double *dinv,
-                            float *dinv32) {
-  int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n) return;
-  double d = 0;
-  bool iso = true;
-  for (int k = rp[row]; k < rp[row + 1]; k++) {
-    if (ci[k] == row) d = v[k];
-    else if (v[k] != 0) iso = false;
+// inverse diagonal for the smoother; isolated rows get 0 (see k_p_fill).
+// One wavefront per row (hub rows have hundreds of thousands of entries).
+__global__ void k_dinv_rows(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
+                            double *dinv, float *dinv32) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
+  for (int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; row < n; row += nw) {
+    double d = 0;
+    bool off = false;
+    for (int k = rp[row] + lane; k < rp[row + 1]; k += WAVE) {
+      if (ci[k] == row) d = v[k];
+      else if (v[k] != 0) off = true;
+    }
+    // the diagonal sits in exactly one lane (sorted, unique columns)
+    for (int o = WAVE / 2; o >= 1; o >>= 1) d += __shfl_xor(d, o, WAVE);
+    const bool iso = !__any(off);
+    if (lane == 0) {
+      const double r = (d != 0 && !iso) ? 1.0 / d : 0.0;
+      dinv[row] = r;
+      if (dinv32) dinv32[row] = (float)r;
+    }
   }
-  const double r = (d != 0 && !iso) ? 1.0 / d : 0.0;
-  dinv[row] = r;
-  if (dinv32) dinv32[row] = (float)r;
 }
 
 // ---- a6: Schur contraction entries, in the oracle's list order: rows in
@@ -1133,8 +1141,8 @@ void setup_dinv(const DCsr &A, DLevel &L, bool fp32, hipStream_t s) {
   const int64_t n = A.n;
   L.dinv.alloc(n);
   if (fp32) L.dinv32.alloc(n);
-  hipLaunchKernelGGL(k_dinv_rows, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p, L.dinv.p,
-                     fp32 ? L.dinv32.p : nullptr);
+  hipLaunchKernelGGL(k_dinv_rows, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p, A.ci.p,
+                     A.v.p, L.dinv.p, fp32 ? L.dinv32.p : nullptr);
   LG_LAUNCH_CHECK();
   LG_HIP(hipStreamSynchronize(s));
 }

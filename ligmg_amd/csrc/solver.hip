@@ -1190,7 +1190,7 @@ void ligmg_options_default(ligmg_options *o) {
   o->precision = LIGMG_PREC_FP64;
   o->use_graphs = 1;
   o->spmv_slice = 0;
-  o->ordering = LIGMG_ORDER_DEGREE;
+  o->ordering = LIGMG_ORDER_NATURAL;
 }
 
 const char *ligmg_last_error(void) { return lg::g_last_error.c_str(); }

@@ -45,9 +45,12 @@ class _Owner:
 
     def __init__(self, g):
         self.g = g
+        # bound now: at interpreter exit module globals may already be gone
+        self._free = _ffi.graphs_lib().ligmg_graph_free
+        self._byref = C.byref
 
     def __del__(self):
-        _ffi.graphs_lib().ligmg_graph_free(C.byref(self.g))
+        self._free(self._byref(self.g))
 
 
 def _view(owner, addr, ctype, count, dtype):

@@ -1112,7 +1112,7 @@ void oracle_default_options(ligmg_options *o) {
   o->precision = LIGMG_PREC_FP64;
   o->use_graphs = 1;
   o->spmv_slice = 0;
-  o->ordering = LIGMG_ORDER_DEGREE;
+  o->ordering = LIGMG_ORDER_NATURAL;
 }
 
 int oracle_create(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,

@@ -306,3 +306,26 @@ def test_fp32_operator_beyond_four_million_rows():
     err = np.abs(y - A @ x) / (abs(A) @ abs(x))
     print(f"max rel err first 4M rows {err[:1 << 22].max():.2e}, beyond {err[1 << 22:].max():.2e}")
     assert err.max() <= 2e-6
+
+
+@pytest.mark.parametrize("case", ["rmat12_pend", "grid_w", "ba"])
+def test_cycle_matches_oracle(case, my_matrix):
+    """One preconditioner cycle from the base level: FP64 against the
+    oracle's cycle (same method, summation order aside), symmetric, and
+    the FP32 cycle within FP32 accuracy of it."""
+    g = make_case(case, my_matrix)
+    d = ligmg_amd.Solver(g, ligmg_amd.Options(precision=_ffi.PREC_MIXED))
+    r = O.OracleSolver(g)
+    base = 1 if d.level_info(0)[0] == _ffi.LEVEL_ELIM else 0
+    n = d.level_info(base)[1]
+    rng = np.random.default_rng(4)
+    u, w = rng.standard_normal(n), rng.standard_normal(n)
+    u -= u.mean()
+    w -= w.mean()
+    zu = d.apply_cycle(base, u)
+    ref = r.apply_cycle(base, u)
+    assert np.linalg.norm(zu - ref) <= 1e-10 * np.linalg.norm(ref)
+    zw = d.apply_cycle(base, w)
+    assert abs(zu @ w - u @ zw) <= 1e-10 * abs(zu @ w)
+    z32 = d.apply_cycle(base, u, _ffi.PREC_MIXED)
+    assert np.linalg.norm(z32 - zu) <= 1e-4 * np.linalg.norm(zu)

@@ -14,6 +14,11 @@ import oracle_ffi as O
 from ligmg_amd import _ffi, graphs
 
 
+def DEGREE(**kw):
+    """Options with the degree-sorted vertex order."""
+    return O.default_options(ordering=_ffi.ORDER_DEGREE, **kw)
+
+
 def NATURAL(**kw):
     """Options keeping the input's vertex order (tests that compare level-0
     maps with the input's indices)."""
@@ -473,7 +478,7 @@ def test_degree_order_is_stable_descending_within_blocks():
     their vertices."""
     gs = [graphs.barabasi_albert(n, 3, seed=s) for s, n in enumerate([500, 120, 900])]
     u = graphs.union(gs)
-    s = O.OracleSolver(u, comp_ptr=u.comp_ptr)
+    s = O.OracleSolver(u, DEGREE(), comp_ptr=u.comp_ptr)
     p = s.ordering()
     ln = np.diff(u.rowptr)
     cp = u.comp_ptr
@@ -486,7 +491,7 @@ def test_degree_order_is_stable_descending_within_blocks():
 
 def test_ordered_level0_is_permuted_input():
     g = graphs.rmat_config(10, "lognormal", pendants=True)
-    s = O.OracleSolver(g)
+    s = O.OracleSolver(g, DEGREE())
     p = s.ordering()
     A = g.to_scipy()
     B = A[p][:, p].tocsr()
@@ -503,7 +508,7 @@ def test_ordering_leaves_solution_unchanged(kcycle):
     g = graphs.rmat_config(11, "uniform", pendants=True)
     b = graphs.rhs(g.n, 1, 2)
     kw = dict(kcycle=kcycle, outer_ksp_type=_ffi.KSP_FCG if kcycle else _ffi.KSP_CG)
-    x1, i1, _ = O.OracleSolver(g, O.default_options(**kw)).solve(b)
+    x1, i1, _ = O.OracleSolver(g, DEGREE(**kw)).solve(b)
     x0, i0, _ = O.OracleSolver(g, NATURAL(**kw)).solve(b)
     assert i1.rel_residual <= 1e-8 and i0.rel_residual <= 1e-8
     xm, ym = x1 - x1.mean(), x0 - x0.mean()
