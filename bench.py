@@ -105,13 +105,12 @@ def main():
          torch.from_numpy(g.vals).to(dev))
     torch.cuda.synchronize()
 
-    # copy kernel of known size: calibrates HBM counters (FETCH_SIZE /
-    # WRITE_SIZE) against 2 GiB read + 2 GiB written
-    src = torch.ones(1 << 28, dtype=torch.float64, device=dev)
-    dst = torch.empty_like(src)
-    dst.copy_(src)
+    # kernels of known traffic calibrate the HBM counters: a 2 GiB fill
+    # (WRITE_SIZE) and a 2 GiB sum (FETCH_SIZE)
+    cal = torch.ones(1 << 28, dtype=torch.float64, device=dev)
+    float(cal.sum())
     torch.cuda.synchronize()
-    del src, dst
+    del cal
 
     # FP64-only solve (the plain method, the drop-in): V-cycle + CG
     verbose = int(os.environ.get("LIGMG_VERBOSE", "0"))

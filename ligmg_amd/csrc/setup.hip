@@ -384,7 +384,9 @@ __global__ void __launch_bounds__(256) k_schur_fill(int n, const int32_t *rp, co
     b = rp[row + 1];
     kept = newid[row] >= 0;
   }
-  const bool lng = valid && kept && (b - a > WAVE);
+  // kept rows longer than 16 entries are copied by the whole wavefront
+  // (coalesced); shorter ones and eliminated rows by their own lane
+  const bool lng = valid && kept && (b - a > 16);
   if (valid && !lng) {
     int o = off[row];
     if (kept) {
@@ -438,13 +440,23 @@ __global__ void __launch_bounds__(256) k_schur_fill(int n, const int32_t *rp, co
 }
 
 // CSR from sorted unique keys: rows counted, columns = low bits.
-__global__ void k_key_rows(const uint64_t *keys, int64_t m, int cb, int32_t *cnt, int32_t *ci) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+// Row pointers of a CSR whose entries are keys (row << cb | col) sorted
+// ascending: rp[r] = first position whose row is >= r.  Position i writes
+// rp for the rows between its predecessor's row and its own, so every
+// rp entry is written exactly once (i = m closes the trailing rows).
+__global__ void k_sorted_rowptr(const uint64_t *keys, int64_t m, int cb, int64_t nrows, int32_t *rp) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= m;
        i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t k = keys[i];
-    atomicAdd(&cnt[k >> cb], 1);
-    ci[i] = (int32_t)(k & (((uint64_t)1 << cb) - 1));
+    const int64_t r = (i < m) ? (int64_t)(keys[i] >> cb) : nrows + 1;
+    const int64_t pr = (i > 0) ? (int64_t)(keys[i - 1] >> cb) : -1;
+    for (int64_t q = pr + 1; q <= r && q <= nrows; q++) rp[q] = (int32_t)i;
   }
+}
+
+__global__ void k_key_cols(const uint64_t *keys, int64_t m, int cb, int32_t *ci) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x)
+    ci[i] = (int32_t)(keys[i] & (((uint64_t)1 << cb) - 1));
 }
 
 // transpose of P: keys (col << rb) | row
@@ -458,13 +470,11 @@ __global__ void k_p_transpose_keys(int n, const int32_t *prp, const int32_t *pci
   }
 }
 
-__global__ void k_unpack_pairs(const uint64_t *keys, const uint64_t *vals, int64_t m, int rb,
-                               int32_t *cnt, int32_t *ci, double *v) {
+__global__ void k_unpack_pairs(const uint64_t *keys, const uint64_t *vals, int64_t m, int rb, int32_t *ci,
+                               double *v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t k = keys[i];
-    atomicAdd(&cnt[k >> rb], 1);
-    ci[i] = (int32_t)(k & (((uint64_t)1 << rb) - 1));
+    ci[i] = (int32_t)(keys[i] & (((uint64_t)1 << rb) - 1));
     v[i] = __longlong_as_double((long long)vals[i]);
   }
 }
@@ -812,13 +822,9 @@ __global__ void k_member_pairs(int n, const int32_t *agg, uint64_t *keys, uint64
   }
 }
 
-__global__ void k_members_unpack(int n, const uint64_t *keys, const uint64_t *vals, int32_t *cnt,
-                                 int32_t *mem) {
+__global__ void k_members_unpack(int n, const uint64_t *vals, int32_t *mem) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    atomicAdd(&cnt[keys[i]], 1);
-    mem[i] = (int32_t)vals[i];
-  }
+  if (i < n) mem[i] = (int32_t)vals[i];
 }
 
 unsigned g1(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256); }
@@ -844,12 +850,13 @@ void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, 
   out.rp.alloc(nrows + 1);
   out.ci.alloc(std::max<int64_t>(nu, 1));
   out.v.alloc(std::max<int64_t>(nu, 1));
-  fill_i32(out.rp.p, 0, nrows + 1, s);
-  if (nu)
-    hipLaunchKernelGGL(k_key_rows, dim3(grid_for(nu, 256, 8192)), dim3(256), 0, s, uk.p, nu, cb,
-                       out.rp.p, out.ci.p);
+  hipLaunchKernelGGL(k_sorted_rowptr, dim3(grid_for(nu + 1, 256, 8192)), dim3(256), 0, s, uk.p, nu, cb, nrows,
+                     out.rp.p);
   LG_LAUNCH_CHECK();
-  exclusive_scan_i32(out.rp.p, out.rp.p, nrows + 1, s);
+  if (nu) {
+    hipLaunchKernelGGL(k_key_cols, dim3(grid_for(nu, 256, 8192)), dim3(256), 0, s, uk.p, nu, cb, out.ci.p);
+    LG_LAUNCH_CHECK();
+  }
   d2d(out.v.p, us.p, nu, s);
   LG_HIP(hipStreamSynchronize(s));
 }
@@ -930,11 +937,12 @@ bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &c
     L.R.rp.alloc(n_kept + 1);
     L.R.ci.alloc(nnzP);
     L.R.v.alloc(nnzP);
-    fill_i32(L.R.rp.p, 0, n_kept + 1, s);
-    hipLaunchKernelGGL(k_unpack_pairs, dim3(grid_for(nnzP, 256, 8192)), dim3(256), 0, s, keys.p, vals.p,
-                       nnzP, rb, L.R.rp.p, L.R.ci.p, L.R.v.p);
+    hipLaunchKernelGGL(k_sorted_rowptr, dim3(grid_for(nnzP + 1, 256, 8192)), dim3(256), 0, s, keys.p, nnzP, rb,
+                       n_kept, L.R.rp.p);
     LG_LAUNCH_CHECK();
-    exclusive_scan_i32(L.R.rp.p, L.R.rp.p, n_kept + 1, s);
+    hipLaunchKernelGGL(k_unpack_pairs, dim3(grid_for(nnzP, 256, 8192)), dim3(256), 0, s, keys.p, vals.p,
+                       nnzP, rb, L.R.ci.p, L.R.v.p);
+    LG_LAUNCH_CHECK();
   }
   // Schur complement by contraction
   DevBuf<int32_t> cnt(n + 1);
@@ -1128,11 +1136,11 @@ void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLe
     L.M.nnz = n;
     L.M.rp.alloc(nc + 1);
     L.M.ci.alloc(n);
-    fill_i32(L.M.rp.p, 0, nc + 1, s);
-    hipLaunchKernelGGL(k_members_unpack, dim3(g1(n)), dim3(256), 0, s, (int)n, keys.p, vals.p, L.M.rp.p,
-                       L.M.ci.p);
+    hipLaunchKernelGGL(k_sorted_rowptr, dim3(grid_for(n + 1, 256, 8192)), dim3(256), 0, s, keys.p, n, 0, nc,
+                       L.M.rp.p);
     LG_LAUNCH_CHECK();
-    exclusive_scan_i32(L.M.rp.p, L.M.rp.p, nc + 1, s);
+    hipLaunchKernelGGL(k_members_unpack, dim3(g1(n)), dim3(256), 0, s, (int)n, vals.p, L.M.ci.p);
+    LG_LAUNCH_CHECK();
   }
 }
 
