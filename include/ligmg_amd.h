@@ -118,6 +118,10 @@ typedef struct ligmg_options {
                                    >0: slice width (cols); -1: autotune     */
   int32_t ordering;             /* LIGMG_ORDER_NATURAL: vertex order used for
                                    setup and solve (see enum)               */
+  int32_t solve_layout;         /* 1: the solve-phase copy of every large
+                                   level is stored in degree order (gather
+                                   locality; the hierarchy is unchanged);
+                                   0: setup order                           */
 } ligmg_options;
 
 /* ConvergenceInfo (util.hpp:143-170) plus the fields the reference only

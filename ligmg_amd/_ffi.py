@@ -31,6 +31,7 @@ class ligmg_options(C.Structure):
         ("cheby_eig_iters", i32), ("precision", i32), ("use_graphs", i32),
         ("spmv_slice", i32),
         ("ordering", i32),
+        ("solve_layout", i32),
     ]
 
 

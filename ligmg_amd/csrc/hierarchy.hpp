@@ -69,6 +69,12 @@ struct DLevel {
   double lam = 0, cheb_lo = 0, cheb_hi = 0;
   // coarse: dense pseudo-inverse, row-major n x n
   DevBuf<double> pinv;
+  // Solve layout: the solve-phase copies (A, dinv, agg, M, P, R, q) of a
+  // large level are renumbered by descending degree inside blocks; sperm maps
+  // a solve position to the setup position (empty: setup order kept),
+  // siperm is its inverse.  agg_setup / new_id keep the setup-order maps for
+  // introspection; new_id_s is new_id in solve order.
+  DevBuf<int32_t> sperm, siperm, agg_setup, new_id_s;
   std::vector<int64_t> blocks;  // contiguous vertex blocks (independent graphs)
 };
 

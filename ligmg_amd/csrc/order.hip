@@ -72,6 +72,41 @@ __global__ void k_order_unpack(int64_t m, const uint64_t *keys, const uint64_t *
   }
 }
 
+// B row i = A row rperm[i]; entries keep their order, columns relabeled by
+// ciperm (null: unchanged).  One wavefront per row.
+template <class V>
+__global__ void k_perm_rows(int64_t n, const int32_t *rp, const int32_t *ci, const V *v, const int32_t *rperm,
+                            const int32_t *ciperm, const int32_t *nrp, int32_t *nci, V *nv) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
+  for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; r < n; r += nw) {
+    const int32_t o = rperm ? rperm[r] : (int32_t)r;
+    const int32_t a = rp[o], len = rp[o + 1] - a, d = nrp[r];
+    for (int32_t t = lane; t < len; t += WAVE) {
+      const int32_t c = ci[a + t];
+      nci[d + t] = ciperm ? ciperm[c] : c;
+      if (v) nv[d + t] = v[a + t];
+    }
+  }
+}
+
+__global__ void k_perm_len(int64_t n, const int32_t *rp, const int32_t *rperm, int32_t *len) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t o = rperm ? rperm[i] : (int32_t)i;
+    len[i] = rp[o + 1] - rp[o];
+  }
+}
+
+// out[i] = niperm[map[perm[i]]] (null perm / niperm: identity)
+__global__ void k_relabel(int64_t n, const int32_t *perm, const int32_t *map, const int32_t *niperm, int32_t *out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t m = map[perm ? perm[i] : i];
+    out[i] = (niperm && m >= 0) ? niperm[m] : m;
+  }
+}
+
 __global__ void k_gather(int64_t n, const int32_t *perm, const double *src, double *dst) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -86,25 +121,63 @@ __global__ void k_scatter(int64_t n, const int32_t *perm, const double *src, dou
 
 }  // namespace
 
-void degree_order(DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &perm, hipStream_t s) {
-  const int64_t n = A.n, m = A.nnz;
+void degree_perm(const DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &perm, DevBuf<int32_t> &iperm,
+                 hipStream_t s) {
+  const int64_t n = A.n;
   LG_CHECK(blocks.size() >= 2 && blocks.front() == 0 && blocks.back() == n, LIGMG_ERR_INVALID,
-           "degree_order: blocks must span [0, n]");
+           "degree_perm: blocks must span [0, n]");
   const int nb = (int)blocks.size() - 1;
-  DevBuf<int32_t> iperm(n), len(n + 1);
   perm.alloc(n);
-  {
-    DevBuf<int64_t> bptr(blocks.size());
-    to_device(bptr.p, blocks.data(), blocks.size(), s);
-    DevBuf<uint64_t> keys(n), vals(n);
-    hipLaunchKernelGGL(k_order_keys, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, A.rp.p, bptr.p, nb,
-                       keys.p, vals.p);
-    LG_LAUNCH_CHECK();
-    radix_sort_pairs(keys.p, vals.p, n, 32 + bits_for((uint64_t)nb), s);
-    hipLaunchKernelGGL(k_order_perm, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, vals.p, A.rp.p,
-                       perm.p, iperm.p, len.p);
+  iperm.alloc(n);
+  DevBuf<int32_t> len(n + 1);
+  DevBuf<int64_t> bptr(blocks.size());
+  to_device(bptr.p, blocks.data(), blocks.size(), s);
+  DevBuf<uint64_t> keys(n), vals(n);
+  hipLaunchKernelGGL(k_order_keys, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, A.rp.p, bptr.p, nb, keys.p,
+                     vals.p);
+  LG_LAUNCH_CHECK();
+  radix_sort_pairs(keys.p, vals.p, n, 32 + bits_for((uint64_t)nb), s);
+  hipLaunchKernelGGL(k_order_perm, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, vals.p, A.rp.p, perm.p,
+                     iperm.p, len.p);
+  LG_LAUNCH_CHECK();
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+void permute_rows(const DCsr &A, const int32_t *rperm, const int32_t *ciperm, int64_t ncols, DCsr &B,
+                  hipStream_t s) {
+  const int64_t n = A.n, m = A.nnz;
+  DevBuf<int32_t> len(n + 1);
+  hipLaunchKernelGGL(k_perm_len, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, A.rp.p, rperm, len.p);
+  LG_LAUNCH_CHECK();
+  fill_i32(len.p + n, 0, 1, s);
+  B.n = n;
+  B.ncols = ncols;
+  B.nnz = m;
+  B.rp.alloc(n + 1);
+  const int64_t total = exclusive_scan_i32(len.p, B.rp.p, n + 1, s);
+  LG_CHECK(total == m, LIGMG_ERR_INVALID, "permute_rows: row pointers inconsistent");
+  B.ci.alloc(std::max<int64_t>(m, 1));
+  if (A.v.p) B.v.alloc(std::max<int64_t>(m, 1));
+  if (m > 0) {
+    hipLaunchKernelGGL(k_perm_rows<double>, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p,
+                       A.ci.p, (const double *)A.v.p, rperm, ciperm, (const int32_t *)B.rp.p, B.ci.p, B.v.p);
     LG_LAUNCH_CHECK();
   }
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+void relabel_map(int64_t n, const int32_t *perm, const int32_t *map, const int32_t *niperm, int32_t *out,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_relabel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, perm, map, niperm, out);
+  LG_LAUNCH_CHECK();
+}
+
+void degree_order(DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &perm, hipStream_t s) {
+  const int64_t n = A.n, m = A.nnz;
+  DevBuf<int32_t> iperm, len(n + 1);
+  degree_perm(A, blocks, perm, iperm, s);
+  hipLaunchKernelGGL(k_perm_len, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, A.rp.p, perm.p, len.p);
+  LG_LAUNCH_CHECK();
   DCsr B;
   B.n = n;
   B.ncols = A.ncols;

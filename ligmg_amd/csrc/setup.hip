@@ -832,6 +832,72 @@ unsigned g1(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256);
 }  // namespace
 
 // ----------------------------------------------------------------------------
+// R = P^T (rows of R = columns of P, in ascending row order of P), with the
+// FP32 values too when P has them.
+// ----------------------------------------------------------------------------
+void transpose_p(const DCsr &P, int64_t ncols, DCsr &R, hipStream_t s) {
+  const int64_t n = P.n, nnzP = P.nnz;
+  DevBuf<uint64_t> keys(nnzP), vals(nnzP);
+  const int rb = std::max(1, bits_for((uint64_t)n));
+  const int cb = std::max(1, bits_for((uint64_t)ncols));
+  hipLaunchKernelGGL(k_p_transpose_keys, dim3(g1(n)), dim3(256), 0, s, (int)n, P.rp.p, P.ci.p, P.v.p, rb, keys.p,
+                     vals.p);
+  LG_LAUNCH_CHECK();
+  DevBuf<uint64_t> keys32, vals32;
+  if (P.v32.p) {
+    keys32.alloc(nnzP);
+    vals32.alloc(nnzP);
+    hipLaunchKernelGGL(k_p_transpose_keys32, dim3(g1(n)), dim3(256), 0, s, (int)n, P.rp.p, P.ci.p, P.v32.p, rb,
+                       keys32.p, vals32.p);
+    LG_LAUNCH_CHECK();
+    radix_sort_pairs(keys32.p, vals32.p, nnzP, rb + cb, s);
+  }
+  radix_sort_pairs(keys.p, vals.p, nnzP, rb + cb, s);
+  R.n = ncols;
+  R.ncols = n;
+  R.nnz = nnzP;
+  R.rp.alloc(ncols + 1);
+  R.ci.alloc(std::max<int64_t>(nnzP, 1));
+  R.v.alloc(std::max<int64_t>(nnzP, 1));
+  hipLaunchKernelGGL(k_sorted_rowptr, dim3(grid_for(nnzP + 1, 256, 8192)), dim3(256), 0, s, keys.p, nnzP, rb, ncols,
+                     R.rp.p);
+  LG_LAUNCH_CHECK();
+  if (nnzP) {
+    hipLaunchKernelGGL(k_unpack_pairs, dim3(grid_for(nnzP, 256, 8192)), dim3(256), 0, s, keys.p, vals.p, nnzP, rb,
+                       R.ci.p, R.v.p);
+    LG_LAUNCH_CHECK();
+  }
+  if (P.v32.p) {
+    R.v32.alloc(std::max<int64_t>(nnzP, 1));
+    if (nnzP) {
+      hipLaunchKernelGGL(k_unpack_vals32, dim3(grid_for(nnzP, 256, 8192)), dim3(256), 0, s, vals32.p, nnzP,
+                         R.v32.p);
+      LG_LAUNCH_CHECK();
+    }
+  }
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+// Member lists of the aggregates (nc x n, unit values): row c lists the
+// vertices i with agg[i] = c in ascending order.
+void build_members(const int32_t *agg, int64_t n, int64_t nc, DCsr &M, hipStream_t s) {
+  DevBuf<uint64_t> keys(n), vals(n);
+  hipLaunchKernelGGL(k_member_pairs, dim3(g1(n)), dim3(256), 0, s, (int)n, agg, keys.p, vals.p);
+  LG_LAUNCH_CHECK();
+  radix_sort_pairs(keys.p, vals.p, n, std::max(1, bits_for((uint64_t)nc)), s);
+  M.n = nc;
+  M.ncols = n;
+  M.nnz = n;
+  M.rp.alloc(nc + 1);
+  M.ci.alloc(std::max<int64_t>(n, 1));
+  hipLaunchKernelGGL(k_sorted_rowptr, dim3(grid_for(n + 1, 256, 8192)), dim3(256), 0, s, keys.p, n, 0, nc, M.rp.p);
+  LG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_members_unpack, dim3(g1(n)), dim3(256), 0, s, (int)n, vals.p, M.ci.p);
+  LG_LAUNCH_CHECK();
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------------------
 // Contraction: (key, value) list -> CSR of nrows rows.
 // ----------------------------------------------------------------------------
 void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, int64_t nrows,
@@ -923,27 +989,7 @@ bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &c
   L.nnzP = nnzP;
   L.nnzQ = n - n_kept;
   // R = P^T
-  {
-    DevBuf<uint64_t> keys(nnzP), vals(nnzP);
-    const int rb = std::max(1, bits_for((uint64_t)n));
-    const int cb = std::max(1, bits_for((uint64_t)n_kept));
-    hipLaunchKernelGGL(k_p_transpose_keys, dim3(g1(n)), dim3(256), 0, s, (int)n, L.P.rp.p, L.P.ci.p,
-                       L.P.v.p, rb, keys.p, vals.p);
-    LG_LAUNCH_CHECK();
-    radix_sort_pairs(keys.p, vals.p, nnzP, rb + cb, s);
-    L.R.n = n_kept;
-    L.R.ncols = n;
-    L.R.nnz = nnzP;
-    L.R.rp.alloc(n_kept + 1);
-    L.R.ci.alloc(nnzP);
-    L.R.v.alloc(nnzP);
-    hipLaunchKernelGGL(k_sorted_rowptr, dim3(grid_for(nnzP + 1, 256, 8192)), dim3(256), 0, s, keys.p, nnzP, rb,
-                       n_kept, L.R.rp.p);
-    LG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_unpack_pairs, dim3(grid_for(nnzP, 256, 8192)), dim3(256), 0, s, keys.p, vals.p,
-                       nnzP, rb, L.R.ci.p, L.R.v.p);
-    LG_LAUNCH_CHECK();
-  }
+  transpose_p(L.P, n_kept, L.R, s);
   // Schur complement by contraction
   DevBuf<int32_t> cnt(n + 1);
   fill_i32(cnt.p, 0, n + 1, s);
@@ -964,8 +1010,9 @@ void make_elim32(DLevel &L, hipStream_t s) {
   const int64_t n = A.n, nnzP = L.P.nnz, n_kept = L.nc;
   L.P.v32.alloc(std::max<int64_t>(nnzP, 1));
   L.q32.alloc(std::max<int64_t>(n, 1));
-  hipLaunchKernelGGL(k_p_fill32, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v32.p, A.d32.p,
-                     L.new_id.p, L.P.rp.p, L.P.v32.p, L.q32.p);
+  const int32_t *newid = L.new_id_s.p ? L.new_id_s.p : L.new_id.p;  // in the order of A's rows
+  hipLaunchKernelGGL(k_p_fill32, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v32.p, A.d32.p, newid,
+                     L.P.rp.p, L.P.v32.p, L.q32.p);
   LG_LAUNCH_CHECK();
   // R32 = P32^T with the same (stable) transposition that built R
   DevBuf<uint64_t> keys(nnzP), vals(nnzP);
@@ -976,9 +1023,11 @@ void make_elim32(DLevel &L, hipStream_t s) {
   LG_LAUNCH_CHECK();
   radix_sort_pairs(keys.p, vals.p, nnzP, rb + cb, s);
   L.R.v32.alloc(std::max<int64_t>(nnzP, 1));
-  hipLaunchKernelGGL(k_unpack_vals32, dim3(grid_for(nnzP, 256, 8192)), dim3(256), 0, s, vals.p, nnzP,
-                     L.R.v32.p);
-  LG_LAUNCH_CHECK();
+  if (nnzP) {
+    hipLaunchKernelGGL(k_unpack_vals32, dim3(grid_for(nnzP, 256, 8192)), dim3(256), 0, s, vals.p, nnzP,
+                       L.R.v32.p);
+    LG_LAUNCH_CHECK();
+  }
   LG_HIP(hipStreamSynchronize(s));
 }
 
@@ -1126,22 +1175,7 @@ void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLe
     contract_to_csr(keys, vals, A.nnz, nc, nc, coarse, s);
   }
   // member lists (restriction operator, unit values)
-  {
-    DevBuf<uint64_t> keys(n), vals(n);
-    hipLaunchKernelGGL(k_member_pairs, dim3(g1(n)), dim3(256), 0, s, (int)n, L.agg.p, keys.p, vals.p);
-    LG_LAUNCH_CHECK();
-    radix_sort_pairs(keys.p, vals.p, n, std::max(1, bits_for((uint64_t)nc)), s);
-    L.M.n = nc;
-    L.M.ncols = n;
-    L.M.nnz = n;
-    L.M.rp.alloc(nc + 1);
-    L.M.ci.alloc(n);
-    hipLaunchKernelGGL(k_sorted_rowptr, dim3(grid_for(n + 1, 256, 8192)), dim3(256), 0, s, keys.p, n, 0, nc,
-                       L.M.rp.p);
-    LG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_members_unpack, dim3(g1(n)), dim3(256), 0, s, (int)n, vals.p, L.M.ci.p);
-    LG_LAUNCH_CHECK();
-  }
+  build_members(L.agg.p, n, nc, L.M, s);
 }
 
 // Inverse diagonal of A (0 where the diagonal is missing or zero).
@@ -1160,9 +1194,11 @@ void setup_dinv(const DCsr &A, DLevel &L, bool fp32, hipStream_t s) {
 // (the CG eigen-estimator of the reference's Chebyshev, smoothing.hpp:158-165)
 // ----------------------------------------------------------------------------
 namespace {
-__global__ void k_eig_init(int n, uint64_t seed, uint64_t stream, double *r) {
+// start vector entry of setup position perm[i] (perm null: i), so a level
+// stored in solve layout starts from the same vector, permuted
+__global__ void k_eig_init(int n, uint64_t seed, uint64_t stream, const int32_t *perm, double *r) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) r[i] = ligmg_rng::u01(seed, stream, i);
+  if (i < n) r[i] = ligmg_rng::u01(seed, stream, perm ? perm[i] : i);
 }
 __global__ void k_axpy_dev(int n, const double *alpha, double sign, const double *x, double *y) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1207,14 +1243,14 @@ double tridiag_max_eig(const std::vector<double> &d, const std::vector<double> &
 }
 
 double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_idx, int iters,
-                    hipStream_t s) {
+                    const int32_t *perm, hipStream_t s) {
   const int64_t n = A.n;
   if (n == 0) return 1.0;
   RedPlan R;
   R.build(n, {0, n}, s);
   DevBuf<double> r(n), z(n), p(n), q(n), sc(4);
   hipLaunchKernelGGL(k_eig_init, dim3(g1(n)), dim3(256), 0, s, (int)n, seed,
-                     ligmg_rng::STREAM_EIG + (uint64_t)level_idx, r.p);
+                     ligmg_rng::STREAM_EIG + (uint64_t)level_idx, perm, r.p);
   LG_LAUNCH_CHECK();
   R.sums(r.p, sc.p, s);
   op_sub_block_mean(r.p, R, sc.p, s);

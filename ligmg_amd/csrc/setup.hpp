@@ -8,6 +8,11 @@ namespace lg {
 void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, int64_t nrows,
                      int64_t ncols, DCsr &out, hipStream_t s);
 
+// R = P^T (FP32 values too when P has them).
+void transpose_p(const DCsr &P, int64_t ncols, DCsr &R, hipStream_t s);
+// Aggregate member lists (nc x n, unit values, members ascending).
+void build_members(const int32_t *agg, int64_t n, int64_t nc, DCsr &M, hipStream_t s);
+
 // Elimination level from A; false if rejected (kept > elim_factor n).
 bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &coarse, hipStream_t s);
 
@@ -27,6 +32,17 @@ void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLe
 // Renumber A's vertices by descending row length inside each block (ties
 // by ascending index); perm receives internal position -> input vertex.
 void degree_order(DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &perm, hipStream_t s);
+// Degree permutation of A's rows inside blocks (perm: new -> old, iperm:
+// old -> new), the one degree_order applies.
+void degree_perm(const DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &perm, DevBuf<int32_t> &iperm,
+                 hipStream_t s);
+// B row i = A row rperm[i] (null: i) with its entries in their order and
+// columns relabeled by ciperm (null: unchanged); FP64 values only.
+void permute_rows(const DCsr &A, const int32_t *rperm, const int32_t *ciperm, int64_t ncols, DCsr &B,
+                  hipStream_t s);
+// out[i] = niperm[map[perm[i]]] (null: identity; negative map values kept)
+void relabel_map(int64_t n, const int32_t *perm, const int32_t *map, const int32_t *niperm, int32_t *out,
+                 hipStream_t s);
 // dst[i] = src[perm[i]]  /  dst[perm[i]] = src[i]
 void gather_vec(int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s);
 void scatter_vec(int64_t n, const int32_t *perm, const double *src, double *dst, hipStream_t s);
@@ -35,8 +51,9 @@ void setup_dinv(const DCsr &A, DLevel &L, bool fp32, hipStream_t s);
 // FP32 P, R = P^T and Q of an elimination level from the FP32 Laplacian form
 // of its matrix (L.A.v32 / L.A.d32 must exist).
 void make_elim32(DLevel &L, hipStream_t s);
+// perm (may be null): setup position of each row, for the start vector
 double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_idx, int iters,
-                    hipStream_t s);
+                    const int32_t *perm, hipStream_t s);
 std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,
                                     const std::vector<int32_t> &ci, const std::vector<double> &v);
 

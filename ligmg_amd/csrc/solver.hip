@@ -198,8 +198,9 @@ struct Solver {
   std::vector<std::vector<double>> al_pre, be_pre, al_post, be_post;
   KWork<double> outer;
   DevBuf<double> b0, sol, tmp0, dsc;
-  // internal position -> caller's vertex (empty under LIGMG_ORDER_NATURAL)
-  DevBuf<int32_t> perm;
+  // solve position of level 0 -> caller's vertex (empty: identity), and
+  // the setup order's permutation of the caller's vertices (ordering)
+  DevBuf<int32_t> perm, perm_input;
   DevBuf<double> rhs_base;  // reduced right-hand side of the outer solve
   double setup_time = 0;
   bool mixed = false;
@@ -252,7 +253,7 @@ struct Solver {
       d2d(cur.v.p, v, cur.nnz, s);
     }
     LG_HIP(hipStreamSynchronize(s));
-    if (o.ordering == LIGMG_ORDER_DEGREE) degree_order(cur, blocks0, perm, s);
+    if (o.ordering == LIGMG_ORDER_DEGREE) degree_order(cur, blocks0, perm_input, s);
     auto last = t0;
     auto lap = [&](const char *what, size_t l, int64_t nn, int64_t nz) {
       if (!o.verbose) return;
@@ -314,6 +315,18 @@ struct Solver {
       }
       H[l + 1].blocks = nb;
     }
+    if (o.solve_layout && o.spmv_slice == 0) apply_solve_layout();
+    // caller's vertex of each level-0 solve position
+    if (H[0].sperm.p && perm_input.p) {
+      perm.alloc(H[0].A.n);
+      relabel_map(H[0].A.n, H[0].sperm.p, perm_input.p, nullptr, perm.p, s);
+    } else if (H[0].sperm.p) {
+      perm.alloc(H[0].A.n);
+      d2d(perm.p, H[0].sperm.p, H[0].A.n, s);
+    } else if (perm_input.p) {
+      perm.alloc(H[0].A.n);
+      d2d(perm.p, perm_input.p, H[0].A.n, s);
+    }
     // smoothers, SpMV units, FP32 copies, work vectors
     red.resize(H.size());
     W64.resize(H.size());
@@ -330,7 +343,7 @@ struct Solver {
         build_units(L.M, s);
         setup_dinv(L.A, L, mixed, s);
         if (o.smoother_type == LIGMG_SMOOTHER_CHEBYSHEV) {
-          L.lam = eig_estimate(L.A, L.dinv.p, o.seed, (int)l, o.cheby_eig_iters, s);
+          L.lam = eig_estimate(L.A, L.dinv.p, o.seed, (int)l, o.cheby_eig_iters, L.sperm.p, s);
           L.cheb_lo = o.cheby_lower * L.lam;
           L.cheb_hi = o.cheby_upper * L.lam;
           al_pre[l] = cheb_alpha(L.cheb_lo, L.cheb_hi, o.pre_smooth, be_pre[l]);
@@ -368,6 +381,50 @@ struct Solver {
     LG_HIP(hipStreamSynchronize(s));
     lap("solve-prep", H.size(), 0, 0);
     setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  // Solve layout (see DLevel::sperm): levels of at least SOLVE_PERM_MIN
+  // vertices other than the coarsest get a degree permutation; the level
+  // matrix keeps each row's entries in their order (same row sums), the
+  // transfer operators are relabeled on both sides.
+  static constexpr int64_t SOLVE_PERM_MIN = 1 << 16;
+  void apply_solve_layout() {
+    const size_t nl = H.size();
+    for (size_t l = 0; l + 1 < nl; l++)
+      if (H[l].A.n >= SOLVE_PERM_MIN) degree_perm(H[l].A, H[l].blocks, H[l].sperm, H[l].siperm, s);
+    for (size_t l = 0; l + 1 < nl; l++) {
+      DLevel &V = H[l];
+      const DLevel &N = H[l + 1];
+      const int32_t *rp = V.sperm.p, *nip = N.siperm.p;
+      if (!rp && !nip) continue;
+      if (V.kind == LIGMG_LEVEL_AGG) {
+        DevBuf<int32_t> a(V.n);
+        relabel_map(V.n, rp, V.agg.p, nip, a.p, s);
+        V.agg_setup = std::move(V.agg);
+        V.agg = std::move(a);
+        build_members(V.agg.p, V.n, V.nc, V.M, s);
+      } else if (V.kind == LIGMG_LEVEL_ELIM) {
+        DCsr P;
+        permute_rows(V.P, rp, nip, V.nc, P, s);
+        V.P = std::move(P);
+        if (rp) {
+          DevBuf<double> q(V.n);
+          gather_vec(V.n, rp, V.q.p, q.p, s);
+          V.q = std::move(q);
+          V.new_id_s.alloc(V.n);
+          relabel_map(V.n, rp, V.new_id.p, nullptr, V.new_id_s.p, s);
+        }
+        transpose_p(V.P, V.nc, V.R, s);
+      }
+    }
+    for (size_t l = 0; l + 1 < nl; l++) {
+      DLevel &V = H[l];
+      if (!V.sperm.p) continue;
+      DCsr A;
+      permute_rows(V.A, V.sperm.p, V.siperm.p, V.A.ncols, A, s);
+      V.A = std::move(A);
+    }
+    LG_HIP(hipStreamSynchronize(s));
   }
 
   void tune_slices(DCsr &A, int prec, size_t l) {
@@ -1191,6 +1248,7 @@ void ligmg_options_default(ligmg_options *o) {
   o->use_graphs = 1;
   o->spmv_slice = 0;
   o->ordering = LIGMG_ORDER_NATURAL;
+  o->solve_layout = 1;
 }
 
 const char *ligmg_last_error(void) { return lg::g_last_error.c_str(); }
@@ -1265,8 +1323,8 @@ int ligmg_ordering(ligmg_handle h, int32_t *perm) {
   try {
     lg::Solver &S = h->S;
     const int64_t n = S.H[0].A.n;
-    if (S.perm.p) {
-      lg::to_host(perm, S.perm.p, n, S.s);
+    if (S.perm_input.p) {
+      lg::to_host(perm, S.perm_input.p, n, S.s);
       LG_HIP(hipStreamSynchronize(S.s));
     } else {
       for (int64_t i = 0; i < n; i++) perm[i] = (int32_t)i;
@@ -1305,10 +1363,31 @@ int ligmg_level_matrix(ligmg_handle h, int32_t l, int32_t *rp, int32_t *ci, doub
   try {
     const lg::DLevel &L = h->S.H[l];
     hipStream_t s = h->S.s;
-    lg::to_host(rp, L.A.rp.p, L.A.n + 1, s);
-    lg::to_host(ci, L.A.ci.p, L.A.nnz, s);
-    lg::to_host(v, L.A.v.p, L.A.nnz, s);
-    LG_HIP(hipStreamSynchronize(s));
+    if (!L.sperm.p) {
+      lg::to_host(rp, L.A.rp.p, L.A.n + 1, s);
+      lg::to_host(ci, L.A.ci.p, L.A.nnz, s);
+      lg::to_host(v, L.A.v.p, L.A.nnz, s);
+      LG_HIP(hipStreamSynchronize(s));
+      return LIGMG_OK;
+    }
+    // solve layout: rows back to setup order; each row kept its entry order,
+    // so mapping the columns back restores the sorted setup row exactly
+    const int64_t n = L.A.n;
+    auto prp = lg::download(L.A.rp.p, n + 1, s);
+    auto pci = lg::download(L.A.ci.p, L.A.nnz, s);
+    auto pv = lg::download(L.A.v.p, L.A.nnz, s);
+    auto sp = lg::download(L.sperm.p, n, s);
+    std::vector<int32_t> len(n);
+    for (int64_t i = 0; i < n; i++) len[sp[i]] = prp[i + 1] - prp[i];
+    rp[0] = 0;
+    for (int64_t i = 0; i < n; i++) rp[i + 1] = rp[i] + len[i];
+    for (int64_t i = 0; i < n; i++) {
+      int32_t d = rp[sp[i]];
+      for (int32_t k = prp[i]; k < prp[i + 1]; k++, d++) {
+        ci[d] = sp[pci[k]];
+        v[d] = pv[k];
+      }
+    }
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return fail(e);
@@ -1321,7 +1400,7 @@ int ligmg_level_map(ligmg_handle h, int32_t l, int32_t *map) {
     const lg::DLevel &L = h->S.H[l];
     hipStream_t s = h->S.s;
     if (L.kind == LIGMG_LEVEL_ELIM) lg::to_host(map, L.new_id.p, L.n, s);
-    else if (L.kind == LIGMG_LEVEL_AGG) lg::to_host(map, L.agg.p, L.n, s);
+    else if (L.kind == LIGMG_LEVEL_AGG) lg::to_host(map, L.agg_setup.p ? L.agg_setup.p : L.agg.p, L.n, s);
     else return LIGMG_ERR_INVALID;
     LG_HIP(hipStreamSynchronize(s));
     return LIGMG_OK;
@@ -1519,8 +1598,11 @@ int ligmg_apply_cycle(ligmg_handle h, int32_t l, int32_t prec, const double *r, 
     LG_CHECK(!f32 || S.mixed, LIGMG_ERR_INVALID,
              "ligmg_apply_cycle: FP32 cycles exist only with precision = LIGMG_PREC_MIXED");
     const int64_t n = S.H[l].A.n;
-    lg::DevBuf<double> dr(n), dz(n);
-    lg::to_device(dr.p, r, n, S.s);
+    const int32_t *sp = S.H[l].sperm.p;  // setup-order vectors in and out
+    lg::DevBuf<double> dr(n), dz(n), tmp(n);
+    lg::to_device(tmp.p, r, n, S.s);
+    if (sp) lg::gather_vec(n, sp, tmp.p, dr.p, S.s);
+    else lg::d2d(dr.p, tmp.p, n, S.s);
     if (f32) {
       lg::DevBuf<float> fr(n);
       lg::op_d2f(dr.p, fr.p, n, S.s);
@@ -1530,7 +1612,9 @@ int ligmg_apply_cycle(ligmg_handle h, int32_t l, int32_t prec, const double *r, 
       const double *out = S.cycle<double>((size_t)l, dr.p);
       lg::d2d(dz.p, out, n, S.s);
     }
-    lg::to_host(z, dz.p, n, S.s);
+    if (sp) lg::scatter_vec(n, sp, dz.p, tmp.p, S.s);
+    else lg::d2d(tmp.p, dz.p, n, S.s);
+    lg::to_host(z, tmp.p, n, S.s);
     LG_HIP(hipStreamSynchronize(S.s));
     return LIGMG_OK;
   } catch (const lg::Error &e) {
@@ -1546,21 +1630,24 @@ int ligmg_level_spmv(ligmg_handle h, int32_t l, int32_t prec, const double *x, d
     hipStream_t s = S.s;
     if (A.U.nunits == 0) lg::build_units(A, s);
     const int64_t n = A.n;
-    lg::DevBuf<double> dx(n), dy(n);
-    lg::to_device(dx.p, x, n, s);
+    const int32_t *sp = S.H[l].sperm.p;  // setup-order vectors in and out
+    lg::DevBuf<double> dx(n), dy(n), tmp(n);
+    lg::to_device(tmp.p, x, n, s);
+    if (sp) lg::gather_vec(n, sp, tmp.p, dx.p, s);
+    else lg::d2d(dx.p, tmp.p, n, s);
     if (prec == LIGMG_PREC_MIXED) {
       if (!A.v32.p) lg::make_v32(A, s, true);
       lg::DevBuf<float> fx(n), fy(n);
       lg::op_d2f(dx.p, fx.p, n, s);
       lg::op_spmv<float>(A, fx.p, fy.p, s);
       lg::op_f2d(fy.p, dy.p, n, s);
-      lg::to_host(y, dy.p, n, s);
-      LG_HIP(hipStreamSynchronize(s));
     } else {
       lg::op_spmv<double>(A, dx.p, dy.p, s);
-      lg::to_host(y, dy.p, n, s);
-      LG_HIP(hipStreamSynchronize(s));
     }
+    if (sp) lg::scatter_vec(n, sp, dy.p, tmp.p, s);
+    else lg::d2d(tmp.p, dy.p, n, s);
+    lg::to_host(y, tmp.p, n, s);
+    LG_HIP(hipStreamSynchronize(s));
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return fail(e);

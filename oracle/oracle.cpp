@@ -1113,6 +1113,7 @@ void oracle_default_options(ligmg_options *o) {
   o->use_graphs = 1;
   o->spmv_slice = 0;
   o->ordering = LIGMG_ORDER_NATURAL;
+  o->solve_layout = 1;
 }
 
 int oracle_create(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,

@@ -329,3 +329,43 @@ def test_cycle_matches_oracle(case, my_matrix):
     assert abs(zu @ w - u @ zw) <= 1e-10 * abs(zu @ w)
     z32 = d.apply_cycle(base, u, _ffi.PREC_MIXED)
     assert np.linalg.norm(z32 - zu) <= 1e-4 * np.linalg.norm(zu)
+
+
+@pytest.mark.parametrize("prec", [_ffi.PREC_FP64, _ffi.PREC_MIXED])
+def test_solve_layout_is_invisible(prec):
+    """Levels of 2^16 vertices and more are stored in degree order for the
+    solve: introspection still reports the setup order bit for bit, and
+    solutions, iteration counts, cycles and SpMVs agree with the setup
+    order's within rounding."""
+    g = graphs.rmat_config(18, "lognormal", pendants=True)
+    kw = dict(precision=prec, outer_ksp_type=_ffi.KSP_FCG)
+    on = ligmg_amd.Solver(g, ligmg_amd.Options(solve_layout=1, **kw))
+    off = ligmg_amd.Solver(g, ligmg_amd.Options(solve_layout=0, **kw))
+    assert on.num_levels() == off.num_levels()
+    assert on.level_info(1)[1] >= 1 << 16  # the base level is renumbered too
+    for l in range(on.num_levels()):
+        assert on.level_info(l) == off.level_info(l)
+        for a, b in zip(on.level_matrix(l), off.level_matrix(l)):
+            assert np.array_equal(a, b), l
+        if on.level_info(l)[0] != _ffi.LEVEL_COARSE:
+            assert np.array_equal(on.level_map(l), off.level_map(l)), l
+    rng = np.random.default_rng(2)
+    base = 1 if on.level_info(0)[0] == _ffi.LEVEL_ELIM else 0
+    n = on.level_info(base)[1]
+    u = rng.standard_normal(n)
+    u -= u.mean()
+    for p in ([_ffi.PREC_FP64, _ffi.PREC_MIXED] if prec == _ffi.PREC_MIXED else [_ffi.PREC_FP64]):
+        za, zb = on.apply_cycle(base, u, p), off.apply_cycle(base, u, p)
+        assert np.linalg.norm(za - zb) <= (1e-10 if p == _ffi.PREC_FP64 else 1e-5) * np.linalg.norm(zb)
+        ya, yb = on.level_spmv(base, u, p), off.level_spmv(base, u, p)
+        assert np.linalg.norm(ya - yb) <= (1e-13 if p == _ffi.PREC_FP64 else 1e-6) * np.linalg.norm(yb)
+    b = graphs.rhs(g.n, 1, 6)
+    ia, ib = on.solve(b), off.solve(b)
+    assert ia.rel_residual() <= 1e-8 and ib.rel_residual() <= 1e-8
+    assert abs(ia.iters() - ib.iters()) <= 1
+    x, y = ia.solution - ia.solution.mean(), ib.solution - ib.solution.mean()
+    assert np.linalg.norm(x - y) <= 1e-7 * np.linalg.norm(y)
+    B = graphs.rhs(g.n, 4, 7).reshape(g.n, 4, order="F")
+    Xa, _, _ = on.solve_batch(B)
+    Xb, _, _ = off.solve_batch(B)
+    assert np.linalg.norm((Xa - Xa.mean(0)) - (Xb - Xb.mean(0))) <= 1e-7 * np.linalg.norm(Xb - Xb.mean(0))
