@@ -172,19 +172,17 @@ void relabel_map(int64_t n, const int32_t *perm, const int32_t *map, const int32
   LG_LAUNCH_CHECK();
 }
 
-void degree_order(DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &perm, hipStream_t s) {
+void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, DCsr &B, hipStream_t s) {
   const int64_t n = A.n, m = A.nnz;
-  DevBuf<int32_t> iperm, len(n + 1);
-  degree_perm(A, blocks, perm, iperm, s);
-  hipLaunchKernelGGL(k_perm_len, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, A.rp.p, perm.p, len.p);
+  DevBuf<int32_t> len(n + 1);
+  hipLaunchKernelGGL(k_perm_len, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, A.rp.p, perm, len.p);
   LG_LAUNCH_CHECK();
-  DCsr B;
   B.n = n;
   B.ncols = A.ncols;
   B.rp.alloc(n + 1);
   fill_i32(len.p + n, 0, 1, s);
   const int64_t total = exclusive_scan_i32(len.p, B.rp.p, n + 1, s);
-  LG_CHECK(total == m, LIGMG_ERR_INVALID, "degree_order: row pointers inconsistent");
+  LG_CHECK(total == m, LIGMG_ERR_INVALID, "permute_sorted: row pointers inconsistent");
   B.nnz = m;
   B.ci.alloc(std::max<int64_t>(m, 1));
   B.v.alloc(std::max<int64_t>(m, 1));
@@ -192,7 +190,7 @@ void degree_order(DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &
     const int cb = bits_for((uint64_t)n);
     DevBuf<uint64_t> keys(m), vals(m);
     hipLaunchKernelGGL(k_order_entries, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p,
-                       A.ci.p, perm.p, iperm.p, B.rp.p, cb, keys.p, vals.p);
+                       A.ci.p, perm, iperm, B.rp.p, cb, keys.p, vals.p);
     LG_LAUNCH_CHECK();
     radix_sort_pairs(keys.p, vals.p, m, 2 * cb, s);
     hipLaunchKernelGGL(k_order_unpack, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, m, keys.p, vals.p,
@@ -200,6 +198,13 @@ void degree_order(DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &
     LG_LAUNCH_CHECK();
   }
   LG_HIP(hipStreamSynchronize(s));
+}
+
+void degree_order(DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &perm, hipStream_t s) {
+  DevBuf<int32_t> iperm;
+  degree_perm(A, blocks, perm, iperm, s);
+  DCsr B;
+  permute_sorted(A, perm.p, iperm.p, B, s);
   A = std::move(B);
 }
 

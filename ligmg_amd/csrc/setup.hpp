@@ -40,6 +40,9 @@ void degree_perm(const DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32
 // columns relabeled by ciperm (null: unchanged); FP64 values only.
 void permute_rows(const DCsr &A, const int32_t *rperm, const int32_t *ciperm, int64_t ncols, DCsr &B,
                   hipStream_t s);
+// B = P A P^T with P given by perm (new -> old) / iperm (old -> new) and
+// every row's columns sorted; FP64 values only.
+void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, DCsr &B, hipStream_t s);
 // out[i] = niperm[map[perm[i]]] (null: identity; negative map values kept)
 void relabel_map(int64_t n, const int32_t *perm, const int32_t *map, const int32_t *niperm, int32_t *out,
                  hipStream_t s);

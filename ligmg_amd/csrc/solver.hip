@@ -417,11 +417,15 @@ struct Solver {
         transpose_p(V.P, V.nc, V.R, s);
       }
     }
+    // aggregation levels (the smoothed matrices) get their columns sorted
+    // in the new order; an elimination level keeps its rows' entry order,
+    // which its P rows follow
     for (size_t l = 0; l + 1 < nl; l++) {
       DLevel &V = H[l];
       if (!V.sperm.p) continue;
       DCsr A;
-      permute_rows(V.A, V.sperm.p, V.siperm.p, V.A.ncols, A, s);
+      if (V.kind == LIGMG_LEVEL_AGG) permute_sorted(V.A, V.sperm.p, V.siperm.p, A, s);
+      else permute_rows(V.A, V.sperm.p, V.siperm.p, V.A.ncols, A, s);
       V.A = std::move(A);
     }
     LG_HIP(hipStreamSynchronize(s));
@@ -1370,8 +1374,7 @@ int ligmg_level_matrix(ligmg_handle h, int32_t l, int32_t *rp, int32_t *ci, doub
       LG_HIP(hipStreamSynchronize(s));
       return LIGMG_OK;
     }
-    // solve layout: rows back to setup order; each row kept its entry order,
-    // so mapping the columns back restores the sorted setup row exactly
+    // solve layout: rows and columns back to setup order, each row sorted
     const int64_t n = L.A.n;
     auto prp = lg::download(L.A.rp.p, n + 1, s);
     auto pci = lg::download(L.A.ci.p, L.A.nnz, s);
@@ -1381,11 +1384,18 @@ int ligmg_level_matrix(ligmg_handle h, int32_t l, int32_t *rp, int32_t *ci, doub
     for (int64_t i = 0; i < n; i++) len[sp[i]] = prp[i + 1] - prp[i];
     rp[0] = 0;
     for (int64_t i = 0; i < n; i++) rp[i + 1] = rp[i] + len[i];
+    std::vector<std::pair<int32_t, double>> row;
     for (int64_t i = 0; i < n; i++) {
+      row.clear();
+      for (int32_t k = prp[i]; k < prp[i + 1]; k++) row.push_back({sp[pci[k]], pv[k]});
+      std::sort(row.begin(), row.end(),
+                [](const std::pair<int32_t, double> &a, const std::pair<int32_t, double> &b) {
+                  return a.first < b.first;
+                });
       int32_t d = rp[sp[i]];
-      for (int32_t k = prp[i]; k < prp[i + 1]; k++, d++) {
-        ci[d] = sp[pci[k]];
-        v[d] = pv[k];
+      for (const auto &e : row) {
+        ci[d] = e.first;
+        v[d++] = e.second;
       }
     }
     return LIGMG_OK;
