@@ -51,13 +51,10 @@ def cpu_baseline(scale, nnz_full):
     g = graphs.rmat_config(scale, "lognormal", pendants=True)
     b = graphs.rhs(g.n, 1, 0)
     s = oracle_ffi.OracleSolver(g)
-    ts, iters = [], 0
-    for _ in range(2):
-        t = time.perf_counter()
-        _, info, _ = s.solve(b)
-        ts.append(time.perf_counter() - t)
-        iters = info.iters
-    t = min(ts)
+    t = time.perf_counter()
+    _, info, _ = s.solve(b)
+    t = time.perf_counter() - t
+    iters = info.iters
     return {
         "value": t * nnz_full / g.nnz,
         "unit": "s",
@@ -69,6 +66,127 @@ def cpu_baseline(scale, nnz_full):
     }
 
 
+def _line(metric, value, unit, hib, ms, args, dtype, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": hib, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(args):
+    """Per-right-hand-side time of batched solves: 48 independent
+    Barabasi-Albert graphs (m=5, n uniform in [20k, 400k], seeds 0..47) with
+    16 mean-zero N(0,1) right-hand sides each, and a 3-D 7-point grid 160^3
+    (weights uniform[0.1, 10]) with 64; FP32 preconditioner inside FP64
+    flexible CG, graph-captured cycles.  A step solves both batches."""
+    import numpy as np
+    import torch
+    import ligmg_amd
+    from ligmg_amd import _ffi, graphs
+    ns = np.random.default_rng(0).integers(20000, 400001, 48)
+    u = graphs.union([graphs.barabasi_albert(int(n), 5, seed=i) for i, n in enumerate(ns)])
+    g3 = graphs.grid3d(160, 160, 160, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=0)
+    opts = ligmg_amd.Options(precision=_ffi.PREC_MIXED, outer_ksp_type=_ffi.KSP_FCG)
+    dev = torch.device("cuda:0")
+    cases = []
+    for name, g, k, cp in (("ba48", u, 16, u.comp_ptr), ("grid160", g3, 64, None)):
+        B = torch.from_numpy(np.ascontiguousarray(
+            graphs.rhs(g.n, k, 0, comp_ptr=cp).reshape(g.n, k, order="F"))).to(dev)
+        t = time.perf_counter()
+        s = ligmg_amd.Solver(g, opts, comp_ptr=cp)
+        setup = time.perf_counter() - t
+        cases.append((name, g, k, s, B, setup))
+        print(f"# {name}: n={g.n} nnz={g.nnz} k={k} setup {setup:.2f}s", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        for name, g, k, s, B, _s in cases:
+            s.solve_batch(B)
+    torch.cuda.synchronize()
+    per = {name: [] for name, *_ in cases}
+    its = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for name, g, k, s, B, _s in cases:
+            X, info, it = s.solve_batch(B)
+            assert info.status == _ffi.CONVERGED and info.full_rel_residual <= 1e-7
+            per[name].append(info.time)
+            its[name] = (int(it.min()), int(it.max()))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nrhs = sum(k for _, _, k, *_ in cases)
+    ms_step = 1e3 * wall / args.steps
+    detail = {name: {"n": g.n, "nnz": g.nnz, "rhs": k, "setup_s": st, "batch_solve_s": min(per[name]),
+                     "per_rhs_ms": 1e3 * min(per[name]) / k, "iters_min_max": its[name]}
+              for name, g, k, s, B, st in cases}
+    for name, d in detail.items():
+        print(f"# {name}: batch {d['batch_solve_s']:.3f}s, {d['per_rhs_ms']:.2f} ms per right-hand side, "
+              f"iterations {d['iters_min_max']}", file=sys.stderr, flush=True)
+    _line("batched solve time per right-hand side to 1e-8 (FP32 preconditioner, FP64 FCG)",
+          ms_step / nrhs, "ms", False, ms_step, args, "f64 outer / f32 preconditioner",
+          {"workload": "ba48x16rhs_plus_grid160^3x64rhs", "rhs": nrhs}, batches=detail)
+
+
+def bench_grad(args):
+    """Forward + backward through x = L(w)^+ b on a random geometric graph:
+    1,000,000 vertices in the unit square, radius for mean degree 12, LCC,
+    w = exp(-d^2/r^2) in FP32, b = +1/-1 at 256 source/sink pairs, loss =
+    sum over 10,000 random vertex pairs of (x_i - x_j)^2.  A step is one
+    forward (assembly, setup, solve) and one backward (adjoint solve,
+    edge-gradient kernel)."""
+    import math
+    import numpy as np
+    import torch
+    from ligmg_amd import graphs
+    from ligmg_amd.autograd import EdgePattern, laplacian_solve
+    n0 = 1000000
+    r = math.sqrt(12 / (math.pi * n0))
+    g = graphs.rgg(n0, r, seed=0)
+    ei, ej, w = g.upper_edges()
+    rng = np.random.default_rng(0)
+    npairs = 256
+    perm = rng.permutation(g.n)
+    b = np.zeros(g.n)
+    b[perm[:npairs]] = 1.0
+    b[perm[npairs:2 * npairs]] = -1.0
+    pi, pj = rng.integers(0, g.n, 10000), rng.integers(0, g.n, 10000)
+    dev = torch.device("cuda:0")
+    pat = EdgePattern(g.n, ei, ej, dev)
+    tpi, tpj = torch.tensor(pi, device=dev), torch.tensor(pj, device=dev)
+    w32 = torch.tensor(w.astype(np.float32), device=dev)
+    bt = torch.tensor(b, device=dev)
+
+    def step():
+        wt = w32.clone().requires_grad_(True)
+        bb = bt.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x = laplacian_solve(wt, bb, pat)
+        loss = ((x[tpi] - x[tpj]) ** 2).sum()
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t
+        loss.backward()
+        loss = loss.detach()
+        torch.cuda.synchronize()
+        return tf, time.perf_counter() - t - tf, loss.item()
+
+    for _ in range(args.warmup):
+        step()
+    fw, bw = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a, c, loss = step()
+        fw.append(a)
+        bw.append(c)
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / args.steps
+    print(f"# rgg n={g.n} edges={len(w)}: forward {min(fw):.3f}s backward {min(bw):.3f}s loss {loss:.6g}",
+          file=sys.stderr, flush=True)
+    _line("forward+backward time of x = L(w)^+ b (gradients in w and b)", ms_step / 1e3, "s", False, ms_step,
+          args, "f64", {"workload": "rgg1M_deg12_gauss_weights_256pairs_10kpair_loss", "n": g.n,
+                        "edges": int(len(w))},
+          forward_s=min(fw), backward_s=min(bw))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,12 +194,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=23)
     ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--cpu-scale", type=int, default=18)
+    ap.add_argument("--cpu-scale", type=int, default=19)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ordering", choices=["degree", "natural"], default="natural",
                     help="vertex order of the hierarchy (default: the input's)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="setup + SpMV timing + copy calibration only (for counter runs)")
+    ap.add_argument("--workload", choices=["rmat", "batched", "grad"], default="rmat",
+                    help="rmat: the headline solve (default); batched: 48 BA graphs x 16 right-hand "
+                         "sides and a 160^3 grid x 64; grad: gradients through a 1M-vertex geometric graph")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -95,6 +216,10 @@ def main():
     from ligmg_amd import _ffi, graphs
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "grad":
+        return bench_grad(args)
     dev = torch.device("cuda:0")
     t = time.perf_counter()
     g = graphs.rmat_config(args.scale, "lognormal", pendants=True)
