@@ -245,6 +245,43 @@ int ligmg_apply_cycle(ligmg_handle h, int32_t level, int32_t precision,
 int ligmg_level_spmv(ligmg_handle h, int32_t level, int32_t precision,
                      const double *x, double *y);
 
+/* ---- graph preprocessing on the device (main.cpp:79-110, cc.hpp) ------ */
+/* All arrays live in mem_kind memory (LIGMG_MEM_HOST or LIGMG_MEM_DEVICE);
+ * the CSR has sorted columns; outputs are caller-allocated.               */
+
+/* connected_components (cc.hpp:22-51): label[i] = the largest vertex index
+ * of i's component (the fixed point of the reference's max-label SpMV).
+ * The pattern is taken as undirected (explicit zeros count as edges, as in
+ * the reference's SpMV).  *ncomp receives the number of components. */
+int ligmg_components(int64_t n, const int32_t *rowptr, const int32_t *colind,
+                     int mem_kind, int32_t *label, int64_t *ncomp);
+
+/* largest_cc (cc.hpp:55-105), first half: new_id[i] = rank of i among the
+ * vertices of the largest component (ascending original order), -1
+ * elsewhere; ties between equally large components go to the smallest
+ * label.  *n_out / *nnz_out size the submatrix for ligmg_extract. */
+int ligmg_largest_component(int64_t n, const int32_t *rowptr,
+                            const int32_t *colind, int mem_kind,
+                            int32_t *new_id, int64_t *n_out, int64_t *nnz_out);
+
+/* largest_cc second half (cc.hpp:90-104): the entries whose row and
+ * column both have new_id >= 0, renumbered (columns stay sorted). */
+int ligmg_extract(int64_t n, const int32_t *rowptr, const int32_t *colind,
+                  const double *vals, const int32_t *new_id, int64_t n_out,
+                  int mem_kind, int32_t *out_rowptr, int32_t *out_colind,
+                  double *out_vals);
+
+/* Adjacency matrix -> Laplacian L = D - A (main.cpp:79-110): D holds the
+ * row sums of A (stored entries, self loops included, summed in the
+ * canonical order); the pattern is A's plus the diagonal.  The first call
+ * returns the output nnz (nnz(A) + rows without a stored diagonal). */
+int ligmg_laplacian_nnz(int64_t n, const int32_t *rowptr,
+                        const int32_t *colind, int mem_kind, int64_t *nnz_out);
+int ligmg_adjacency_to_laplacian(int64_t n, const int32_t *rowptr,
+                                 const int32_t *colind, const double *vals,
+                                 int mem_kind, int32_t *out_rowptr,
+                                 int32_t *out_colind, double *out_vals);
+
 /* ---- stage entry points for parity tests (device execution) ---------- */
 /* Voting aggregation (aggregation.hpp:838-964) driven by caller-supplied
  * strength values on the pattern of a CSR (host arrays).  Writes the

@@ -35,12 +35,20 @@ typedef struct ligmg_graph {
   int64_t nnz;      /* stored entries of the Laplacian (incl. diagonal)   */
   int32_t *rowptr;  /* n+1                                                */
   int32_t *colind;  /* nnz, ascending within each row                     */
-  double *vals;     /* nnz, L = D - W                                     */
+  double *vals;     /* nnz, L = D - W (generators) or the file's values   */
   int32_t ncomp;    /* number of blocks (components) laid out contiguously */
   int64_t *comp_ptr;/* ncomp+1 vertex offsets of the blocks               */
 } ligmg_graph;
 
 void ligmg_graph_free(ligmg_graph *g);
+
+/* MatrixMarket coordinate file (real / integer / pattern, general /
+ * symmetric) into CSR as stored: the matrix itself, not a Laplacian of it
+ * (convert an adjacency matrix with ligmg_adjacency_to_laplacian).  The
+ * stored triangle of a symmetric file is mirrored, duplicates are summed
+ * in file order, columns are sorted; ncomp = 1.  Returns 3 when the file
+ * cannot be read, 4 when it is not a square coordinate matrix. */
+int ligmg_graph_read_mtx(const char *path, ligmg_graph *out);
 
 /* 2-D 5-point grid nx*ny, row-major ids (id = y*nx + x). */
 int ligmg_gen_grid2d(int64_t nx, int64_t ny, int wkind, double p0, double p1,

@@ -98,6 +98,14 @@ SOLVER_PROTOS = {
     "ligmg_apply_cycle": (C.c_int, [HANDLE, i32, i32, P(f64), P(f64)]),
     "ligmg_bench_spmv": (C.c_int, [HANDLE, i32, i32, i32, P(f64), P(i64)]),
     "ligmg_bench_smoother": (C.c_int, [HANDLE, i32, i32, i32, P(f64), P(i64)]),
+    "ligmg_components": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, P(i64)]),
+    "ligmg_largest_component": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, P(i64),
+                                          P(i64)]),
+    "ligmg_extract": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, i64, C.c_int,
+                                C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ligmg_laplacian_nnz": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_int, P(i64)]),
+    "ligmg_adjacency_to_laplacian": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                               C.c_void_p, C.c_void_p, C.c_void_p]),
     "ligmg_aggregate_with_strength": (C.c_int, [i64, P(i32), P(i32), P(f64),
                                                 P(ligmg_options), P(i32), P(i32),
                                                 P(i32), P(i64)]),
@@ -105,6 +113,7 @@ SOLVER_PROTOS = {
 
 GRAPH_PROTOS = {
     "ligmg_graph_free": (None, [P(ligmg_graph)]),
+    "ligmg_graph_read_mtx": (C.c_int, [C.c_char_p, P(ligmg_graph)]),
     "ligmg_gen_grid2d": (C.c_int, [i64, i64, C.c_int, f64, f64, u64, P(ligmg_graph)]),
     "ligmg_gen_grid3d": (C.c_int, [i64, i64, i64, C.c_int, f64, f64, u64, P(ligmg_graph)]),
     "ligmg_gen_rmat": (C.c_int, [C.c_int, C.c_int, f64, f64, f64, C.c_int, f64, f64,

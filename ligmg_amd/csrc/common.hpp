@@ -11,6 +11,9 @@
 namespace lg {
 
 // Error carrying a C-ABI status code (include/ligmg_amd.h).
+// message of the last failed C-ABI call on this thread
+extern thread_local std::string g_last_error;
+
 struct Error : std::runtime_error {
   int code;
   Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}

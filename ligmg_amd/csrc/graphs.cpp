@@ -12,6 +12,8 @@
 #include <sys/mman.h>
 
 #include <algorithm>
+#include <cctype>
+#include <climits>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -562,6 +564,182 @@ int ligmg_graph_from_edges(int64_t n, int64_t m, const int32_t *u, const int32_t
     }
   }
   return build_laplacian(n, E, out);
+}
+
+// MatrixMarket coordinate reader (the reference reads its input with
+// CombBLAS ParallelReadMM, src/main.cpp:33-40).  Fields real, integer or
+// pattern (value 1); symmetry general or symmetric (the stored triangle is
+// mirrored).  Indices are 1-based in the file.  Duplicates are summed in
+// file order; every row's columns come out sorted.  Square only.
+int ligmg_graph_read_mtx(const char *path, ligmg_graph *out) {
+  if (!path || !out) return 1;
+  memset(out, 0, sizeof(*out));
+  FILE *f = fopen(path, "rb");
+  if (!f) return 3;
+  fseek(f, 0, SEEK_END);
+  const long sz = ftell(f);
+  fseek(f, 0, SEEK_SET);
+  if (sz <= 0) {
+    fclose(f);
+    return 4;
+  }
+  std::vector<char> buf((size_t)sz + 1);
+  const size_t got = fread(buf.data(), 1, (size_t)sz, f);
+  fclose(f);
+  if (got != (size_t)sz) return 3;
+  buf[sz] = 0;
+  char *p = buf.data(), *end = buf.data() + sz;
+  auto next_line = [&](char *q) {
+    while (q < end && *q != '\n') q++;
+    return q < end ? q + 1 : end;
+  };
+  // header: %%MatrixMarket matrix coordinate <field> <symmetry>
+  char obj[64] = {0}, fmt[64] = {0}, field[64] = {0}, sym[64] = {0};
+  if (sscanf(p, "%%%%MatrixMarket %63s %63s %63s %63s", obj, fmt, field, sym) != 4) return 4;
+  for (char *c : {obj, fmt, field, sym})
+    for (; *c; c++) *c = (char)tolower(*c);
+  if (strcmp(obj, "matrix") || strcmp(fmt, "coordinate")) return 4;
+  const bool pattern = !strcmp(field, "pattern");
+  if (!pattern && strcmp(field, "real") && strcmp(field, "integer")) return 4;
+  const bool symmetric = !strcmp(sym, "symmetric");
+  if (!symmetric && strcmp(sym, "general")) return 4;
+  p = next_line(p);
+  while (p < end && (*p == '%' || *p == '\n' || *p == '\r')) p = next_line(p);
+  long long nr = 0, nc = 0, ne = 0;
+  if (sscanf(p, "%lld %lld %lld", &nr, &nc, &ne) != 3 || nr != nc || nr <= 0 || nr >= INT32_MAX || ne < 0)
+    return 4;
+  p = next_line(p);
+  // entries: one chunk per thread, cut at line boundaries
+  const int T = std::max(1, omp_get_max_threads());
+  std::vector<char *> cut(T + 1);
+  cut[0] = p;
+  cut[T] = end;
+  for (int t = 1; t < T; t++) {
+    char *q = p + (end - p) * t / T;
+    cut[t] = std::max(cut[t - 1], (q > p && q[-1] == '\n') ? q : next_line(q));
+  }
+  std::vector<std::vector<int32_t>> I(T), J(T);
+  std::vector<std::vector<double>> V(T);
+  std::vector<int> bad(T, 0);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    char *q = cut[t];
+    while (q < cut[t + 1]) {
+      char *e2;
+      const long long i = strtoll(q, &e2, 10);
+      if (e2 == q) {  // blank line
+        q = next_line(q);
+        continue;
+      }
+      q = e2;
+      const long long j = strtoll(q, &e2, 10);
+      if (e2 == q) {
+        bad[t] = 1;
+        break;
+      }
+      q = e2;
+      double v = 1.0;
+      if (!pattern) {
+        v = strtod(q, &e2);
+        if (e2 == q) {
+          bad[t] = 1;
+          break;
+        }
+        q = e2;
+      }
+      if (i < 1 || j < 1 || i > nr || j > nr) {
+        bad[t] = 1;
+        break;
+      }
+      I[t].push_back((int32_t)(i - 1));
+      J[t].push_back((int32_t)(j - 1));
+      V[t].push_back(v);
+      q = next_line(q);
+    }
+  }
+  for (int t = 0; t < T; t++)
+    if (bad[t]) return 4;
+  const int64_t n = nr;
+  std::vector<int64_t> base(T + 1, 0);
+  for (int t = 0; t < T; t++) base[t + 1] = base[t] + (int64_t)I[t].size();
+  if (base[T] != ne) return 4;
+  // COO (file order, mirrored entries right after their originals) -> CSR
+  std::vector<int64_t> cnt(n + 1, 0);
+  for (int t = 0; t < T; t++)
+    for (size_t k = 0; k < I[t].size(); k++) {
+      cnt[I[t][k] + 1]++;
+      if (symmetric && I[t][k] != J[t][k]) cnt[J[t][k] + 1]++;
+    }
+  for (int64_t r = 0; r < n; r++) cnt[r + 1] += cnt[r];
+  const int64_t m = cnt[n];
+  if (m >= INT32_MAX) return 4;
+  std::vector<int32_t> col(m);
+  std::vector<double> val(m);
+  std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int t = 0; t < T; t++)
+    for (size_t k = 0; k < I[t].size(); k++) {
+      const int32_t i = I[t][k], j = J[t][k];
+      col[pos[i]] = j;
+      val[pos[i]++] = V[t][k];
+      if (symmetric && i != j) {
+        col[pos[j]] = i;
+        val[pos[j]++] = V[t][k];
+      }
+    }
+  // sort each row by column (stable: duplicates keep file order) and sum them
+  std::vector<int64_t> rlen(n, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t r = 0; r < n; r++) {
+    const int64_t a = cnt[r], b = cnt[r + 1];
+    std::vector<std::pair<int32_t, int64_t>> e;
+    e.reserve(b - a);
+    for (int64_t k = a; k < b; k++) e.push_back({col[k], k});
+    std::stable_sort(e.begin(), e.end(),
+                     [](const std::pair<int32_t, int64_t> &x, const std::pair<int32_t, int64_t> &y) {
+                       return x.first < y.first;
+                     });
+    std::vector<int32_t> c2;
+    std::vector<double> v2;
+    for (auto &x : e) {
+      if (!c2.empty() && c2.back() == x.first) v2.back() += val[x.second];
+      else {
+        c2.push_back(x.first);
+        v2.push_back(val[x.second]);
+      }
+    }
+    for (size_t k = 0; k < c2.size(); k++) {
+      col[a + k] = c2[k];
+      val[a + k] = v2[k];
+    }
+    rlen[r] = (int64_t)c2.size();
+  }
+  int64_t nnz = 0;
+  for (int64_t r = 0; r < n; r++) nnz += rlen[r];
+  out->n = n;
+  out->nnz = nnz;
+  out->rowptr = (int32_t *)huge_alloc(sizeof(int32_t) * (n + 1));
+  out->colind = (int32_t *)huge_alloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1));
+  out->vals = (double *)huge_alloc(sizeof(double) * std::max<int64_t>(nnz, 1));
+  out->ncomp = 1;
+  out->comp_ptr = (int64_t *)malloc(sizeof(int64_t) * 2);
+  if (!out->rowptr || !out->colind || !out->vals || !out->comp_ptr) {
+    ligmg_graph_free(out);
+    return 2;
+  }
+  out->comp_ptr[0] = 0;
+  out->comp_ptr[1] = n;
+  out->rowptr[0] = 0;
+  for (int64_t r = 0; r < n; r++) out->rowptr[r + 1] = out->rowptr[r] + (int32_t)rlen[r];
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; r++) {
+    const int64_t a = cnt[r], d = out->rowptr[r];
+    for (int64_t k = 0; k < rlen[r]; k++) {
+      out->colind[d + k] = col[a + k];
+      out->vals[d + k] = val[a + k];
+    }
+  }
+  return 0;
 }
 
 int ligmg_gen_rhs(int64_t n, int k, uint64_t seed, int ncomp, const int64_t *comp_ptr,

@@ -659,6 +659,15 @@ struct OpScale : NoAcc {
 };
 
 // ---- a10: col_sum = Reduce(Column) in ascending row order
+// row sums over every stored entry (the weighted degree of an adjacency
+// matrix, main.cpp:82-84), in the canonical order
+struct OpRowSum : DblSum {
+  const double *v;
+  double *out;
+  __device__ void add(double &acc, int, int k) const { acc += v[k]; }
+  __device__ void finish(int row, double acc) const { out[row] = acc; }
+};
+
 struct OpColSum : DblSum {
   const int32_t *tpos;
   const double *S;
@@ -830,6 +839,13 @@ __global__ void k_members_unpack(int n, const uint64_t *vals, int32_t *mem) {
 unsigned g1(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256); }
 
 }  // namespace
+
+void row_sums_canonical(int64_t n, const int32_t *rp, const double *v, double *out, hipStream_t s) {
+  HugeRows Hr;
+  find_huge(n, rp, Hr, s);
+  run_rows(n, rp, OpRowSum{{}, v, out}, Hr, s);
+  LG_HIP(hipStreamSynchronize(s));
+}
 
 // ----------------------------------------------------------------------------
 // R = P^T (rows of R = columns of P, in ascending row order of P), with the

@@ -24,9 +24,9 @@
 
 namespace lg {
 
-namespace {
-
 thread_local std::string g_last_error;
+
+namespace {
 
 // ------------------------------------------------------------ Krylov kernels
 enum { SC_BN2 = 0, SC_RN2, SC_GAMMA, SC_DELTA, SC_T1, SC_ALPHA, SC_BETA, SC_ACTIVE, SC_GN, SC_N };
@@ -253,6 +253,7 @@ struct Solver {
       d2d(cur.v.p, v, cur.nnz, s);
     }
     LG_HIP(hipStreamSynchronize(s));
+    validate_csr(n, n, cur.rp.p, cur.ci.p, s, true);
     if (o.ordering == LIGMG_ORDER_DEGREE) degree_order(cur, blocks0, perm_input, s);
     auto last = t0;
     auto lap = [&](const char *what, size_t l, int64_t nn, int64_t nz) {

@@ -78,6 +78,17 @@ def _call(fn, *args):
     return _take(g)
 
 
+def read_mtx(path):
+    """MatrixMarket coordinate file -> Graph holding the matrix as stored
+    (symmetric files mirrored, duplicates summed, rows sorted).  An
+    adjacency matrix becomes a Laplacian with prep.adjacency_to_laplacian."""
+    g = _ffi.ligmg_graph()
+    rc = _ffi.graphs_lib().ligmg_graph_read_mtx(str(path).encode(), C.byref(g))
+    if rc != 0:
+        raise ValueError(f"cannot read MatrixMarket file {path} (code {rc})")
+    return _take(g)
+
+
 def grid2d(nx, ny, wkind=W_UNIT, p0=0.0, p1=0.0, seed=0):
     return _call(_ffi.graphs_lib().ligmg_gen_grid2d, nx, ny, wkind, p0, p1, seed)
 

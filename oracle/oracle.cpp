@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -1323,6 +1324,111 @@ int oracle_eig_estimate(int64_t n, const int32_t *rp, const int32_t *ci, const d
     dinv[i] = (d != 0 && !isolated(A, i)) ? 1.0 / d : 0.0;
   }
   *lam = eig_estimate(A, dinv, seed, level, iters);
+  return 0;
+}
+
+// ---- graph preprocessing (cc.hpp:22-105, main.cpp:79-110) ---------------
+// connected_components: cc = iota; repeat cc_i = max(cc_i, max over stored
+// j of row i of cc_j) (the MaxSRing SpMV and the max with the previous
+// round, cc.hpp:42-48) until nothing changes.
+int oracle_components(int64_t n, const int32_t *rp, const int32_t *ci, int32_t *label, int64_t *ncomp) {
+  std::vector<int32_t> cc(n), nx(n);
+  std::iota(cc.begin(), cc.end(), 0);
+  while (true) {
+    for (int64_t i = 0; i < n; i++) {
+      int32_t m = std::numeric_limits<int32_t>::min();
+      for (int32_t k = rp[i]; k < rp[i + 1]; k++) m = std::max(m, cc[ci[k]]);
+      nx[i] = std::max(m, cc[i]);
+    }
+    if (nx == cc) break;
+    cc.swap(nx);
+  }
+  std::vector<char> seen(n, 0);
+  int64_t k = 0;
+  for (int64_t i = 0; i < n; i++) {
+    label[i] = cc[i];
+    if (!seen[cc[i]]) {
+      seen[cc[i]] = 1;
+      k++;
+    }
+  }
+  *ncomp = k;
+  return 0;
+}
+
+// largest_cc: counts per label, the first maximum in label order, vertices
+// of that component renumbered in ascending order (cc.hpp:60-84); the kept
+// entries are those with both ends in it (cc.hpp:90-99).
+int oracle_largest_component(int64_t n, const int32_t *rp, const int32_t *ci, int32_t *new_id, int64_t *n_out,
+                             int64_t *nnz_out) {
+  std::vector<int32_t> cc(n);
+  int64_t nc;
+  oracle_components(n, rp, ci, cc.data(), &nc);
+  std::vector<uint64_t> counts(n, 0);
+  for (int64_t i = 0; i < n; i++) counts[cc[i]]++;
+  const int64_t id = std::max_element(counts.begin(), counts.end()) - counts.begin();
+  int32_t next = 0;
+  for (int64_t i = 0; i < n; i++) new_id[i] = (cc[i] == id) ? next++ : -1;
+  int64_t m = 0;
+  for (int64_t i = 0; i < n; i++)
+    if (new_id[i] >= 0)
+      for (int32_t k = rp[i]; k < rp[i + 1]; k++) m += new_id[ci[k]] >= 0;
+  *n_out = next;
+  *nnz_out = m;
+  return 0;
+}
+
+int oracle_extract(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const int32_t *new_id,
+                   int64_t n_out, int32_t *orp, int32_t *oci, double *ov) {
+  std::vector<int64_t> row_of(n_out, -1);
+  for (int64_t i = 0; i < n; i++)
+    if (new_id[i] >= 0) row_of[new_id[i]] = i;
+  orp[0] = 0;
+  int64_t o = 0;
+  for (int64_t r = 0; r < n_out; r++) {
+    const int64_t i = row_of[r];
+    for (int32_t k = rp[i]; k < rp[i + 1]; k++)
+      if (new_id[ci[k]] >= 0) {
+        oci[o] = new_id[ci[k]];
+        if (ov) ov[o] = v[k];
+        o++;
+      }
+    orp[r + 1] = (int32_t)o;
+  }
+  return 0;
+}
+
+// L = D - A: D the row sums of A (every stored entry, canonical order), the
+// union of A's pattern and the diagonal; EWiseApply gives d - a on the
+// diagonal and 0 - a elsewhere (main.cpp:82-104).
+int oracle_adjacency_to_laplacian(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
+                                  int32_t *orp, int32_t *oci, double *ov) {
+  orp[0] = 0;
+  int64_t o = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const double d = canon_sum(v + rp[i], rp[i + 1] - rp[i]);
+    bool done = false;
+    for (int32_t k = rp[i]; k < rp[i + 1]; k++) {
+      if (!done && ci[k] > i) {
+        oci[o] = (int32_t)i;
+        ov[o++] = d;
+        done = true;
+      }
+      if (ci[k] == i) {
+        oci[o] = ci[k];
+        ov[o++] = d - v[k];
+        done = true;
+      } else {
+        oci[o] = ci[k];
+        ov[o++] = 0.0 - v[k];
+      }
+    }
+    if (!done) {
+      oci[o] = (int32_t)i;
+      ov[o++] = d;
+    }
+    orp[i + 1] = (int32_t)o;
+  }
   return 0;
 }
 

@@ -85,6 +85,17 @@ int oracle_cheby_apply(int64_t n, const int32_t *rp, const int32_t *ci,
 int oracle_eig_estimate(int64_t n, const int32_t *rp, const int32_t *ci,
                         const double *v, uint64_t seed, int32_t level,
                         int32_t iters, double *lam);
+int oracle_components(int64_t n, const int32_t *rp, const int32_t *ci,
+                      int32_t *label, int64_t *ncomp);
+int oracle_largest_component(int64_t n, const int32_t *rp, const int32_t *ci,
+                             int32_t *new_id, int64_t *n_out,
+                             int64_t *nnz_out);
+int oracle_extract(int64_t n, const int32_t *rp, const int32_t *ci,
+                   const double *v, const int32_t *new_id, int64_t n_out,
+                   int32_t *orp, int32_t *oci, double *ov);
+int oracle_adjacency_to_laplacian(int64_t n, const int32_t *rp,
+                                  const int32_t *ci, const double *v,
+                                  int32_t *orp, int32_t *oci, double *ov);
 int oracle_pinv(int64_t n, const int32_t *rp, const int32_t *ci,
                 const double *v, double *dense);
 

@@ -21,3 +21,9 @@ def my_matrix():
     from ligmg_amd import graphs
     A = scipy.io.mmread(os.path.join(ROOT, "tests", "golden", "my_matrix.mtx")).tocsr()
     return graphs.from_scipy(A)
+
+
+@pytest.fixture(scope="session")
+def golden_dir():
+    import pathlib
+    return pathlib.Path(ROOT) / "tests" / "golden"

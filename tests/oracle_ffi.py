@@ -20,6 +20,12 @@ PROTOS = {
     "oracle_create": (C.c_int, [i64, P(i32), P(i32), P(f64), P(_ffi.ligmg_options), i32, P(i64),
                                 P(H)]),
     "oracle_destroy": (C.c_int, [H]),
+    "oracle_components": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, P(i64)]),
+    "oracle_largest_component": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, P(i64), P(i64)]),
+    "oracle_extract": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, i64, C.c_void_p,
+                                 C.c_void_p, C.c_void_p]),
+    "oracle_adjacency_to_laplacian": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.c_void_p]),
     "oracle_ordering": (C.c_int, [H, C.c_void_p]),
     "oracle_solve": (C.c_int, [H, P(f64), P(f64), P(_ffi.ligmg_info), P(f64), i32]),
     "oracle_num_levels": (C.c_int, [H, P(i32)]),
@@ -171,3 +177,43 @@ class OracleSolver:
         if rc != 0:
             raise RuntimeError(f"oracle_solve failed ({rc})")
         return x, info, hist[:info.history_len].copy()
+
+
+# ---- graph preprocessing (cc.hpp, main.cpp:79-110) -------------------------
+def components(rp, ci):
+    rp, ci = np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32)
+    n = len(rp) - 1
+    lab = np.empty(n, np.int32)
+    k = i64()
+    assert lib().oracle_components(n, rp.ctypes.data, ci.ctypes.data, lab.ctypes.data, C.byref(k)) == 0
+    return lab, k.value
+
+
+def largest_component(rp, ci, v):
+    rp, ci = np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    n = len(rp) - 1
+    new_id = np.empty(n, np.int32)
+    no, mo = i64(), i64()
+    assert lib().oracle_largest_component(n, rp.ctypes.data, ci.ctypes.data, new_id.ctypes.data,
+                                          C.byref(no), C.byref(mo)) == 0
+    orp = np.empty(no.value + 1, np.int32)
+    oci = np.empty(max(mo.value, 1), np.int32)
+    ov = np.empty(max(mo.value, 1), np.float64)
+    assert lib().oracle_extract(n, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, new_id.ctypes.data, no.value,
+                                orp.ctypes.data, oci.ctypes.data, ov.ctypes.data) == 0
+    return orp, oci[:mo.value], ov[:mo.value], new_id
+
+
+def adjacency_to_laplacian(rp, ci, v):
+    rp, ci = np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    n = len(rp) - 1
+    has = np.array([np.any(ci[rp[i]:rp[i + 1]] == i) for i in range(n)])
+    m = len(ci) + int((~has).sum())
+    orp = np.empty(n + 1, np.int32)
+    oci = np.empty(max(m, 1), np.int32)
+    ov = np.empty(max(m, 1), np.float64)
+    assert lib().oracle_adjacency_to_laplacian(n, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, orp.ctypes.data,
+                                               oci.ctypes.data, ov.ctypes.data) == 0
+    return orp, oci[:m], ov[:m]
