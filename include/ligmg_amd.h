@@ -122,6 +122,11 @@ typedef struct ligmg_options {
                                    level is stored in degree order (gather
                                    locality; the hierarchy is unchanged);
                                    0: setup order                           */
+  int32_t tail_nnz;             /* 4096: the coarse levels whose operators
+                                   hold at most this many stored entries
+                                   (with every coarser level) run as one
+                                   single-workgroup launch per coarse-grid
+                                   correction, same results; 0: off         */
 } ligmg_options;
 
 /* ConvergenceInfo (util.hpp:143-170) plus the fields the reference only
@@ -197,6 +202,10 @@ const char *ligmg_last_error(void);
 /* ---- hierarchy introspection (print_hierarchy, multigrid.hpp:265-292;
  *      dump_hierarchy, ligmg.hpp:47-52) ---------------------------------- */
 int ligmg_num_levels(ligmg_handle h, int32_t *out);
+/* First level of the fused coarse tail (new; see tail_nnz): levels from
+ * *out down run in one launch per coarse-grid correction; *out equals the
+ * number of levels when nothing is fused. */
+int ligmg_tail_start(ligmg_handle h, int32_t *out);
 int ligmg_setup_time(ligmg_handle h, double *out);
 int ligmg_level_info(ligmg_handle h, int32_t level, int32_t *kind, int64_t *n,
                      int64_t *nnz, int64_t *n_coarse);

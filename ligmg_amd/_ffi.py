@@ -32,6 +32,7 @@ class ligmg_options(C.Structure):
         ("spmv_slice", i32),
         ("ordering", i32),
         ("solve_layout", i32),
+        ("tail_nnz", i32),
     ]
 
 
@@ -87,6 +88,7 @@ SOLVER_PROTOS = {
     "ligmg_ordering": (C.c_int, [HANDLE, C.c_void_p]),
     "ligmg_last_error": (C.c_char_p, []),
     "ligmg_num_levels": (C.c_int, [HANDLE, P(i32)]),
+    "ligmg_tail_start": (C.c_int, [HANDLE, P(i32)]),
     "ligmg_setup_time": (C.c_int, [HANDLE, P(f64)]),
     "ligmg_level_info": (C.c_int, [HANDLE, i32, P(i32), P(i64), P(i64), P(i64)]),
     "ligmg_level_matrix": (C.c_int, [HANDLE, i32, P(i32), P(i32), P(f64)]),

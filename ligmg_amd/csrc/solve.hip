@@ -5,62 +5,13 @@
 
 #include "ops.hpp"
 #include "prims.hpp"
+#include "solve_dev.hpp"
 
 namespace lg {
 
 namespace {
 
-// ------------------------------------------------------------ value access
-// Matrix data is streamed once per product: non-temporal loads keep it from
-// displacing the gathered vector in L2.
-template <class T>
-struct ValOne {
-  __device__ __forceinline__ T operator()(int64_t) const { return (T)1; }
-};
-template <class T>
-struct ValArr {
-  const T *v;
-  __device__ __forceinline__ T operator()(int64_t k) const { return __builtin_nontemporal_load(v + k); }
-};
-
-// --------------------------------------------------------------- epilogues
-// Row sums are always accumulated in FP64 (a product of two FP32 numbers
-// is exact in FP64), so the FP32 path only rounds storage, not the
-// heavily cancelling Laplacian row sums of hub vertices.
-template <class T>
-struct EpiStore {
-  T *y;
-  __device__ __forceinline__ void operator()(int64_t r, double s) const { y[r] = (T)s; }
-};
-template <class T>
-struct EpiResid {
-  const T *b;
-  T *r;
-  __device__ __forceinline__ void operator()(int64_t i, double s) const {
-    r[i] = (T)((double)b[i] - s);
-  }
-};
-template <class T>
-struct EpiCheb {
-  const T *dinv, *b, *xc, *xp;
-  T *xn;
-  double alpha, beta;
-  __device__ __forceinline__ void operator()(int64_t i, double s) const {
-    double c = (double)xc[i];
-    // a null xp is a zero previous iterate (second step from a zero guess);
-    // every other caller passes alpha = 0 with it
-    double d = alpha * (c - (xp ? (double)xp[i] : 0.0));
-    xn[i] = (T)(c + d + beta * (double)dinv[i] * ((double)b[i] - s));
-  }
-};
-template <class T>
-struct EpiInterp {
-  const T *q, *b;
-  T *x;
-  __device__ __forceinline__ void operator()(int64_t i, double s) const {
-    x[i] = (T)(s + (double)q[i] * (double)b[i]);
-  }
-};
+using namespace sdev;
 
 // slice passes: first writes the FP64 accumulator, middle ones add to it,
 // the last one hands accumulator + its own sum to the real epilogue
@@ -78,24 +29,6 @@ struct EpiAccFinal {
   Epi e;
   __device__ __forceinline__ void operator()(int64_t i, double s) const { e(i, acc[i] + s); }
 };
-
-// adds d[i] x[i] (FP64) before the wrapped epilogue: the diagonal term of an
-// FP32 operator stored in Laplacian form
-template <class T, class Epi>
-struct EpiDiag {
-  const double *d;
-  const T *x;
-  Epi e;
-  __device__ __forceinline__ void operator()(int64_t i, double s) const {
-    e(i, s + d[i] * (double)x[i]);
-  }
-};
-
-template <class T>
-__device__ __forceinline__ T wave_sum(T v) {
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
-  return v;
-}
 
 template <class T>
 __device__ __forceinline__ T block_sum(T v) {
@@ -262,7 +195,7 @@ template <class T>
 __global__ void k_cheb_zero(const T *dinv, const T *b, T *xn, double beta, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    xn[i] = (T)(beta * (double)dinv[i] * (double)b[i]);
+    xn[i] = cheb_first<T>(beta, dinv[i], b[i]);
 }
 
 template <class T>

@@ -21,6 +21,8 @@
 #include "ops.hpp"
 #include "prims.hpp"
 #include "setup.hpp"
+#include "solve_dev.hpp"
+#include "tail.hpp"
 
 namespace lg {
 
@@ -29,38 +31,24 @@ thread_local std::string g_last_error;
 namespace {
 
 // ------------------------------------------------------------ Krylov kernels
-enum { SC_BN2 = 0, SC_RN2, SC_GAMMA, SC_DELTA, SC_T1, SC_ALPHA, SC_BETA, SC_ACTIVE, SC_GN, SC_N };
+using namespace sdev;
 
 // per-block scalar logic of CG / FCG (one thread per block)
 __global__ void k_kr_alpha(int nb, int fcg, double *sc) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nb) return;
-  double *S = sc + c * SC_N;
-  double num = fcg ? S[SC_T1] : S[SC_GAMMA];
-  S[SC_ALPHA] = (S[SC_ACTIVE] != 0 && S[SC_DELTA] != 0) ? num / S[SC_DELTA] : 0.0;
+  if (c < nb) kr_alpha(sc + c * SC_N, fcg);
 }
 __global__ void k_kr_check(int nb, double rtol, double *sc) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nb) return;
-  double *S = sc + c * SC_N;
-  if (S[SC_ACTIVE] != 0 && !(sqrt(S[SC_RN2]) > rtol * sqrt(S[SC_BN2]))) S[SC_ACTIVE] = 0;
+  if (c < nb) kr_check(sc + c * SC_N, rtol);
 }
 __global__ void k_kr_beta(int nb, int fcg, double *sc) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nb) return;
-  double *S = sc + c * SC_N;
-  if (fcg) {
-    S[SC_BETA] = (S[SC_ACTIVE] != 0 && S[SC_DELTA] != 0) ? -S[SC_T1] / S[SC_DELTA] : 0.0;
-  } else {
-    S[SC_BETA] = (S[SC_ACTIVE] != 0 && S[SC_GAMMA] != 0) ? S[SC_GN] / S[SC_GAMMA] : 0.0;
-    S[SC_GAMMA] = S[SC_GN];
-  }
+  if (c < nb) kr_beta(sc + c * SC_N, fcg);
 }
 __global__ void k_kr_init(int nb, double *sc) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nb) return;
-  for (int j = 0; j < SC_N; j++) sc[c * SC_N + j] = 0;
-  sc[c * SC_N + SC_ACTIVE] = 1;
+  if (c < nb) kr_init(sc + c * SC_N);
 }
 // scatter a packed dot result out[c*np + j] into scalar slots
 __global__ void k_kr_store(int nb, int np, const double *out, int s0, int s1, double *sc) {
@@ -210,6 +198,24 @@ struct Solver {
     const void *r, *z;
   };
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+  // fused coarse tail: levels tail_l0.. (H.size(): none), the row
+  // schedules of their operators (A, P, R, M per level), the device tables
+  // the programs index, and one recorded program per entry level and
+  // precision
+  size_t tail_l0 = 0;
+  std::vector<TailSched> tail_sched;
+  DevBuf<TailOp<double>> tail_ops64;
+  DevBuf<TailOp<float>> tail_ops32;
+  DevBuf<TailLevel<double>> tail_lv64;
+  DevBuf<TailLevel<float>> tail_lv32;
+  struct TailProg {
+    DevBuf<TailInstr> code;
+    int64_t len = 0;
+    const void *in = nullptr, *out = nullptr;
+  };
+  std::vector<TailProg> tail_prog[2];
+  TailParams tail_p;
+  static constexpr int64_t TAIL_MAX_INSTR = 1 << 18;
 
   ~Solver() {
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
@@ -379,6 +385,7 @@ struct Solver {
     int maxnb = 1;
     for (auto &r : red) maxnb = std::max(maxnb, r.nb);
     dsc.alloc((size_t)8 * maxnb);
+    setup_tail();
     LG_HIP(hipStreamSynchronize(s));
     lap("solve-prep", H.size(), 0, 0);
     setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -430,6 +437,346 @@ struct Solver {
       V.A = std::move(A);
     }
     LG_HIP(hipStreamSynchronize(s));
+  }
+
+  // ------------------------------------------------------ fused coarse tail
+  // A level joins the tail when its solve operators hold at most
+  // o.tail_nnz entries; the tail is the longest run of such levels that
+  // ends at the coarsest one.
+  bool tail_ok(const DLevel &L) const {
+    const int64_t lim = o.tail_nnz;
+    auto unsliced = [](const DCsr &A) { return A.slices[0].size() <= 1 && A.slices[1].size() <= 1; };
+    if (L.kind == LIGMG_LEVEL_ELIM) return L.P.nnz <= lim && L.R.nnz <= lim && L.n <= lim;
+    if (L.kind == LIGMG_LEVEL_COARSE) return L.A.nnz <= lim && L.n * L.n <= 8 * lim && unsliced(L.A);
+    return L.A.nnz <= lim && L.n <= lim && unsliced(L.A);
+  }
+
+  template <class T>
+  static constexpr int pidx() {
+    return std::is_same<T, float>::value ? 1 : 0;
+  }
+  template <class T>
+  const TailOp<T> *tail_ops() const {
+    if constexpr (std::is_same<T, double>::value) return tail_ops64.p;
+    else return tail_ops32.p;
+  }
+  template <class T>
+  const TailLevel<T> *tail_lv() const {
+    if constexpr (std::is_same<T, double>::value) return tail_lv64.p;
+    else return tail_lv32.p;
+  }
+
+  template <class T>
+  TailOp<T> tail_op(const DCsr &A, const TailSched &S) const {
+    TailOp<T> op;
+    op.rp = A.rp.p;
+    op.ci = A.ci.p;
+    // the value / diagonal choice of dispatch_spmv
+    if constexpr (std::is_same<T, double>::value) {
+      op.v = A.v.p;
+    } else {
+      op.v = A.v32.p;
+      op.d = A.d32.p;
+    }
+    op.trow = S.trow.p;
+    op.wrow = S.wrow.p;
+    op.chunk = S.chunk.p;
+    op.ntrow = S.ntrow;
+    op.nwrow = S.nwrow;
+    op.nchunk = S.nchunk;
+    op.lrow = A.U.lrow.p;
+    op.lpart = A.U.lpart.p;
+    op.nlong = (int32_t)A.U.nlong;
+    op.parts = A.U.parts.p;
+    return op;
+  }
+
+  // ---- recording: the same operations, operands and buffers as cycle /
+  // recurse / krylov / precond above, as instructions
+  static TailInstr ti(int op, int64_t n = 0) {
+    TailInstr I;
+    I.op = op;
+    I.i0 = (int32_t)n;
+    return I;
+  }
+  template <class T>
+  void rec_spmv(std::vector<TailInstr> &c, size_t l, int which, const T *x, int epi, const void *p1,
+                const void *p2 = nullptr, const void *p3 = nullptr, const void *p4 = nullptr, double f0 = 0,
+                double f1 = 0) {
+    TailInstr I = ti(TI_SPMV, (int64_t)(4 * l + which));
+    I.i1 = epi;
+    I.p[0] = x;
+    I.p[1] = p1;
+    I.p[2] = p2;
+    I.p[3] = p3;
+    I.p[4] = p4;
+    I.f[0] = f0;
+    I.f[1] = f1;
+    c.push_back(I);
+  }
+  template <class T>
+  void rec_dots(std::vector<TailInstr> &c, size_t l, int np, const T *a0, const T *b0, const T *a1,
+                const T *b1, int s0, int s1, int post) {
+    TailInstr I = ti(TI_DOTS, np);
+    I.lvl = (int32_t)l;
+    I.i1 = s0;
+    I.i2 = s1;
+    I.i3 = post;
+    I.p[0] = a0;
+    I.p[1] = b0;
+    I.p[2] = a1;
+    I.p[3] = b1;
+    c.push_back(I);
+  }
+  template <class T>
+  void rec_cheb(std::vector<TailInstr> &c, size_t l, const T *dinv, const T *b, const T *xc, const T *xp,
+                T *xn, double alpha, double beta) {
+    rec_spmv<T>(c, l, 0, xc, TE_CHEB, dinv, b, xp, xn, alpha, beta);
+  }
+  template <class T>
+  int rec_smooth_from_zero(std::vector<TailInstr> &c, size_t l, const T *dinv, const T *b) {
+    LevelWork<T> &w = work<T>(l);
+    const int64_t n = H[l].n;
+    const int m = o.pre_smooth;
+    if (m <= 0 || o.smoother_type == LIGMG_SMOOTHER_JACOBI) {
+      TailInstr I = ti(TI_ZERO, n);
+      I.p[0] = w.B[0].p;
+      c.push_back(I);
+      if (m <= 0) return 0;
+      rec_cheb<T>(c, l, dinv, b, w.B[0].p, nullptr, w.B[1].p, 0.0, 2.0 / 3.0);
+      return 1;
+    }
+    TailInstr I = ti(TI_CHEB0, n);
+    I.p[0] = dinv;
+    I.p[1] = b;
+    I.p[2] = w.B[1].p;
+    I.f[0] = be_pre[l][0];
+    c.push_back(I);
+    for (int k = 1; k < m; k++) {
+      const T *xp = (k == 1) ? nullptr : w.B[(k - 1) % 3].p;
+      rec_cheb<T>(c, l, dinv, b, w.B[k % 3].p, xp, w.B[(k + 1) % 3].p, al_pre[l][k], be_pre[l][k]);
+    }
+    return m % 3;
+  }
+  template <class T>
+  int rec_smooth_from(std::vector<TailInstr> &c, size_t l, const T *dinv, const T *b, int s0) {
+    LevelWork<T> &w = work<T>(l);
+    const int m = o.post_smooth;
+    if (m <= 0) return s0;
+    if (o.smoother_type == LIGMG_SMOOTHER_JACOBI) {
+      const int s1 = (s0 + 1) % 3;
+      rec_cheb<T>(c, l, dinv, b, w.B[s0].p, nullptr, w.B[s1].p, 0.0, 2.0 / 3.0);
+      return s1;
+    }
+    for (int k = 0; k < m; k++) {
+      const T *xp = (k == 0) ? nullptr : w.B[(s0 + k - 1) % 3].p;
+      rec_cheb<T>(c, l, dinv, b, w.B[(s0 + k) % 3].p, xp, w.B[(s0 + k + 1) % 3].p, al_post[l][k],
+                  be_post[l][k]);
+    }
+    return (s0 + m) % 3;
+  }
+  template <class T>
+  const T *rec_cycle(std::vector<TailInstr> &c, size_t l, const T *b) {
+    DLevel &L = H[l];
+    LevelWork<T> &w = work<T>(l);
+    if (L.kind == LIGMG_LEVEL_COARSE) {
+      TailInstr I = ti(TI_DENSE, L.n);
+      I.p[0] = L.pinv.p;
+      I.p[1] = b;
+      I.p[2] = w.B[0].p;
+      c.push_back(I);
+      return w.B[0].p;
+    }
+    LevelWork<T> &wn = work<T>(l + 1);
+    if (L.kind == LIGMG_LEVEL_ELIM) {
+      rec_spmv<T>(c, l, 2, b, TE_STORE, wn.b.p);
+      const T *xc = rec_recurse<T>(c, l, wn.b.p);
+      rec_spmv<T>(c, l, 1, xc, TE_INTERP, qv<T>(L), b, w.B[0].p);
+      return w.B[0].p;
+    }
+    const T *dinv = dinvv<T>(L);
+    int cur = rec_smooth_from_zero<T>(c, l, dinv, b);
+    rec_spmv<T>(c, l, 0, w.B[cur].p, TE_RESID, b, w.R.p);
+    rec_spmv<T>(c, l, 3, w.R.p, TE_STORE, wn.b.p);
+    const T *xc = rec_recurse<T>(c, l, wn.b.p);
+    TailInstr I = ti(TI_PROLONG, L.n);
+    I.p[0] = L.agg.p;
+    I.p[1] = xc;
+    I.p[2] = w.B[cur].p;
+    c.push_back(I);
+    cur = rec_smooth_from<T>(c, l, dinv, b, cur);
+    if (o.constant_correction) {
+      TailInstr S = ti(TI_SCALE, L.n);
+      S.p[0] = w.B[cur].p;
+      S.f[0] = 4.0 / 3.0;
+      c.push_back(S);
+    }
+    return w.B[cur].p;
+  }
+  template <class T>
+  const T *rec_recurse(std::vector<TailInstr> &c, size_t l, const T *bc) {
+    const DLevel &N = H[l + 1];
+    if (!o.kcycle || N.kind == LIGMG_LEVEL_ELIM) return rec_cycle<T>(c, l + 1, bc);
+    KWork<T> &K = work<T>(l + 1).kw;
+    TailInstr I = ti(TI_ZERO, N.A.n);
+    I.p[0] = K.x.p;
+    c.push_back(I);
+    rec_krylov<T>(c, l + 1, K.x.p, bc);
+    return K.x.p;
+  }
+  template <class T>
+  void rec_precond(std::vector<TailInstr> &c, size_t lvl, const T *r, T *z) {
+    const int64_t n = H[lvl].A.n;
+    const T *cy = rec_cycle<T>(c, lvl, r);
+    TailInstr I = ti(TI_COPY, n);
+    I.p[0] = z;
+    I.p[1] = cy;
+    c.push_back(I);
+    rec_dots<T>(c, lvl, 1, z, nullptr, nullptr, nullptr, -1, -1, TP_NONE);
+    TailInstr M = ti(TI_SUBMEAN, n);
+    M.lvl = (int32_t)lvl;
+    M.p[0] = z;
+    c.push_back(M);
+  }
+  template <class T>
+  void rec_krylov(std::vector<TailInstr> &c, size_t lvl, T *x, const T *b) {
+    KWork<T> &K = work<T>(lvl).kw;
+    const int64_t n = H[lvl].A.n;
+    const bool fcg = o.kcycle_ksp_type == LIGMG_KSP_FCG;
+    TailInstr I = ti(TI_KR_INIT);
+    I.lvl = (int32_t)lvl;
+    c.push_back(I);
+    rec_spmv<T>(c, lvl, 0, x, TE_RESID, b, K.r.p);
+    rec_dots<T>(c, lvl, 2, b, b, K.r.p, K.r.p, SC_BN2, SC_RN2, TP_CHECK);
+    rec_precond<T>(c, lvl, K.r.p, K.z.p);
+    TailInstr C = ti(TI_COPY, n);
+    C.p[0] = K.p.p;
+    C.p[1] = K.z.p;
+    c.push_back(C);
+    rec_dots<T>(c, lvl, 1, K.r.p, K.z.p, nullptr, nullptr, SC_GAMMA, SC_GAMMA, TP_NONE);
+    int it = 0;
+    while (it < o.kcycle_iters) {
+      rec_spmv<T>(c, lvl, 0, K.p.p, TE_STORE, K.q.p);
+      if (fcg) rec_dots<T>(c, lvl, 2, K.p.p, K.q.p, K.p.p, K.r.p, SC_DELTA, SC_T1, TP_ALPHA);
+      else rec_dots<T>(c, lvl, 1, K.p.p, K.q.p, nullptr, nullptr, SC_DELTA, SC_DELTA, TP_ALPHA);
+      TailInstr U = ti(TI_UPDATE);
+      U.lvl = (int32_t)lvl;
+      U.p[0] = x;
+      c.push_back(U);
+      it++;
+      rec_dots<T>(c, lvl, 1, K.r.p, K.r.p, nullptr, nullptr, SC_RN2, SC_RN2, TP_CHECK);
+      if (it >= o.kcycle_iters) break;
+      rec_precond<T>(c, lvl, K.r.p, K.z.p);
+      if (fcg) rec_dots<T>(c, lvl, 1, K.z.p, K.q.p, nullptr, nullptr, SC_T1, SC_T1, TP_BETA);
+      else rec_dots<T>(c, lvl, 1, K.r.p, K.z.p, nullptr, nullptr, SC_GN, SC_GN, TP_BETA);
+      TailInstr B = ti(TI_PUPDATE);
+      B.lvl = (int32_t)lvl;
+      c.push_back(B);
+    }
+  }
+
+  template <class T>
+  void make_tail(DevBuf<TailOp<T>> &ops, DevBuf<TailLevel<T>> &lv) {
+    const size_t nl = H.size();
+    std::vector<TailOp<T>> ho(4 * nl);
+    std::vector<TailLevel<T>> hl(nl);
+    for (size_t l = tail_l0; l < nl; l++) {
+      const DLevel &L = H[l];
+      if (L.kind == LIGMG_LEVEL_ELIM) {
+        ho[4 * l + 1] = tail_op<T>(L.P, tail_sched[4 * l + 1]);
+        ho[4 * l + 2] = tail_op<T>(L.R, tail_sched[4 * l + 2]);
+      } else {
+        ho[4 * l] = tail_op<T>(L.A, tail_sched[4 * l]);
+      }
+      if (L.kind == LIGMG_LEVEL_AGG) ho[4 * l + 3] = tail_op<T>(L.M, tail_sched[4 * l + 3]);
+      const RedPlan &R = red[l];
+      const KWork<T> &K = work<T>(l).kw;
+      TailLevel<T> &t = hl[l];
+      t.n = (int32_t)L.A.n;
+      t.nb = R.nb;
+      t.nseg = R.nseg;
+      t.seg_lo = R.seg_lo.p;
+      t.seg_hi = R.seg_hi.p;
+      t.bptr = R.bptr.p;
+      t.bseg = R.bseg.p;
+      t.part = R.part.p;
+      t.sc = K.sc.p;
+      t.bsum = dsc.p;
+      t.kr = K.r.p;
+      t.kz = K.z.p;
+      t.kp = K.p.p;
+      t.kq = K.q.p;
+    }
+    ops.alloc(4 * nl);
+    lv.alloc(nl);
+    to_device(ops.p, ho.data(), ho.size(), s);
+    to_device(lv.p, hl.data(), hl.size(), s);
+    // one program per entry level: the correction recurse(e) computes
+    auto &progs = tail_prog[pidx<T>()];
+    progs.clear();
+    progs.resize(nl);
+    for (size_t e = tail_l0 - 1; e + 1 < nl; e++) {
+      std::vector<TailInstr> code;
+      const T *out = rec_recurse<T>(code, e, work<T>(e + 1).b.p);
+      TailProg &P = progs[e];
+      P.code.alloc(code.size());
+      to_device(P.code.p, code.data(), code.size(), s);
+      P.len = (int64_t)code.size();
+      P.in = work<T>(e + 1).b.p;
+      P.out = out;
+    }
+  }
+
+  // instructions of the correction below level e (growing with
+  // kcycle_iters^depth for a K-cycle), without recording it
+  int64_t tail_count(size_t e) const {
+    const size_t nl = H.size();
+    std::vector<int64_t> cyc(nl + 1, 0), rec(nl + 1, 0);
+    const int m = o.kcycle_iters;
+    for (size_t l = nl; l-- > e + 1;) {
+      const DLevel &L = H[l];
+      const int pre = o.pre_smooth, post = o.post_smooth;
+      if (L.kind == LIGMG_LEVEL_COARSE) cyc[l] = 1;
+      else if (L.kind == LIGMG_LEVEL_ELIM) cyc[l] = 2 + rec[l];
+      else cyc[l] = 3 + std::max(pre, 1) + std::max(post, 1) + 1 + rec[l];
+      // recurse(l - 1)
+      const bool k = o.kcycle && L.kind != LIGMG_LEVEL_ELIM;
+      rec[l - 1] = k ? 1 + 6 + (int64_t)m * (6 + cyc[l] + 3) : cyc[l];
+    }
+    return rec[e];
+  }
+
+  void setup_tail() {
+    const size_t nl = H.size();
+    tail_l0 = nl;
+    if (o.tail_nnz <= 0) return;
+    const size_t base = (o.top_elim && H[0].kind == LIGMG_LEVEL_ELIM) ? 1 : 0;
+    size_t l0 = nl;
+    while (l0 > base + 1 && tail_ok(H[l0 - 1])) l0--;
+    // bounded program size (a deep K-cycle tail grows geometrically)
+    while (l0 < nl && tail_count(l0 - 1) > TAIL_MAX_INSTR) l0++;
+    if (l0 >= nl) return;
+    tail_l0 = l0;
+    tail_sched.clear();
+    tail_sched.resize(4 * nl);
+    for (size_t l = l0; l < nl; l++) {
+      const DLevel &L = H[l];
+      if (L.kind == LIGMG_LEVEL_ELIM) {
+        tail_schedule(L.P, tail_sched[4 * l + 1], s);
+        tail_schedule(L.R, tail_sched[4 * l + 2], s);
+        continue;
+      }
+      tail_schedule(L.A, tail_sched[4 * l], s);
+      if (L.kind == LIGMG_LEVEL_AGG) tail_schedule(L.M, tail_sched[4 * l + 3], s);
+    }
+    make_tail<double>(tail_ops64, tail_lv64);
+    if (mixed) make_tail<float>(tail_ops32, tail_lv32);
+    tail_p.fcg = o.kcycle_ksp_type == LIGMG_KSP_FCG;
+    tail_p.rtol = o.tol;
+    LG_HIP(hipStreamSynchronize(s));
+    if (o.verbose)
+      fprintf(stderr, "[ligmg] fused tail: levels %zu-%zu, %lld instructions per correction\n", l0, nl - 1,
+              (long long)tail_prog[0][l0 - 1].len);
   }
 
   void tune_slices(DCsr &A, int prec, size_t l) {
@@ -548,6 +895,12 @@ struct Solver {
   template <class T>
   const T *recurse(size_t l, const T *bc) {
     const DLevel &N = H[l + 1];
+    if (l + 1 >= tail_l0) {
+      const TailProg &P = tail_prog[pidx<T>()][l];
+      LG_CHECK(P.in == bc, LIGMG_ERR_INVALID, "fused tail entered with an unrecorded right-hand side");
+      tail_launch<T>(P.code.p, P.len, tail_ops<T>(), tail_lv<T>(), tail_p, s);
+      return (const T *)P.out;
+    }
     if (!o.kcycle || N.kind == LIGMG_LEVEL_ELIM) return cycle<T>(l + 1, bc);
     // K-cycle: kcycle_iters steps of FCG on the next level (multigrid.hpp:113-126)
     LevelWork<T> &wn = work<T>(l + 1);
@@ -1254,6 +1607,7 @@ void ligmg_options_default(ligmg_options *o) {
   o->spmv_slice = 0;
   o->ordering = LIGMG_ORDER_NATURAL;
   o->solve_layout = 1;
+  o->tail_nnz = 4096;
 }
 
 const char *ligmg_last_error(void) { return lg::g_last_error.c_str(); }
@@ -1264,7 +1618,7 @@ static int check_options(const ligmg_options &o) {
   if (o.neighbor_of_neighbor) return LIGMG_ERR_UNSUPPORTED;
   if (o.num_tvs < 1 || o.num_tvs > 16) return LIGMG_ERR_UNSUPPORTED;
   if (o.aff_scaling != LIGMG_AFF_MAX && o.aff_scaling != LIGMG_AFF_NONE) return LIGMG_ERR_UNSUPPORTED;
-  if (o.pre_smooth < 0 || o.post_smooth < 0 || o.iters < 0 || o.kcycle_iters < 1)
+  if (o.pre_smooth < 0 || o.post_smooth < 0 || o.iters < 0 || o.kcycle_iters < 1 || o.tail_nnz < 0)
     return LIGMG_ERR_INVALID;
   return LIGMG_OK;
 }
@@ -1343,6 +1697,12 @@ int ligmg_ordering(ligmg_handle h, int32_t *perm) {
 int ligmg_num_levels(ligmg_handle h, int32_t *out) {
   if (!h || !out) return LIGMG_ERR_INVALID;
   *out = (int32_t)h->S.H.size();
+  return LIGMG_OK;
+}
+
+int ligmg_tail_start(ligmg_handle h, int32_t *out) {
+  if (!h || !out) return LIGMG_ERR_INVALID;
+  *out = (int32_t)h->S.tail_l0;
   return LIGMG_OK;
 }
 

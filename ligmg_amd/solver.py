@@ -241,6 +241,12 @@ class Solver:
         _check(self._lib.ligmg_num_levels(self._h, C.byref(n)), "ligmg_num_levels")
         return n.value
 
+    def tail_start(self):
+        """First level of the fused coarse tail (num_levels() if none)."""
+        n = C.c_int32()
+        _check(self._lib.ligmg_tail_start(self._h, C.byref(n)), "ligmg_tail_start")
+        return n.value
+
     def level_info(self, l):
         k, n, nnz, nc = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64()
         _check(self._lib.ligmg_level_info(self._h, l, C.byref(k), C.byref(n), C.byref(nnz),

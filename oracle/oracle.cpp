@@ -1115,6 +1115,7 @@ void oracle_default_options(ligmg_options *o) {
   o->spmv_slice = 0;
   o->ordering = LIGMG_ORDER_NATURAL;
   o->solve_layout = 1;
+  o->tail_nnz = 4096;
 }
 
 int oracle_create(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
