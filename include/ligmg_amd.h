@@ -33,7 +33,8 @@ enum {
   LIGMG_ERR_UNSUPPORTED = 2, /* option value not implemented                */
   LIGMG_ERR_DEVICE = 3,      /* HIP runtime failure                          */
   LIGMG_ERR_NOMEM = 4,       /* allocation failed                            */
-  LIGMG_ERR_NUMERIC = 5      /* NaN/Inf in input or a singular coarse level  */
+  LIGMG_ERR_NUMERIC = 5,     /* NaN/Inf in input or a singular coarse level  */
+  LIGMG_ERR_IO = 6           /* file missing, unreadable or malformed        */
 };
 
 /* Krylov methods (options.hpp:205-206 KSPType values used by the reference) */
@@ -202,6 +203,28 @@ const char *ligmg_last_error(void);
 /* ---- hierarchy introspection (print_hierarchy, multigrid.hpp:265-292;
  *      dump_hierarchy, ligmg.hpp:47-52) ---------------------------------- */
 int ligmg_num_levels(ligmg_handle h, int32_t *out);
+/* dump_hierarchy (ligmg.hpp:47-52): creates directory dir and writes each
+ * level in PETSc binary format, in setup order, as the reference's dump
+ * methods name them: <dir>/<level>_A.petsc for every level; P, R, Q for an
+ * elimination level (elimination.hpp:62-67); P, R and strength for an
+ * aggregation level (aggregation.hpp:254-272), whose AggState history the
+ * reference writes per voting round and this build as the final
+ * agg_ind.petsc / agg_state.petsc vectors.  blocks.petsc holds comp_ptr and
+ * ordering.petsc the input order (LIGMG_ORDER_DEGREE). */
+int ligmg_dump_hierarchy(ligmg_handle h, const char *dir);
+/* A solver from a dump (new; the reference only writes dumps): the levels
+ * are read back instead of constructed, then the solve-phase set-up runs as
+ * in ligmg_create (smoothers, eigen-estimates, FP32 copies, layout), so
+ * solves match those of the dumped solver for the same solve options.
+ * Returns LIGMG_ERR_IO for missing or malformed files. */
+int ligmg_load_hierarchy(const char *dir, const ligmg_options *opts, ligmg_handle *out);
+/* dump_csv (ligmg.hpp:42-45): the header line of print_csv_header
+ * (util.hpp:187-193, options.hpp:139-147) and the result line of print_csv
+ * (util.hpp:195-205, options.hpp:149-164) for this solver and the solve
+ * described by info, NUL-terminated in buf.  graph names the graph (the
+ * reference prints the input file name without directory and extension);
+ * options without a field here are printed with the reference's defaults. */
+int ligmg_dump_csv(ligmg_handle h, const char *graph, const ligmg_info *info, char *buf, int64_t cap);
 /* First level of the fused coarse tail (new; see tail_nnz): levels from
  * *out down run in one launch per coarse-grid correction; *out equals the
  * number of levels when nothing is fused. */

@@ -4,6 +4,8 @@
 //   ligmg::Solver<T,F,DER>(A, opts) / Solver(A)   include/ligmg/ligmg.hpp:17-28
 //   Solver::solve(b)                              include/ligmg/ligmg.hpp:31-33
 //   Solver::print()                               include/ligmg/ligmg.hpp:40
+//   Solver::dump_csv(os, info)                    include/ligmg/ligmg.hpp:42-45
+//   Solver::dump_hierarchy(dir)                   include/ligmg/ligmg.hpp:47-52
 //   ConvergenceInfo<T,F> (residuals, iters(), time(), work(), solution,
 //                         wda(), ecf(), cf(), cycle_complexity(),
 //                         rel_residual())         include/ligmg/util.hpp:143-170
@@ -17,6 +19,7 @@
 #pragma once
 #include <cmath>
 #include <cstdio>
+#include <ostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -129,10 +132,35 @@ class Solver {
     }
   }
 
+  // dump_csv: header and result line of a finished solve
+  void dump_csv(std::ostream &os, const ConvergenceInfo &cv, const std::string &graph = "") const {
+    std::vector<char> buf(1 << 14);
+    check(ligmg_dump_csv(h_, graph.c_str(), &cv.info, buf.data(), (int64_t)buf.size()), "ligmg_dump_csv");
+    os << buf.data();
+  }
+
+  // dump_hierarchy: PETSc binary files of every level under dir
+  void dump_hierarchy(const std::string &dir) const {
+    check(ligmg_dump_hierarchy(h_, dir.c_str()), "ligmg_dump_hierarchy");
+  }
+
+  // a solver from a dump (new)
+  static Solver load(const std::string &dir, const Options &o = default_options()) {
+    Solver S;
+    S.opts = o;
+    check(ligmg_load_hierarchy(dir.c_str(), &S.opts, &S.h_), "ligmg_load_hierarchy");
+    int32_t kind;
+    int64_t nnz, nc;
+    ligmg_level_info(S.h_, 0, &kind, &S.n_, &nnz, &nc);
+    ligmg_setup_time(S.h_, &S.setup_time);
+    return S;
+  }
+
   ligmg_handle handle() const { return h_; }
   int64_t size() const { return n_; }
 
  private:
+  Solver() = default;
   ligmg_handle h_ = nullptr;
   int64_t n_ = 0;
 };

@@ -54,7 +54,7 @@ class ligmg_graph(C.Structure):
 
 
 # enums (include/ligmg_amd.h)
-OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_DEVICE, ERR_NOMEM, ERR_NUMERIC = range(6)
+OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_DEVICE, ERR_NOMEM, ERR_NUMERIC, ERR_IO = range(7)
 KSP_CG, KSP_FCG = 0, 1
 SMOOTHER_JACOBI, SMOOTHER_CHEBYSHEV = 0, 1
 AFF_NONE, AFF_ROW, AFF_COLUMN, AFF_ROWCOLUMN, AFF_SYM, AFF_MAX = range(6)
@@ -89,6 +89,9 @@ SOLVER_PROTOS = {
     "ligmg_last_error": (C.c_char_p, []),
     "ligmg_num_levels": (C.c_int, [HANDLE, P(i32)]),
     "ligmg_tail_start": (C.c_int, [HANDLE, P(i32)]),
+    "ligmg_dump_hierarchy": (C.c_int, [HANDLE, C.c_char_p]),
+    "ligmg_load_hierarchy": (C.c_int, [C.c_char_p, P(ligmg_options), P(HANDLE)]),
+    "ligmg_dump_csv": (C.c_int, [HANDLE, C.c_char_p, P(ligmg_info), C.c_char_p, i64]),
     "ligmg_setup_time": (C.c_int, [HANDLE, P(f64)]),
     "ligmg_level_info": (C.c_int, [HANDLE, i32, P(i32), P(i64), P(i64), P(i64)]),
     "ligmg_level_matrix": (C.c_int, [HANDLE, i32, P(i32), P(i32), P(f64)]),

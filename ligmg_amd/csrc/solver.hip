@@ -12,10 +12,12 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cerrno>
 #include <cstring>
 #include <memory>
 #include <string>
 #include <type_traits>
+#include <sys/stat.h>
 
 #include "../../include/ligmg_amd.h"
 #include "ops.hpp"
@@ -23,6 +25,7 @@
 #include "setup.hpp"
 #include "solve_dev.hpp"
 #include "tail.hpp"
+#include "petsc_io.hpp"
 
 namespace lg {
 
@@ -304,8 +307,17 @@ struct Solver {
       H.push_back(std::move(C));
       lap("coarse", H.size() - 1, H.back().A.n, H.back().A.nnz);
     }
-    // blocks per level: level-0 blocks mapped through kept ids / roots
     H[0].blocks = blocks0;
+    finish_setup();
+    lap("solve-prep", H.size(), 0, 0);
+    setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  // Everything after the hierarchy itself: per-level blocks, the solve
+  // layout, SpMV units, smoothers and eigen-estimates, FP32 copies, work
+  // vectors, slicing, the fused tail.  Shared by build and load.
+  void finish_setup() {
+    // blocks per level: level-0 blocks mapped through kept ids / roots
     for (size_t l = 0; l + 1 < H.size(); l++) {
       const DLevel &L = H[l];
       std::vector<int64_t> nb(L.blocks.size(), 0);
@@ -387,8 +399,280 @@ struct Solver {
     dsc.alloc((size_t)8 * maxnb);
     setup_tail();
     LG_HIP(hipStreamSynchronize(s));
-    lap("solve-prep", H.size(), 0, 0);
+    LG_HIP(hipStreamSynchronize(s));
+  }
+
+  // ---------------------------------------------------------- dump / load
+  // An operator stored with solve-order rows (row i is setup row rperm[i])
+  // and columns (column c is setup column cperm[c]) back in setup order,
+  // rows sorted by column as setup left them.  Null perms: identity.
+  pio::HostCsr to_setup(const DCsr &A, const int32_t *rperm, const int32_t *cperm, bool vals) {
+    pio::HostCsr H;
+    const int64_t m = A.n;
+    H.m = m;
+    H.ncols = A.ncols;
+    auto prp = download(A.rp.p, m + 1, s);
+    auto pci = download(A.ci.p, A.nnz, s);
+    std::vector<double> pv = vals && A.v.p ? download(A.v.p, A.nnz, s) : std::vector<double>(A.nnz, 1.0);
+    std::vector<int32_t> rp_ = rperm ? download(rperm, m, s) : std::vector<int32_t>();
+    std::vector<int32_t> cp_ = cperm ? download(cperm, A.ncols, s) : std::vector<int32_t>();
+    std::vector<int32_t> len(m);
+    for (int64_t i = 0; i < m; i++) len[rperm ? rp_[i] : i] = prp[i + 1] - prp[i];
+    H.rp.assign(m + 1, 0);
+    for (int64_t i = 0; i < m; i++) H.rp[i + 1] = H.rp[i] + len[i];
+    H.ci.resize(A.nnz);
+    H.v.resize(A.nnz);
+    std::vector<std::pair<int32_t, double>> row;
+    for (int64_t i = 0; i < m; i++) {
+      row.clear();
+      for (int32_t k = prp[i]; k < prp[i + 1]; k++) row.push_back({cperm ? cp_[pci[k]] : pci[k], pv[k]});
+      std::stable_sort(row.begin(), row.end(),
+                       [](const std::pair<int32_t, double> &a, const std::pair<int32_t, double> &b) {
+                         return a.first < b.first;
+                       });
+      int32_t d = H.rp[rperm ? rp_[i] : i];
+      for (const auto &e : row) {
+        H.ci[d] = e.first;
+        H.v[d++] = e.second;
+      }
+    }
+    return H;
+  }
+
+  static void write_host(const std::string &p, const pio::HostCsr &A) {
+    pio::write_mat(p, A.m, A.ncols, A.rp.data(), A.ci.data(), A.v.data());
+  }
+
+  // dump_hierarchy (ligmg.hpp:47-52) with the level files of the dump
+  // methods (elimination.hpp:62-67, aggregation.hpp:254-272,
+  // multigrid.hpp:81-83); the aggregation states are the final ones.
+  void dump(const std::string &dir) {
+    if (mkdir(dir.c_str(), 0777) != 0 && errno != EEXIST)
+      throw Error(pio::ERR_IO, "cannot create directory " + dir);
+    const size_t nl = H.size();
+    for (size_t l = 0; l < nl; l++) {
+      const DLevel &L = H[l];
+      const std::string pre = dir + "/" + std::to_string(l) + "_";
+      const int32_t *sp = L.sperm.p, *nsp = (l + 1 < nl) ? H[l + 1].sperm.p : nullptr;
+      const pio::HostCsr A = to_setup(L.A, sp, sp, true);
+      write_host(pre + "A.petsc", A);
+      const int64_t n = L.A.n;
+      if (L.kind == LIGMG_LEVEL_ELIM) {
+        write_host(pre + "P.petsc", to_setup(L.P, sp, nsp, true));
+        write_host(pre + "R.petsc", to_setup(L.R, nsp, sp, true));
+        const auto nid = download(L.new_id.p, n, s);
+        auto qs = download(L.q.p, n, s);
+        std::vector<double> q(n);
+        if (sp) {
+          const auto spv = download(sp, n, s);
+          for (int64_t i = 0; i < n; i++) q[spv[i]] = qs[i];
+        } else {
+          q = qs;
+        }
+        std::vector<int32_t> qrp(n + 1, 0), qci;
+        std::vector<double> qv;
+        for (int64_t i = 0; i < n; i++) {
+          if (nid[i] < 0) {
+            qci.push_back((int32_t)i);
+            qv.push_back(q[i]);
+          }
+          qrp[i + 1] = (int32_t)qci.size();
+        }
+        pio::write_mat(pre + "Q.petsc", n, n, qrp.data(), qci.data(), qv.data());
+      } else if (L.kind == LIGMG_LEVEL_AGG) {
+        const auto agg = download(L.agg_setup.p ? L.agg_setup.p : L.agg.p, n, s);
+        std::vector<int32_t> prp(n + 1), rrp(L.nc + 1, 0), rci(n);
+        for (int64_t i = 0; i <= n; i++) prp[i] = (int32_t)i;
+        for (int64_t i = 0; i < n; i++) rrp[agg[i] + 1]++;
+        for (int64_t c = 0; c < L.nc; c++) rrp[c + 1] += rrp[c];
+        std::vector<int32_t> fill(rrp.begin(), rrp.end() - 1);
+        for (int64_t i = 0; i < n; i++) rci[fill[agg[i]]++] = (int32_t)i;
+        pio::write_mat(pre + "P.petsc", n, L.nc, prp.data(), agg.data(), nullptr);
+        pio::write_mat(pre + "R.petsc", L.nc, n, rrp.data(), rci.data(), nullptr);
+        const auto st = download(L.strength.p, L.A.nnz, s);
+        pio::write_mat(pre + "strength.petsc", n, n, A.rp.data(), A.ci.data(), st.data());
+        const auto ind = download(L.ind.p, n, s), state = download(L.state.p, n, s);
+        std::vector<double> di(ind.begin(), ind.end()), ds(state.begin(), state.end());
+        pio::write_vec(pre + "agg_ind.petsc", di.data(), n);
+        pio::write_vec(pre + "agg_state.petsc", ds.data(), n);
+      }
+    }
+    std::vector<double> b(H[0].blocks.begin(), H[0].blocks.end());
+    pio::write_vec(dir + "/blocks.petsc", b.data(), (int64_t)b.size());
+    if (perm_input.p) {
+      const auto pi = download(perm_input.p, H[0].A.n, s);
+      std::vector<double> d(pi.begin(), pi.end());
+      pio::write_vec(dir + "/ordering.petsc", d.data(), (int64_t)d.size());
+    }
+  }
+
+  static std::vector<int32_t> as_ints(const std::vector<double> &x, int64_t n, const std::string &what) {
+    LG_CHECK((int64_t)x.size() == n, pio::ERR_IO, what + ": wrong length");
+    std::vector<int32_t> r(n);
+    for (int64_t i = 0; i < n; i++) {
+      LG_CHECK(x[i] == (double)(int32_t)x[i], pio::ERR_IO, what + ": not an integer vector");
+      r[i] = (int32_t)x[i];
+    }
+    return r;
+  }
+
+  void upload_csr(const pio::HostCsr &A, DCsr &D, bool vals) {
+    D.n = A.m;
+    D.ncols = A.ncols;
+    D.nnz = (int64_t)A.ci.size();
+    D.rp.alloc(A.m + 1);
+    D.ci.alloc(std::max<int64_t>(D.nnz, 1));
+    to_device(D.rp.p, A.rp.data(), A.m + 1, s);
+    to_device(D.ci.p, A.ci.data(), D.nnz, s);
+    if (vals) {
+      D.v.alloc(std::max<int64_t>(D.nnz, 1));
+      to_device(D.v.p, A.v.data(), D.nnz, s);
+    }
+  }
+
+  // A solver from a dumped hierarchy: the levels are read back, not
+  // constructed; the solve-phase set-up runs as after build.
+  void load(const std::string &dir) {
+    LG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    mixed = o.precision == LIGMG_PREC_MIXED;
+    auto t0 = std::chrono::steady_clock::now();
+    for (size_t l = 0;; l++) {
+      const std::string pre = dir + "/" + std::to_string(l) + "_";
+      if (!pio::exists(pre + "A.petsc")) {
+        LG_CHECK(l > 0, pio::ERR_IO, "no hierarchy dump in " + dir);
+        break;
+      }
+      const pio::HostCsr A = pio::read_mat(pre + "A.petsc");
+      const int64_t n = A.m;
+      LG_CHECK(n > 0 && A.ncols == n, pio::ERR_IO, pre + "A.petsc: not a square matrix");
+      LG_CHECK(l == 0 || H.back().nc == n, pio::ERR_IO, pre + "A.petsc: size does not match the level above");
+      LG_CHECK(H.empty() || H.back().kind != LIGMG_LEVEL_COARSE, pio::ERR_IO, "levels after the coarse level");
+      DLevel L;
+      L.n = n;
+      upload_csr(A, L.A, true);
+      validate_csr(n, n, L.A.rp.p, L.A.ci.p, s, true);
+      if (pio::exists(pre + "Q.petsc")) {
+        const pio::HostCsr P = pio::read_mat(pre + "P.petsc"), Q = pio::read_mat(pre + "Q.petsc");
+        LG_CHECK(P.m == n && Q.m == n && Q.ncols == n, pio::ERR_IO, pre + "P/Q: wrong shape");
+        std::vector<int32_t> nid(n);
+        std::vector<double> q(n, 0.0);
+        int64_t kept = 0;
+        for (int64_t i = 0; i < n; i++) {
+          const int32_t a = Q.rp[i], b = Q.rp[i + 1];
+          LG_CHECK(b - a <= 1 && (b == a || Q.ci[a] == i), pio::ERR_IO, pre + "Q.petsc: not diagonal");
+          if (b > a) {
+            nid[i] = -1;
+            q[i] = Q.v[a];
+          } else {
+            nid[i] = (int32_t)kept++;
+            LG_CHECK(P.rp[i + 1] - P.rp[i] == 1 && P.ci[P.rp[i]] == nid[i] && P.v[P.rp[i]] == 1.0, pio::ERR_IO,
+                     pre + "P.petsc: a kept row is not a unit row");
+          }
+        }
+        LG_CHECK(P.ncols == kept, pio::ERR_IO, pre + "P.petsc: column count differs from the kept vertices");
+        L.kind = LIGMG_LEVEL_ELIM;
+        L.nc = kept;
+        L.new_id.alloc(n);
+        to_device(L.new_id.p, nid.data(), n, s);
+        L.q.alloc(n);
+        to_device(L.q.p, q.data(), n, s);
+        upload_csr(P, L.P, true);
+        L.nnzP = L.P.nnz;
+        L.nnzQ = n - kept;
+        transpose_p(L.P, kept, L.R, s);
+      } else if (pio::exists(pre + "P.petsc")) {
+        const pio::HostCsr P = pio::read_mat(pre + "P.petsc");
+        LG_CHECK(P.m == n, pio::ERR_IO, pre + "P.petsc: wrong shape");
+        std::vector<int32_t> agg(n);
+        for (int64_t i = 0; i < n; i++) {
+          LG_CHECK(P.rp[i + 1] - P.rp[i] == 1 && P.v[P.rp[i]] == 1.0, pio::ERR_IO,
+                   pre + "P.petsc: not a piecewise-constant interpolation");
+          agg[i] = P.ci[P.rp[i]];
+        }
+        const pio::HostCsr S = pio::read_mat(pre + "strength.petsc");
+        LG_CHECK(S.m == n && S.rp == A.rp && S.ci == A.ci, pio::ERR_IO,
+                 pre + "strength.petsc: pattern differs from A");
+        const auto ind = as_ints(pio::read_vec(pre + "agg_ind.petsc"), n, pre + "agg_ind.petsc");
+        const auto state = as_ints(pio::read_vec(pre + "agg_state.petsc"), n, pre + "agg_state.petsc");
+        L.kind = LIGMG_LEVEL_AGG;
+        L.nc = P.ncols;
+        L.agg.alloc(n);
+        L.ind.alloc(n);
+        L.state.alloc(n);
+        to_device(L.agg.p, agg.data(), n, s);
+        to_device(L.ind.p, ind.data(), n, s);
+        to_device(L.state.p, state.data(), n, s);
+        L.strength.alloc(std::max<size_t>(S.v.size(), 1));
+        to_device(L.strength.p, S.v.data(), S.v.size(), s);
+        build_members(L.agg.p, n, L.nc, L.M, s);
+      } else {
+        L.kind = LIGMG_LEVEL_COARSE;
+        auto pinv = dense_pinv_host(n, A.rp, A.ci, A.v);
+        L.pinv.alloc(std::max<size_t>(pinv.size(), 1));
+        to_device(L.pinv.p, pinv.data(), pinv.size(), s);
+      }
+      H.push_back(std::move(L));
+    }
+    LG_CHECK(H.back().kind == LIGMG_LEVEL_COARSE, pio::ERR_IO, "the dump does not end with a coarse level");
+    const int64_t n0 = H[0].A.n;
+    std::vector<int64_t> blocks = {0, n0};
+    if (pio::exists(dir + "/blocks.petsc")) {
+      const auto b = pio::read_vec(dir + "/blocks.petsc");
+      blocks.assign(b.begin(), b.end());
+      LG_CHECK(blocks.size() >= 2 && blocks.front() == 0 && blocks.back() == n0, pio::ERR_IO,
+               "blocks.petsc must span [0, n]");
+      for (size_t c = 0; c + 1 < blocks.size(); c++)
+        LG_CHECK(blocks[c] < blocks[c + 1], pio::ERR_IO, "blocks.petsc: empty block");
+    }
+    if (pio::exists(dir + "/ordering.petsc")) {
+      const auto pi = as_ints(pio::read_vec(dir + "/ordering.petsc"), n0, "ordering.petsc");
+      perm_input.alloc(n0);
+      to_device(perm_input.p, pi.data(), n0, s);
+    }
+    H[0].blocks = blocks;
+    finish_setup();
     setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  // print_csv_header / print_csv (util.hpp:187-205, options.hpp:139-164):
+  // the reference's columns; options this build has no field for are
+  // printed with the reference's defaults.
+  static std::string csv_header() {
+    return "StrengthMetric,Redistribute,MaxLevels,MinSize,ElimDegree,AlwaysElim,TopElim,Tolerance,Threshold,"
+           "AggregationIterations,MaxIterations,RedistributionFactor,RedistributionBound,PreSmoothing,"
+           "PostSmoothing,ConstantCorrection,Smoother,NumberTVs,TVSmoothingIterations,KCycle,KCycleIterations,"
+           "OuterKSPType,KCycleKSPType,VotesToSeed,MaxAggSize,AggIterFactor,ChebyLower,ChebyUpper,AffScaling,"
+           "RandPerm,RejectionRatio,CRThreshold,CRTVs,CRIters,CRSmoothingIters,CRNorm,"
+           "graph,solver,iters,rel_error,cycle_complexity,wda,setup_time,solve_time,nnz,nodes\n";
+  }
+  std::string csv_line(const std::string &graph, const ligmg_info &info) const {
+    auto g = [](double x) {
+      char b[64];
+      snprintf(b, sizeof b, "%g", x);  // std::ostream's default for doubles
+      return std::string(b);
+    };
+    auto ksp = [](int t) { return std::string(t == LIGMG_KSP_FCG ? "fcg" : "cg"); };
+    static const char *aff[] = {"None", "combblas::Row", "Column", "RowColumn", "Sym", "Max"};
+    static const char *sm[] = {"affinity", "algebraic-distance", "none"};
+    std::string r;
+    r += std::string(sm[std::min(std::max(o.strength_metric, 0), 2)]) + ",1," + std::to_string(o.max_levels) + ",";
+    r += std::to_string(o.min_size) + "," + std::to_string(o.elim_degree) + "," +
+         std::to_string(o.always_elim ? 1 : 0) + "," + std::to_string(o.top_elim ? 1 : 0) + ",";
+    r += g(o.tol) + "," + g(o.thresh) + "," + std::to_string(o.agg_iters) + "," + std::to_string(o.iters) + ",";
+    r += g(10) + "," + g(10000) + "," + std::to_string(o.pre_smooth) + "," + std::to_string(o.post_smooth) + ",";
+    r += std::to_string(o.constant_correction ? 1 : 0) + "," +
+         (o.smoother_type == LIGMG_SMOOTHER_JACOBI ? "jacobi" : "chebyshev") + ",";
+    r += std::to_string(o.num_tvs) + "," + std::to_string(o.tvs_smooth) + "," + std::to_string(o.kcycle ? 1 : 0) +
+         "," + std::to_string(o.kcycle_iters) + ",";
+    r += ksp(o.outer_ksp_type) + "," + ksp(o.kcycle_ksp_type) + "," + std::to_string(o.votes_to_seed) + "," +
+         std::to_string(o.max_agg_size) + ",";
+    r += g(o.agg_iter_factor) + "," + g(o.cheby_lower) + "," + g(o.cheby_upper) + "," +
+         aff[std::min(std::max(o.aff_scaling, 0), 5)] + ",";
+    r += "1," + g(-1) + "," + g(0.1) + ",4,2,4,inf,";
+    r += graph + ",ligmg," + std::to_string(info.iters) + "," + g(info.rel_residual) + "," +
+         g(info.cycle_complexity) + "," + g(info.wda) + "," + g(setup_time) + "," + g(info.time) + ",";
+    r += std::to_string(H[0].A.nnz) + ",1\n";
+    return r;
   }
 
   // Solve layout (see DLevel::sperm): levels of at least SOLVE_PERM_MIN
@@ -1660,6 +1944,55 @@ int ligmg_create(int64_t n, const int32_t *rowptr, const int32_t *colind, const 
   }
 }
 
+int ligmg_load_hierarchy(const char *dir, const ligmg_options *opts, ligmg_handle *out) {
+  lg::g_last_error.clear();
+  if (!out || !dir) {
+    lg::g_last_error = "ligmg_load_hierarchy: invalid arguments";
+    return LIGMG_ERR_INVALID;
+  }
+  *out = nullptr;
+  try {
+    std::unique_ptr<ligmg_solver> h(new ligmg_solver());
+    if (opts) h->S.o = *opts;
+    else ligmg_options_default(&h->S.o);
+    int rc = check_options(h->S.o);
+    if (rc) {
+      lg::g_last_error = "ligmg_load_hierarchy: unsupported option value";
+      return rc;
+    }
+    h->S.load(dir);
+    *out = h.release();
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  } catch (const std::exception &e) {
+    return fail_unknown(e.what());
+  }
+}
+
+int ligmg_dump_hierarchy(ligmg_handle h, const char *dir) {
+  if (!h || !dir) return LIGMG_ERR_INVALID;
+  try {
+    h->S.dump(dir);
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  } catch (const std::exception &e) {
+    return fail_unknown(e.what());
+  }
+}
+
+int ligmg_dump_csv(ligmg_handle h, const char *graph, const ligmg_info *info, char *buf, int64_t cap) {
+  if (!h || !graph || !info || !buf || cap <= 0) return LIGMG_ERR_INVALID;
+  const std::string t = lg::Solver::csv_header() + h->S.csv_line(graph, *info);
+  if ((int64_t)t.size() + 1 > cap) {
+    lg::g_last_error = "ligmg_dump_csv: buffer too small (" + std::to_string(t.size() + 1) + " bytes needed)";
+    return LIGMG_ERR_INVALID;
+  }
+  memcpy(buf, t.c_str(), t.size() + 1);
+  return LIGMG_OK;
+}
+
 int ligmg_solve(ligmg_handle h, const double *b, double *x, int mem_kind, ligmg_info *info,
                 double *history, int32_t cap) {
   if (!h || !b || !x) return LIGMG_ERR_INVALID;
@@ -1727,38 +2060,11 @@ int ligmg_level_matrix(ligmg_handle h, int32_t l, int32_t *rp, int32_t *ci, doub
   if (!h || l < 0 || l >= (int)h->S.H.size()) return LIGMG_ERR_INVALID;
   try {
     const lg::DLevel &L = h->S.H[l];
-    hipStream_t s = h->S.s;
-    if (!L.sperm.p) {
-      lg::to_host(rp, L.A.rp.p, L.A.n + 1, s);
-      lg::to_host(ci, L.A.ci.p, L.A.nnz, s);
-      lg::to_host(v, L.A.v.p, L.A.nnz, s);
-      LG_HIP(hipStreamSynchronize(s));
-      return LIGMG_OK;
-    }
-    // solve layout: rows and columns back to setup order, each row sorted
-    const int64_t n = L.A.n;
-    auto prp = lg::download(L.A.rp.p, n + 1, s);
-    auto pci = lg::download(L.A.ci.p, L.A.nnz, s);
-    auto pv = lg::download(L.A.v.p, L.A.nnz, s);
-    auto sp = lg::download(L.sperm.p, n, s);
-    std::vector<int32_t> len(n);
-    for (int64_t i = 0; i < n; i++) len[sp[i]] = prp[i + 1] - prp[i];
-    rp[0] = 0;
-    for (int64_t i = 0; i < n; i++) rp[i + 1] = rp[i] + len[i];
-    std::vector<std::pair<int32_t, double>> row;
-    for (int64_t i = 0; i < n; i++) {
-      row.clear();
-      for (int32_t k = prp[i]; k < prp[i + 1]; k++) row.push_back({sp[pci[k]], pv[k]});
-      std::sort(row.begin(), row.end(),
-                [](const std::pair<int32_t, double> &a, const std::pair<int32_t, double> &b) {
-                  return a.first < b.first;
-                });
-      int32_t d = rp[sp[i]];
-      for (const auto &e : row) {
-        ci[d] = e.first;
-        v[d++] = e.second;
-      }
-    }
+    // the solve layout's copy back in setup order
+    const lg::pio::HostCsr A = h->S.to_setup(L.A, L.sperm.p, L.sperm.p, true);
+    memcpy(rp, A.rp.data(), A.rp.size() * sizeof(int32_t));
+    memcpy(ci, A.ci.data(), A.ci.size() * sizeof(int32_t));
+    memcpy(v, A.v.data(), A.v.size() * sizeof(double));
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return fail(e);

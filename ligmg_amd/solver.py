@@ -10,6 +10,7 @@ or torch tensors (host or ``cuda`` device memory; device tensors are read in
 place).  All compute runs in the HIP library; there is no CPU fallback.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -162,6 +163,37 @@ class Solver:
         t = C.c_double()
         self._lib.ligmg_setup_time(self._h, C.byref(t))
         self.setup_time = t.value
+
+    @classmethod
+    def load(cls, path, opts=None):
+        """A solver from a hierarchy written by dump_hierarchy (new): the
+        levels are read back, the solve-phase set-up runs for ``opts``."""
+        self = cls.__new__(cls)
+        self._lib = _ffi.solver_lib()
+        self.opts = opts if opts is not None else Options()
+        h = _ffi.HANDLE()
+        _check(self._lib.ligmg_load_hierarchy(os.fsencode(str(path)), C.byref(self.opts._c), C.byref(h)),
+               "ligmg_load_hierarchy")
+        self._h = h
+        self.comp_ptr = None
+        self.n = self.level_info(0)[1]
+        t = C.c_double()
+        self._lib.ligmg_setup_time(self._h, C.byref(t))
+        self.setup_time = t.value
+        return self
+
+    def dump_hierarchy(self, path):
+        """Solver::dump_hierarchy (ligmg.hpp:47-52): one directory of PETSc
+        binary files, <level>_A.petsc, _P, _R, _Q / _strength ..."""
+        _check(self._lib.ligmg_dump_hierarchy(self._h, os.fsencode(str(path))), "ligmg_dump_hierarchy")
+
+    def dump_csv(self, info, graph=""):
+        """Solver::dump_csv (ligmg.hpp:42-45): the reference's CSV header
+        and result line for the solve ``info`` (a ConvergenceInfo)."""
+        buf = C.create_string_buffer(1 << 14)
+        _check(self._lib.ligmg_dump_csv(self._h, graph.encode(), C.byref(info._info), buf, len(buf)),
+               "ligmg_dump_csv")
+        return buf.value.decode()
 
     def close(self):
         if getattr(self, "_h", None):

@@ -14,7 +14,7 @@
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: example matrix.mtx\n");
+    std::fprintf(stderr, "usage: example matrix.mtx [dump-dir]\n");
     return 2;
   }
   std::ifstream in(argv[1]);
@@ -70,5 +70,15 @@ int main(int argc, char **argv) {
     bb += rhs[i] * rhs[i];
   }
   std::printf("full residual %.3e\n", std::sqrt(rr / bb));
+  // --csv and --dump of the reference's driver (main.cpp:119-127)
+  solver.dump_csv(std::cout, result, "my_matrix");
+  if (argc > 2) {
+    solver.dump_hierarchy(argv[2]);
+    ligmg::Solver again = ligmg::Solver::load(argv[2]);
+    auto r2 = again.solve(rhs);
+    bool same = r2.iters() == result.iters() && r2.solution == result.solution;
+    std::printf("reloaded hierarchy: %s\n", same ? "same solution" : "DIFFERENT solution");
+    if (!same) return 1;
+  }
   return (result.converged() && std::sqrt(rr / bb) < 1e-7) ? 0 : 1;
 }

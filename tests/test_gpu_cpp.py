@@ -18,8 +18,10 @@ def test_cpp_example_on_reference_fixture(tmp_path):
     subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "cpp", "example.cpp"), "-o", str(exe),
                     f"-L{libdir}", "-lligmg_amd", f"-Wl,-rpath,{libdir}"], check=True)
-    p = subprocess.run([str(exe), os.path.join(ROOT, "tests", "golden", "my_matrix.mtx")],
+    p = subprocess.run([str(exe), os.path.join(ROOT, "tests", "golden", "my_matrix.mtx"), str(tmp_path / "dump")],
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     print(p.stdout)
     assert p.returncode == 0, p.stdout
     assert "full residual" in p.stdout
+    assert "reloaded hierarchy: same solution" in p.stdout
+    assert "StrengthMetric,Redistribute" in p.stdout and ",my_matrix,ligmg," in p.stdout
