@@ -101,7 +101,7 @@ typedef struct ligmg_options {
   int32_t kcycle_ksp_type;      /* LIGMG_KSP_FCG                             */
   int32_t kcycle_iters;         /* 2                                         */
   int64_t votes_to_seed;        /* 1                                         */
-  int64_t max_agg_size;         /* -1 (only -1 supported)                    */
+  int64_t max_agg_size;         /* -1: no aggregate-size rejection           */
   double agg_iter_factor;       /* 0.5                                       */
   double agg_init_factor;       /* 0.5                                       */
   double cheby_lower;           /* 0.3                                       */
@@ -109,7 +109,7 @@ typedef struct ligmg_options {
   int32_t aff_scaling;          /* LIGMG_AFF_MAX                             */
   double elim_factor;           /* 0.9                                       */
   int32_t elim_iters;           /* 1 (only 1 supported)                      */
-  int32_t neighbor_of_neighbor; /* false (only false supported)              */
+  int32_t neighbor_of_neighbor; /* false                                     */
   /* (new) */
   uint64_t seed;                /* 0: seed of test vectors / eigen estimate  */
   int32_t cheby_eig_iters;      /* 10: CG-Lanczos steps for lambda_max       */
@@ -128,6 +128,12 @@ typedef struct ligmg_options {
                                    (with every coarser level) run as one
                                    single-workgroup launch per coarse-grid
                                    correction, same results; 0: off         */
+  double rejection_ratio;       /* -1: conductance rejection off; >= 0: in the
+                                   second half of the voting rounds, joined
+                                   vertices whose external / internal edge
+                                   weight exceeds it leave their aggregate  */
+  int32_t rejection_to_seed;    /* false: a rejected vertex becomes undecided;
+                                   true: a seed                             */
 } ligmg_options;
 
 /* ConvergenceInfo (util.hpp:143-170) plus the fields the reference only

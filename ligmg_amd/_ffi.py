@@ -33,6 +33,8 @@ class ligmg_options(C.Structure):
         ("ordering", i32),
         ("solve_layout", i32),
         ("tail_nnz", i32),
+        ("rejection_ratio", f64),
+        ("rejection_to_seed", i32),
     ]
 
 

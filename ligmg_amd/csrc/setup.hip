@@ -688,6 +688,7 @@ struct OpVote {
   const int32_t *ci, *state, *ind;
   const double *S;
   double thresh;
+  int nn;  // neighbor_of_neighbor: a SeedNeighbor attracts like its seed
   int8_t *vst;
   int32_t *vind;
   double *vw;
@@ -712,7 +713,7 @@ struct OpVote {
     const int j = ci[k];
     const int st = state[j];
     int p;
-    if (st == ST_SEED) p = (s > thresh) ? PRIO_SEED : PRIO_DECIDED;
+    if (st == ST_SEED || (nn && st == ST_SEEDNEIGHBOR)) p = (s > thresh) ? PRIO_SEED : PRIO_DECIDED;
     else if (st == ST_UNDECIDED) p = PRIO_UNDECIDED;
     else p = PRIO_DECIDED;
     Acc m = {p, k, ind[j], s};
@@ -765,10 +766,11 @@ __global__ void k_add_votes(int n, const double *nv, double *votes) {
 // ---- a13: mark_nodes (aggregation.hpp:702-750) -----------------------------
 __global__ void k_mark(int n, int first, const int32_t *state, const int32_t *ind, const int8_t *vst,
                        const int32_t *vind, const int32_t *vf, const double *votes,
-                       const double *cs, double votes_to_seed, double max_agg, int32_t *nstate,
+                       const double *cs, double votes_to_seed, double max_agg, int nn, int32_t *nstate,
                        int32_t *nind) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int joined = nn ? ST_SEEDNEIGHBOR : ST_DECIDED;
   int st = state[i], id = ind[i];
   if (st == ST_DECIDED || st == ST_DECIDEDROOT || st == ST_SEEDNEIGHBOR) {
   } else if (st == ST_SEED) {
@@ -778,11 +780,11 @@ __global__ void k_mark(int n, int first, const int32_t *state, const int32_t *in
       st = ST_SEED;
       id = i;
     } else {
-      st = ST_DECIDED;
+      st = joined;
       id = vf[i];
     }
   } else if (vst[i] == PRIO_SEED) {
-    st = ST_DECIDED;
+    st = joined;
     id = vind[i];
   } else if (votes[i] >= votes_to_seed * cs[i]) {
     st = ST_SEED;
@@ -790,6 +792,67 @@ __global__ void k_mark(int n, int first, const int32_t *state, const int32_t *in
   }
   nstate[i] = st;
   nind[i] = id;
+}
+
+__device__ __forceinline__ bool aggregated(int st) {
+  return st == ST_SEED || st == ST_DECIDED || st == ST_DECIDEDROOT || st == ST_SEEDNEIGHBOR;
+}
+
+// ---- reject_aggregates (aggregation.hpp:622-670).  An aggregate's size
+// counts its root (a Seed or DecidedRoot) and its members, so the
+// reference's |members|^2 / (size - 1) first members are all of them: every
+// member of an aggregate above max_size returns to {-1, Undecided} and the
+// root becomes DecidedRoot.
+__global__ void k_agg_sizes(int n, const int32_t *state, const int32_t *ind, int32_t *sizes) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && aggregated(state[i])) atomicAdd(&sizes[ind[i]], 1);
+}
+__global__ void k_reject_large(int n, const int32_t *sizes, int64_t max_size, int32_t *state, int32_t *ind) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int st = state[i];
+  if ((st == ST_DECIDED || st == ST_SEEDNEIGHBOR) && sizes[ind[i]] > max_size) {
+    state[i] = ST_UNDECIDED;
+    ind[i] = -1;
+  } else if ((st == ST_SEED || st == ST_DECIDEDROOT) && ind[i] == i && sizes[i] > max_size) {
+    state[i] = ST_DECIDEDROOT;
+  }
+}
+
+// ---- ext_int_ratio (aggregation.hpp:563-620): weight of an aggregated
+// vertex's off-diagonal edges leaving (ext) / staying in (!ext) its
+// aggregate, canonical row sums with the other entries +0
+struct OpEdgeSplit : DblSum {
+  const int32_t *ci, *state, *ind;
+  const double *v;
+  int ext;
+  double *out;
+  __device__ void add(double &acc, int row, int k) const {
+    const int j = ci[k];
+    double t = 0.0;
+    if (aggregated(state[row]) && j != row) {
+      const bool inside = aggregated(state[j]) && ind[j] == ind[row];
+      if (inside != (ext != 0)) t = -v[k];
+    }
+    acc += t;
+  }
+  __device__ void finish(int row, double acc) const { out[row] = acc; }
+};
+// conductance_rejection (aggregation.hpp:752-776)
+__global__ void k_conductance(int n, const double *ext, const double *in, double ratio, int to_seed,
+                              int32_t *state, int32_t *ind) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int st = state[i];
+  if (ext[i] / in[i] > ratio && (st == ST_DECIDED || st == ST_SEEDNEIGHBOR)) {
+    state[i] = to_seed ? ST_SEED : ST_UNDECIDED;
+    ind[i] = i;
+  }
+}
+// a vertex a rejection left undecided with no target is its own aggregate
+__global__ void k_self_unassigned(int n, int32_t *ind) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && ind[i] < 0) ind[i] = i;
 }
 
 // ---- a14: R with zero rows removed -> coarse id = rank of the root
@@ -1050,6 +1113,27 @@ void make_elim32(DLevel &L, hipStream_t s) {
 // ----------------------------------------------------------------------------
 // Strength of connection (affinity_matrix + col_sum)
 // ----------------------------------------------------------------------------
+namespace {
+__global__ void k_div_rows(int n, const int32_t *rp, const double *rs, double *v) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const double d = rs[row];
+  for (int k = rp[row]; k < rp[row + 1]; k++) v[k] = d > 0 ? v[k] / d : 0.0;
+}
+__global__ void k_div_cols(int64_t nnz, const int32_t *ci, const double *cs, double *v) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+    const double d = cs[ci[k]];
+    v[k] = d > 0 ? v[k] / d : 0.0;
+  }
+}
+// (S^T + S) / 2 on a structurally symmetric pattern; tpos[k] is the
+// position of the transposed entry
+__global__ void k_sym_half(int64_t nnz, const int32_t *tpos, const double *v, double *out) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x)
+    out[k] = (v[tpos[k]] + v[k]) / 2;
+}
+}  // namespace
+
 void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevBuf<double> &S,
                       DevBuf<double> &cs, hipStream_t s) {
   const int64_t n = A.n, nnz = A.nnz;
@@ -1085,7 +1169,29 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
   to_host(&hb, bad.p, 1, s);
   LG_HIP(hipStreamSynchronize(s));
   LG_CHECK(hb == 0, LIGMG_ERR_INVALID, "matrix pattern is not structurally symmetric");
-  run_rows(n, A.rp.p, OpColMax{{}, tpos.p, raw.p, colmax.p}, Hr, s);
+  // Row / Column / RowColumn / Sym scaling (aggregation.hpp:350-366): row
+  // sums, then column sums (ascending row), then (S^T + S) / 2; a zero sum
+  // leaves its (zero) entries at 0
+  const int sc = o.aff_scaling;
+  if (sc == LIGMG_AFF_ROW || sc == LIGMG_AFF_ROWCOLUMN || sc == LIGMG_AFF_SYM) {
+    DevBuf<double> rs(n);
+    run_rows(n, A.rp.p, OpRowSum{{}, raw.p, rs.p}, Hr, s);
+    hipLaunchKernelGGL(k_div_rows, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, rs.p, raw.p);
+    LG_LAUNCH_CHECK();
+  }
+  if (sc == LIGMG_AFF_COLUMN || sc == LIGMG_AFF_ROWCOLUMN) {
+    DevBuf<double> csum(n);
+    run_rows(n, A.rp.p, OpColSum{{}, tpos.p, raw.p, csum.p}, Hr, s);
+    hipLaunchKernelGGL(k_div_cols, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, A.ci.p, csum.p, raw.p);
+    LG_LAUNCH_CHECK();
+  }
+  if (sc == LIGMG_AFF_SYM) {
+    DevBuf<double> sym(std::max<int64_t>(nnz, 1));
+    hipLaunchKernelGGL(k_sym_half, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, tpos.p, raw.p, sym.p);
+    LG_LAUNCH_CHECK();
+    raw = std::move(sym);
+  }
+  if (sc == LIGMG_AFF_MAX) run_rows(n, A.rp.p, OpColMax{{}, tpos.p, raw.p, colmax.p}, Hr, s);
   S.alloc(nnz);
   run_rows(n, A.rp.p, OpScale{{}, A.ci.p, raw.p, rowmax.p, colmax.p, o.thresh, o.aff_scaling, S.p}, Hr,
            s);
@@ -1113,7 +1219,8 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
   const int kb = std::max(1, bits_for((uint64_t)n));
   for (int iter = 0; iter < o.agg_iters; iter++) {
     const double th = o.agg_init_factor * std::pow(o.agg_iter_factor, iter);
-    run_rows(n, A.rp.p, OpVote{A.ci.p, state.p, ind.p, S, th, vst.p, vind.p, vw.p}, Hr, s);
+    run_rows(n, A.rp.p, OpVote{A.ci.p, state.p, ind.p, S, th, o.neighbor_of_neighbor ? 1 : 0, vst.p, vind.p, vw.p},
+             Hr, s);
     hipLaunchKernelGGL(k_voters, dim3(g1(n)), dim3(256), 0, s, (int)n, state.p, vind.p, vw.p, th, flag.p,
                        vf.p);
     LG_LAUNCH_CHECK();
@@ -1136,11 +1243,32 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
     LG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_mark, dim3(g1(n)), dim3(256), 0, s, (int)n, iter == 0 ? 1 : 0, state.p, ind.p,
                        vst.p, vind.p, vf.p, votes.p, cs, (double)o.votes_to_seed,
-                       (double)o.max_agg_size, nstate.p, nind.p);
+                       (double)o.max_agg_size, o.neighbor_of_neighbor ? 1 : 0, nstate.p, nind.p);
     LG_LAUNCH_CHECK();
     std::swap(state.p, nstate.p);
     std::swap(ind.p, nind.p);
+    if (o.max_agg_size > 0) {
+      DevBuf<int32_t> sizes(n);
+      fill_i32(sizes.p, 0, n, s);
+      hipLaunchKernelGGL(k_agg_sizes, dim3(g1(n)), dim3(256), 0, s, (int)n, state.p, ind.p, sizes.p);
+      LG_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_reject_large, dim3(g1(n)), dim3(256), 0, s, (int)n, sizes.p, (int64_t)o.max_agg_size,
+                         state.p, ind.p);
+      LG_LAUNCH_CHECK();
+    }
+    if (iter >= o.agg_iters / 2 && o.rejection_ratio >= 0) {
+      LG_CHECK(A.v.p, LIGMG_ERR_UNSUPPORTED, "conductance rejection needs the level matrix values");
+      DevBuf<double> ext(n), in(n);
+      run_rows(n, A.rp.p, OpEdgeSplit{{}, A.ci.p, state.p, ind.p, A.v.p, 1, ext.p}, Hr, s);
+      run_rows(n, A.rp.p, OpEdgeSplit{{}, A.ci.p, state.p, ind.p, A.v.p, 0, in.p}, Hr, s);
+      hipLaunchKernelGGL(k_conductance, dim3(g1(n)), dim3(256), 0, s, (int)n, ext.p, in.p, o.rejection_ratio,
+                         o.rejection_to_seed ? 1 : 0, state.p, ind.p);
+      LG_LAUNCH_CHECK();
+      LG_HIP(hipStreamSynchronize(s));
+    }
   }
+  hipLaunchKernelGGL(k_self_unassigned, dim3(g1(n)), dim3(256), 0, s, (int)n, ind.p);
+  LG_LAUNCH_CHECK();
   DevBuf<int32_t> root(n), cid(n);
   fill_i32(root.p, 0, n, s);
   hipLaunchKernelGGL(k_mark_roots, dim3(g1(n)), dim3(256), 0, s, (int)n, ind.p, root.p);
@@ -1314,6 +1442,11 @@ double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_
 // ----------------------------------------------------------------------------
 std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,
                                     const std::vector<int32_t> &ci, const std::vector<double> &v) {
+  // the hierarchy stops at min_size entries, so only coarsening that stalls
+  // until max_levels leaves a larger coarse level
+  LG_CHECK(n <= COARSE_DENSE_MAX, LIGMG_ERR_UNSUPPORTED,
+           "the coarsest level has " + std::to_string(n) + " vertices; the dense coarse solve takes at most " +
+               std::to_string(COARSE_DENSE_MAX) + " (coarsening stalled: check the aggregation options)");
   std::vector<int32_t> comp(n, -1);
   std::vector<int64_t> csize;
   std::vector<int64_t> stack;

@@ -64,6 +64,7 @@ void make_elim32(DLevel &L, hipStream_t s);
 // perm (may be null): setup position of each row, for the start vector
 double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_idx, int iters,
                     const int32_t *perm, hipStream_t s);
+constexpr int64_t COARSE_DENSE_MAX = 8192;
 std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,
                                     const std::vector<int32_t> &ci, const std::vector<double> &v);
 

@@ -1892,16 +1892,16 @@ void ligmg_options_default(ligmg_options *o) {
   o->ordering = LIGMG_ORDER_NATURAL;
   o->solve_layout = 1;
   o->tail_nnz = 4096;
+  o->rejection_ratio = -1;
+  o->rejection_to_seed = 0;
 }
 
 const char *ligmg_last_error(void) { return lg::g_last_error.c_str(); }
 
 static int check_options(const ligmg_options &o) {
-  if (o.max_agg_size != -1) return LIGMG_ERR_UNSUPPORTED;
   if (o.elim_iters != 1) return LIGMG_ERR_UNSUPPORTED;
-  if (o.neighbor_of_neighbor) return LIGMG_ERR_UNSUPPORTED;
   if (o.num_tvs < 1 || o.num_tvs > 16) return LIGMG_ERR_UNSUPPORTED;
-  if (o.aff_scaling != LIGMG_AFF_MAX && o.aff_scaling != LIGMG_AFF_NONE) return LIGMG_ERR_UNSUPPORTED;
+  if (o.aff_scaling < LIGMG_AFF_NONE || o.aff_scaling > LIGMG_AFF_MAX) return LIGMG_ERR_INVALID;
   if (o.pre_smooth < 0 || o.post_smooth < 0 || o.iters < 0 || o.kcycle_iters < 1 || o.tail_nnz < 0)
     return LIGMG_ERR_INVALID;
   return LIGMG_OK;
@@ -2132,6 +2132,12 @@ int ligmg_aggregate_with_strength(int64_t n, const int32_t *rowptr, const int32_
     ligmg_options o;
     if (opts) o = *opts;
     else ligmg_options_default(&o);
+    // conductance rejection weighs the Laplacian's edges, which this entry
+    // point does not receive
+    if (o.rejection_ratio >= 0) {
+      lg::g_last_error = "ligmg_aggregate_with_strength: rejection_ratio needs the level matrix";
+      return LIGMG_ERR_UNSUPPORTED;
+    }
     LG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     lg::DCsr A;
     A.n = n;

@@ -371,6 +371,27 @@ double affinity_raw(int metric, const double *X, int T, const double *nrm, int64
   return a_rc < 0 ? -a_rc : 0;
 }
 
+// position of entry (c, r) in row c (the pattern is structurally symmetric)
+int32_t transpose_pos(const Csr &A, int64_t r, int32_t c) {
+  const int32_t *b = A.ci.data() + A.rp[c], *e = A.ci.data() + A.rp[c + 1];
+  const int32_t *p = std::lower_bound(b, e, (int32_t)r);
+  if (p == e || *p != r) throw std::runtime_error("pattern not symmetric");
+  return (int32_t)(p - A.ci.data());
+}
+
+// column sums of values on A's pattern in ascending row order: column c's
+// entries are V(r, c) for r in row c's columns
+void column_sums(const Csr &A, const std::vector<double> &V, std::vector<double> &out) {
+  out.assign(A.n, 0.0);
+  std::vector<double> buf;
+  for (int64_t c = 0; c < A.n; c++) {
+    const int32_t len = A.rp[c + 1] - A.rp[c];
+    buf.assign(len, 0.0);
+    for (int32_t k = A.rp[c]; k < A.rp[c + 1]; k++) buf[k - A.rp[c]] = V[transpose_pos(A, c, A.ci[k])];
+    out[c] = canon_sum(buf.data(), len);
+  }
+}
+
 void strength(const Csr &A, double *X, int T, int sweeps, int metric, int scaling, double thresh,
               std::vector<double> &S, std::vector<double> &col_sum) {
   if (metric != LIGMG_STRENGTH_NONE) tv_jacobi(A, X, T, sweeps);
@@ -387,6 +408,27 @@ void strength(const Csr &A, double *X, int T, int sweeps, int metric, int scalin
       rowmax[r] = std::max(rowmax[r], raw[k]);
       colmax[A.ci[k]] = std::max(colmax[A.ci[k]], raw[k]);
     }
+  // Row / Column / RowColumn / Sym scaling (aggregation.hpp:350-366): row
+  // sums, then column sums (ascending row), then (S^T + S) / 2.  A zero sum
+  // leaves its (zero) entries at 0.
+  if (scaling == LIGMG_AFF_ROW || scaling == LIGMG_AFF_ROWCOLUMN || scaling == LIGMG_AFF_SYM) {
+    for (int64_t r = 0; r < A.n; r++) {
+      const double rs = canon_sum(raw.data() + A.rp[r], A.rp[r + 1] - A.rp[r]);
+      for (int32_t k = A.rp[r]; k < A.rp[r + 1]; k++) raw[k] = rs > 0 ? raw[k] / rs : 0.0;
+    }
+  }
+  if (scaling == LIGMG_AFF_COLUMN || scaling == LIGMG_AFF_ROWCOLUMN) {
+    std::vector<double> cs;
+    column_sums(A, raw, cs);
+    for (int64_t r = 0; r < A.n; r++)
+      for (int32_t k = A.rp[r]; k < A.rp[r + 1]; k++) raw[k] = cs[A.ci[k]] > 0 ? raw[k] / cs[A.ci[k]] : 0.0;
+  }
+  if (scaling == LIGMG_AFF_SYM) {
+    std::vector<double> sym(raw.size());
+    for (int64_t r = 0; r < A.n; r++)
+      for (int32_t k = A.rp[r]; k < A.rp[r + 1]; k++) sym[k] = (raw[transpose_pos(A, r, A.ci[k])] + raw[k]) / 2;
+    raw.swap(sym);
+  }
   S.assign(A.nnz(), 0.0);
   for (int64_t r = 0; r < A.n; r++)
     for (int32_t k = A.rp[r]; k < A.rp[r + 1]; k++) {
@@ -397,23 +439,7 @@ void strength(const Csr &A, double *X, int T, int sweeps, int metric, int scalin
       }
       S[k] = (s < thresh || s != s) ? 0.0 : s;  // Prune (aggregation.hpp:401)
     }
-  // column sums in ascending row order through the symmetric pattern:
-  // column c's entries are S(r, c) for r in row c's columns.
-  col_sum.assign(A.n, 0.0);
-  std::vector<double> buf;
-  for (int64_t c = 0; c < A.n; c++) {
-    int32_t len = A.rp[c + 1] - A.rp[c];
-    buf.assign(len, 0.0);
-    for (int32_t k = A.rp[c]; k < A.rp[c + 1]; k++) {
-      int32_t r = A.ci[k];
-      // position of c in row r
-      const int32_t *b = A.ci.data() + A.rp[r], *e = A.ci.data() + A.rp[r + 1];
-      const int32_t *p = std::lower_bound(b, e, (int32_t)c);
-      if (p == e || *p != c) throw std::runtime_error("pattern not symmetric");
-      buf[k - A.rp[c]] = S[p - A.ci.data()];
-    }
-    col_sum[c] = canon_sum(buf.data(), len);
-  }
+  column_sums(A, S, col_sum);
 }
 
 // ---- a11: AggSR / VoteAgg / vote (aggregation.hpp:406-518) ---------------
@@ -433,11 +459,68 @@ Vote vadd(const Vote &a, const Vote &b) {
   if (a.st == b.st) return (a.w > b.w) ? a : b;
   return gt(a.st, b.st) ? a : b;
 }
-Vote vmul(int state, int32_t ind, double w, double thresh) {
-  if (state == ST_SEED) return {w > thresh ? V_SEED : V_DECIDED, ind, w};
+Vote vmul(int state, int32_t ind, double w, double thresh, bool nn) {
+  if (state == ST_SEED || (nn && state == ST_SEEDNEIGHBOR)) return {w > thresh ? V_SEED : V_DECIDED, ind, w};
   if (state == ST_DECIDED || state == ST_DECIDEDROOT) return {V_DECIDED, ind, w};
   if (state == ST_UNDECIDED) return {V_UNDECIDED, ind, w};
   return {V_DECIDED, ind, w};
+}
+
+bool aggregated(int st) {
+  return st == ST_SEED || st == ST_DECIDED || st == ST_DECIDEDROOT || st == ST_SEEDNEIGHBOR;
+}
+
+// reject_aggregates (aggregation.hpp:622-670): an aggregate larger than
+// max_size loses its first |members|^2 / (size - 1) members (Decided or
+// SeedNeighbor, ascending index), which return to the default AggState
+// {-1, Undecided}; its root becomes DecidedRoot.
+void reject_aggregates(std::vector<int32_t> &state, std::vector<int32_t> &ind, int64_t max_size) {
+  const int64_t n = (int64_t)state.size();
+  std::vector<int64_t> sizes(n, 0);
+  for (int64_t i = 0; i < n; i++)
+    if (aggregated(state[i])) sizes[ind[i]]++;
+  std::vector<std::vector<int32_t>> members(n);
+  for (int64_t i = 0; i < n; i++)
+    if (state[i] == ST_DECIDED || state[i] == ST_SEEDNEIGHBOR) members[ind[i]].push_back((int32_t)i);
+  for (int64_t r = 0; r < n; r++) {
+    const auto &c = members[r];
+    if (c.empty() || sizes[r] <= max_size) continue;
+    const int64_t cnt = (int64_t)c.size() * (int64_t)c.size() / (sizes[r] - 1);
+    for (int64_t t = 0; t < cnt; t++) {
+      state[c[t]] = ST_UNDECIDED;
+      ind[c[t]] = -1;
+    }
+    state[r] = ST_DECIDEDROOT;
+  }
+}
+
+// ext_int_ratio + conductance_rejection (aggregation.hpp:563-620, 752-776):
+// per aggregated vertex, the weight of its off-diagonal edges leaving its
+// aggregate over the weight staying inside (canonical sums); Decided and
+// SeedNeighbor vertices above rejection_ratio become {i, Undecided} (or
+// {i, Seed} with rejection_to_seed).
+void conductance_rejection(const Csr &A, std::vector<int32_t> &state, std::vector<int32_t> &ind,
+                           const ligmg_options &o) {
+  const int64_t n = A.n;
+  std::vector<double> ratio(n), ext, in;
+  for (int64_t i = 0; i < n; i++) {
+    const int32_t len = A.rp[i + 1] - A.rp[i];
+    ext.assign(len, 0.0);
+    in.assign(len, 0.0);
+    if (aggregated(state[i]))
+      for (int32_t k = A.rp[i]; k < A.rp[i + 1]; k++) {
+        const int32_t j = A.ci[k];
+        if (j == i) continue;
+        if (aggregated(state[j]) && ind[j] == ind[i]) in[k - A.rp[i]] = -A.v[k];
+        else ext[k - A.rp[i]] = -A.v[k];
+      }
+    ratio[i] = canon_sum(ext.data(), len) / canon_sum(in.data(), len);
+  }
+  for (int64_t i = 0; i < n; i++)
+    if (ratio[i] > o.rejection_ratio && (state[i] == ST_DECIDED || state[i] == ST_SEEDNEIGHBOR)) {
+      state[i] = o.rejection_to_seed ? ST_SEED : ST_UNDECIDED;
+      ind[i] = (int32_t)i;
+    }
 }
 
 // ---- a12-a14: tally_votes, mark_nodes, aggregation driver
@@ -460,7 +543,7 @@ int64_t aggregate(const Csr &A, const std::vector<double> &S, const std::vector<
       for (int32_t k = A.rp[i]; k < A.rp[i + 1]; k++) {
         if (S[k] == 0.0) continue;  // pruned entry: not in the strength matrix
         int32_t j = A.ci[k];
-        y = vadd(y, vmul(state[j], ind[j], S[k], th));
+        y = vadd(y, vmul(state[j], ind[j], S[k], th, o.neighbor_of_neighbor != 0));
       }
       va[i] = y;
     }
@@ -477,6 +560,7 @@ int64_t aggregate(const Csr &A, const std::vector<double> &S, const std::vector<
       votes[t] = nv + votes[t];
     }
     std::vector<int32_t> ns = state, ni = ind;
+    const int joined = o.neighbor_of_neighbor ? ST_SEEDNEIGHBOR : ST_DECIDED;
     for (int64_t i = 0; i < n; i++) {
       int st = state[i];
       if (st == ST_DECIDED || st == ST_DECIDEDROOT || st == ST_SEEDNEIGHBOR) {
@@ -487,11 +571,11 @@ int64_t aggregate(const Csr &A, const std::vector<double> &S, const std::vector<
           ns[i] = ST_SEED;
           ni[i] = (int32_t)i;
         } else {
-          ns[i] = ST_DECIDED;
+          ns[i] = joined;
           ni[i] = vf[i];
         }
       } else if (va[i].st == V_SEED) {
-        ns[i] = ST_DECIDED;
+        ns[i] = joined;
         ni[i] = va[i].ind;
       } else if (votes[i] >= (double)o.votes_to_seed * col_sum[i]) {
         ns[i] = ST_SEED;
@@ -500,7 +584,14 @@ int64_t aggregate(const Csr &A, const std::vector<double> &S, const std::vector<
     }
     state.swap(ns);
     ind.swap(ni);
+    if (o.max_agg_size > 0) reject_aggregates(state, ind, o.max_agg_size);
+    if (iter >= o.agg_iters / 2 && o.rejection_ratio >= 0) conductance_rejection(A, state, ind, o);
   }
+  // an unaggregated vertex is its own aggregate; a vertex a rejection left
+  // undecided with no target (ind -1, which the reference's
+  // piecewise_constant_restriction would use as a row index) is one too
+  for (int64_t i = 0; i < n; i++)
+    if (ind[i] < 0) ind[i] = (int32_t)i;
   // piecewise_constant_restriction + remove_zero_rows (aggregation.hpp:950-951)
   std::vector<int32_t> root(n, 0);
   for (int64_t i = 0; i < n; i++) root[ind[i]] = 1;
@@ -641,8 +732,16 @@ double eig_estimate(const Csr &A, const std::vector<double> &dinv, uint64_t seed
 // ---- a20: coarse level as a dense pseudo-inverse ---------------------------
 // For each connected component c of the coarse graph,
 //   L_c^+ = (L_c + 1 1^T / n_c)^-1 - 1 1^T / n_c.
+// largest coarse level the dense solve takes: the hierarchy stops at
+// min_size entries, so only coarsening that stalls until max_levels exceeds it
+constexpr int64_t COARSE_DENSE_MAX = 8192;
+struct CoarseTooLarge : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 std::vector<double> dense_pinv(const Csr &A) {
   const int64_t n = A.n;
+  if (n > COARSE_DENSE_MAX) throw CoarseTooLarge("coarse level too large for the dense solve");
   std::vector<int32_t> comp(n, -1);
   std::vector<int64_t> csize;
   for (int64_t s = 0; s < n; s++) {
@@ -1062,9 +1161,8 @@ int solve(const Solver &S, const double *bin, double *xout, ligmg_info *info, do
 }
 
 bool check_opts(const ligmg_options *o) {
-  return o->max_agg_size == -1 && o->elim_iters == 1 && !o->neighbor_of_neighbor &&
-         o->num_tvs >= 1 && o->num_tvs <= 16 &&
-         (o->aff_scaling == LIGMG_AFF_MAX || o->aff_scaling == LIGMG_AFF_NONE);
+  return o->elim_iters == 1 && o->num_tvs >= 1 && o->num_tvs <= 16 &&
+         o->aff_scaling >= LIGMG_AFF_NONE && o->aff_scaling <= LIGMG_AFF_MAX;
 }
 
 }  // namespace
@@ -1116,6 +1214,8 @@ void oracle_default_options(ligmg_options *o) {
   o->ordering = LIGMG_ORDER_NATURAL;
   o->solve_layout = 1;
   o->tail_nnz = 4096;
+  o->rejection_ratio = -1;
+  o->rejection_to_seed = 0;
 }
 
 int oracle_create(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
@@ -1141,6 +1241,8 @@ int oracle_create(int64_t n, const int32_t *rp, const int32_t *ci, const double 
     S->setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     *out = S;
     return 0;
+  } catch (const CoarseTooLarge &) {
+    return LIGMG_ERR_UNSUPPORTED;
   } catch (...) {
     return LIGMG_ERR_NUMERIC;
   }
@@ -1277,6 +1379,7 @@ int oracle_strength(int64_t n, const int32_t *rp, const int32_t *ci, const doubl
 int oracle_aggregate_with_strength(int64_t n, const int32_t *rp, const int32_t *ci,
                                    const double *S, const ligmg_options *opts, int32_t *agg,
                                    int32_t *ind, int32_t *state, int64_t *n_coarse) {
+  if (opts->rejection_ratio >= 0) return LIGMG_ERR_UNSUPPORTED;  // needs the Laplacian's values
   try {
     Csr A = make_csr(n, rp, ci, S);
     std::vector<double> s(S + rp[0], S + rp[n]), cs(n);

@@ -57,6 +57,66 @@ def test_hierarchy_bit_exact(case, ordering, my_matrix):
     d = ligmg_amd.Solver(g, do)
     r = O.OracleSolver(g, oo)
     assert np.array_equal(d.ordering(), r.ordering())
+    assert_same_hierarchy(d, r)
+
+
+@pytest.mark.parametrize("scaling", [_ffi.AFF_NONE, _ffi.AFF_ROW, _ffi.AFF_COLUMN, _ffi.AFF_ROWCOLUMN,
+                                     _ffi.AFF_SYM])
+@pytest.mark.parametrize("case", ["rmat12_pend", "grid_w", "ba"])
+def test_hierarchy_bit_exact_aff_scalings(case, scaling, my_matrix):
+    """The other affinity scalings of aggregation.hpp:350-366 (row, column,
+    row then column, symmetrised row): strengths, aggregates and coarse
+    matrices bit-exact, and the hierarchy solves."""
+    g = make_case(case, my_matrix)
+    print(f"case={case} aff_scaling={scaling}", flush=True)
+    do, oo = opts_pair(aff_scaling=scaling)
+    d = ligmg_amd.Solver(g, do)
+    assert_same_hierarchy(d, O.OracleSolver(g, oo))
+    info = d.solve(graphs.rhs(g.n, 1, 0))
+    assert info.rel_residual() <= 1e-8
+
+
+AGG_VARIANTS = {
+    "neighbor_of_neighbor": {"neighbor_of_neighbor": 1},
+    "max_agg_size_3": {"max_agg_size": 3},
+    "rejection_0.8": {"rejection_ratio": 0.8},
+    "rejection_to_seed": {"rejection_ratio": 0.8, "rejection_to_seed": 1},
+    "all_three": {"neighbor_of_neighbor": 1, "max_agg_size": 4, "rejection_ratio": 1.0},
+}
+
+
+# conductance rejection at these ratios rejects nearly every member on the
+# unit-weight BA graph (ext/int ~ 4-9), so its coarsening stalls there
+VARIANT_CASES = [(c, v) for c in ["rmat12_pend", "grid_w", "ba"] for v in AGG_VARIANTS
+                 if c != "ba" or "rejection_ratio" not in AGG_VARIANTS[v]]
+
+
+@pytest.mark.parametrize("case,variant", VARIANT_CASES)
+def test_hierarchy_bit_exact_aggregation_variants(case, variant, my_matrix):
+    """Neighbour-of-neighbour voting, aggregate-size rejection and
+    conductance rejection (aggregation.hpp:452-466, 622-670, 752-776):
+    states, aggregates and coarse matrices bit-exact, and the hierarchy
+    solves."""
+    g = make_case(case, my_matrix)
+    print(f"case={case} variant={variant}", flush=True)
+    do, oo = opts_pair(**AGG_VARIANTS[variant])
+    d = ligmg_amd.Solver(g, do)
+    assert_same_hierarchy(d, O.OracleSolver(g, oo))
+    info = d.solve(graphs.rhs(g.n, 1, 0))
+    assert info.rel_residual() <= 1e-8
+
+
+def test_stalled_coarsening_is_reported():
+    """Rejection that removes nearly every member stalls coarsening until
+    max_levels; a coarse level too large for the dense solve is an error,
+    on the device and in the oracle, instead of minutes of factorisation."""
+    g = graphs.barabasi_albert(12000, 5, seed=3)
+    with pytest.raises(ligmg_amd.LigmgError) as e:
+        ligmg_amd.Solver(g, ligmg_amd.Options(rejection_ratio=0.1, max_levels=4))
+    assert e.value.code == _ffi.ERR_UNSUPPORTED and b"coarse" in _ffi.solver_lib().ligmg_last_error()
+
+
+def assert_same_hierarchy(d, r):
     assert d.num_levels() == r.num_levels()
     for l in range(d.num_levels()):
         assert d.level_info(l) == r.level_info(l), f"level {l}"

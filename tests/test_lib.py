@@ -48,6 +48,7 @@ def test_options_defaults_match_reference():
     assert (o.cheby_lower, o.cheby_upper, o.elim_factor, o.elim_iters) == (0.3, 1.1, 0.9, 1)
     assert o.outer_ksp_type == _ffi.KSP_CG and o.kcycle_ksp_type == _ffi.KSP_FCG
     assert o.aff_scaling == _ffi.AFF_MAX and o.smoother_type == _ffi.SMOOTHER_CHEBYSHEV
+    assert (o.neighbor_of_neighbor, o.rejection_ratio, o.rejection_to_seed) == (0, -1.0, 0)
     with pytest.raises(AttributeError):
         o.not_an_option = 1
 

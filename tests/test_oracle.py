@@ -201,7 +201,9 @@ def test_tv_init_exact_mean():
     assert np.all(np.abs(np.corrcoef(X.T) - np.eye(10)) < 0.15)
 
 
-def test_strength_matches_numpy_formula():
+@pytest.mark.parametrize("scaling", [_ffi.AFF_MAX, _ffi.AFF_NONE, _ffi.AFF_ROW, _ffi.AFF_COLUMN,
+                                     _ffi.AFF_ROWCOLUMN, _ffi.AFF_SYM])
+def test_strength_matches_numpy_formula(scaling):
     g = random_graph(40, 0.12, 9)
     n, T = g.n, 10
     X = np.empty(n * T)
@@ -211,7 +213,7 @@ def test_strength_matches_numpy_formula():
     S = np.empty(g.nnz)
     cs = np.empty(n)
     assert O.lib().oracle_strength(n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(v, O.f64),
-                                   O.ptr(X, O.f64), T, 4, _ffi.STRENGTH_AFFINITY, _ffi.AFF_MAX,
+                                   O.ptr(X, O.f64), T, 4, _ffi.STRENGTH_AFFINITY, scaling,
                                    1e-8, O.ptr(S, O.f64), O.ptr(cs, O.f64)) == 0
     # numpy: 4 weighted-Jacobi sweeps with b = 0 on the negative entries
     A = dense(g)
@@ -227,13 +229,32 @@ def test_strength_matches_numpy_formula():
     top = (Y[rows] * Y[cols]).sum(1)
     raw = np.where(rows != cols, top ** 2 / (nrm[cols] ** 2 * nrm[rows] ** 2), 0.0)
     Raw = sp.csr_matrix((raw, cols, g.rowptr), shape=(n, n)).toarray()
-    rmax, cmax = Raw.max(1), Raw.max(0)
-    sc = raw / np.maximum(rmax[rows], cmax[cols])
+    # aggregation.hpp:350-398: Row (and RowColumn, Sym) divide by row sums,
+    # Column (and RowColumn) by column sums, Sym averages with the
+    # transpose, Max divides by the larger of row and column maximum
+    if scaling in (_ffi.AFF_ROW, _ffi.AFF_ROWCOLUMN, _ffi.AFF_SYM):
+        Raw = Raw / Raw.sum(1)[:, None]
+    if scaling in (_ffi.AFF_COLUMN, _ffi.AFF_ROWCOLUMN):
+        Raw = Raw / Raw.sum(0)[None, :]
+    if scaling == _ffi.AFF_SYM:
+        Raw = (Raw.T + Raw) / 2
+    if scaling == _ffi.AFF_MAX:
+        rmax, cmax = Raw.max(1), Raw.max(0)
+        sc = raw / np.maximum(rmax[rows], cmax[cols])
+    else:
+        sc = Raw[rows, cols]
     sc[sc < 1e-8] = 0.0
     assert np.allclose(S, sc, rtol=1e-10, atol=0)
+    if scaling == _ffi.AFF_SYM:
+        Sd = sp.csr_matrix((S, cols, g.rowptr), shape=(n, n)).toarray()
+        assert np.array_equal(Sd, Sd.T)
+    if scaling == _ffi.AFF_ROW:
+        assert np.allclose(sp.csr_matrix((S, cols, g.rowptr), shape=(n, n)).sum(1), 1.0, atol=1e-6)
     Sm = sp.csr_matrix((S, cols, g.rowptr), shape=(n, n)).toarray()
     assert np.allclose(cs, Sm.sum(0), rtol=1e-12)
-    assert S.max() <= 1.0 and np.all(S[rows == cols] == 0)
+    assert np.all(S[rows == cols] == 0)
+    if scaling != _ffi.AFF_NONE:
+        assert S.max() <= 1.0 + 1e-15
 
 
 # ------------------------------------------------------------ voting (a11-a14)
