@@ -126,6 +126,42 @@ def bench_batched(args):
           {"workload": "ba48x16rhs_plus_grid160^3x64rhs", "rhs": nrhs}, batches=detail)
 
 
+def bench_kcycle(args):
+    """The plain method on an irregular graph: R-MAT scale 22 (edge factor
+    16, a/b/c/d = 0.57/0.19/0.19/0.05, symmetrised, largest component),
+    weights uniform[0.5, 1.5], one mean-zero N(0,1) right-hand side, FP64
+    K-cycle (FCG(2) on every non-exact coarse level) inside FP64 FCG to 1e-8.
+    A step is one solve; the fused coarse tail is on (its default)."""
+    import torch
+    import ligmg_amd
+    from ligmg_amd import _ffi, graphs
+    g = graphs.rmat_config(22, "uniform")
+    dev = torch.device("cuda:0")
+    b = torch.from_numpy(graphs.rhs(g.n, 1, 0)).to(dev)
+    A = (torch.from_numpy(g.rowptr).to(dev), torch.from_numpy(g.colind).to(dev), torch.from_numpy(g.vals).to(dev))
+    s = ligmg_amd.Solver(A, ligmg_amd.Options(kcycle=1, outer_ksp_type=_ffi.KSP_FCG))
+    print(f"# kcycle: n={g.n} nnz={g.nnz} levels {s.num_levels()} fused tail from level {s.tail_start()} "
+          f"setup {s.setup_time:.2f}s", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        s.solve(b)
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        info = s.solve(b)
+        assert info.status == _ffi.CONVERGED and info.rel_residual() <= 1e-8
+        times.append(info.time())
+    torch.cuda.synchronize()
+    ms_step = 1e3 * (time.perf_counter() - t0) / args.steps
+    v = ligmg_amd.Solver(A, ligmg_amd.Options())
+    vt = min(v.solve(b).time() for _ in range(3))
+    print(f"# kcycle solve {min(times):.4f}s, {info.iters()} iterations; V-cycle/CG on the same graph {vt:.4f}s",
+          file=sys.stderr, flush=True)
+    _line("time to 1e-8 rel. residual, FP64 K-cycle inside FCG", ms_step / 1e3, "s", False, ms_step, args,
+          "f64", {"workload": "rmat22_lcc_uniform0.5-1.5_kcycle_fcg", "n": g.n, "nnz": g.nnz},
+          iters=info.iters(), setup_s=s.setup_time, vcycle_cg_s=vt)
+
+
 def bench_grad(args):
     """Forward + backward through x = L(w)^+ b on a random geometric graph:
     1,000,000 vertices in the unit square, radius for mean degree 12, LCC,
@@ -200,9 +236,10 @@ def main():
                     help="vertex order of the hierarchy (default: the input's)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="setup + SpMV timing + copy calibration only (for counter runs)")
-    ap.add_argument("--workload", choices=["rmat", "batched", "grad"], default="rmat",
+    ap.add_argument("--workload", choices=["rmat", "batched", "grad", "kcycle"], default="rmat",
                     help="rmat: the headline solve (default); batched: 48 BA graphs x 16 right-hand "
-                         "sides and a 160^3 grid x 64; grad: gradients through a 1M-vertex geometric graph")
+                         "sides and a 160^3 grid x 64; grad: gradients through a 1M-vertex geometric graph; "
+                         "kcycle: the FP64 K-cycle on R-MAT 22")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -220,6 +257,8 @@ def main():
         return bench_batched(args)
     if args.workload == "grad":
         return bench_grad(args)
+    if args.workload == "kcycle":
+        return bench_kcycle(args)
     dev = torch.device("cuda:0")
     t = time.perf_counter()
     g = graphs.rmat_config(args.scale, "lognormal", pendants=True)
