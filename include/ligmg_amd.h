@@ -37,8 +37,15 @@ enum {
   LIGMG_ERR_IO = 6           /* file missing, unreadable or malformed        */
 };
 
-/* Krylov methods (options.hpp:205-206 KSPType values used by the reference) */
-enum { LIGMG_KSP_CG = 0, LIGMG_KSP_FCG = 1 };
+/* Krylov methods (the KSPType values parse_options.hpp:373-407 accepts).
+ * FCG is flexible CG with one previous direction (Notay's FCG(1)).
+ * PIPEFCG, PETSc's pipelined FCG, produces the same iterates up to rounding
+ * (it reorders the reductions to overlap them with the preconditioner) and
+ * runs as FCG here.  FGMRES is restarted flexible GMRES with PETSc's defaults
+ * (restart 30, classical Gram-Schmidt, right preconditioning, unpreconditioned
+ * residual norm), for the outer solve of ligmg_solve; K-cycle inner solves
+ * and batched solves use CG / FCG / PIPEFCG. */
+enum { LIGMG_KSP_CG = 0, LIGMG_KSP_FCG = 1, LIGMG_KSP_PIPEFCG = 2, LIGMG_KSP_FGMRES = 3 };
 /* options.hpp:10-12 */
 enum { LIGMG_SMOOTHER_JACOBI = 0, LIGMG_SMOOTHER_CHEBYSHEV = 1 };
 enum {

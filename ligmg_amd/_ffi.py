@@ -57,7 +57,7 @@ class ligmg_graph(C.Structure):
 
 # enums (include/ligmg_amd.h)
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_DEVICE, ERR_NOMEM, ERR_NUMERIC, ERR_IO = range(7)
-KSP_CG, KSP_FCG = 0, 1
+KSP_CG, KSP_FCG, KSP_PIPEFCG, KSP_FGMRES = 0, 1, 2, 3
 SMOOTHER_JACOBI, SMOOTHER_CHEBYSHEV = 0, 1
 AFF_NONE, AFF_ROW, AFF_COLUMN, AFF_ROWCOLUMN, AFF_SYM, AFF_MAX = range(6)
 STRENGTH_AFFINITY, STRENGTH_ALGEBRAIC_DISTANCE, STRENGTH_NONE = 0, 1, 2

@@ -85,6 +85,28 @@ __global__ void __launch_bounds__(256) k_kr_pupdate(T *p, const T *z, const int6
   for (int64_t i = lo[s] + threadIdx.x; i < hi[s]; i += 256) p[i] = z[i] + b * p[i];
 }
 
+// FGMRES: y += sum_t coef[blk * K + t] V_t (V_t = V + t ld), added in t order
+__global__ void __launch_bounds__(256) k_blk_lincomb(double *y, const double *V, int64_t ld, int k,
+                                                     const double *coef, int K, const int64_t *lo,
+                                                     const int64_t *hi, const int32_t *segblk) {
+  const int s = blockIdx.x;
+  const double *cf = coef + (int64_t)segblk[s] * K;
+  for (int64_t i = lo[s] + threadIdx.x; i < hi[s]; i += 256) {
+    double acc = y[i];
+    for (int t = 0; t < k; t++) acc += cf[t] * V[t * ld + i];
+    y[i] = acc;
+  }
+}
+// FGMRES: y = x * sc[blk]
+__global__ void __launch_bounds__(256) k_blk_scale(double *y, const double *x, const double *sc,
+                                                   const int64_t *lo, const int64_t *hi, const int32_t *segblk) {
+  const int s = blockIdx.x;
+  const double a = sc[segblk[s]];
+  for (int64_t i = lo[s] + threadIdx.x; i < hi[s]; i += 256) y[i] = x[i] * a;
+}
+
+inline bool fcg_like(int t) { return t == LIGMG_KSP_FCG || t == LIGMG_KSP_PIPEFCG; }
+
 // batch: dot results out[(blk * np + j) * kp + c] -> scalar sets of (blk, c)
 __global__ void k_kr_bstore(int nb, int kp, int np, const double *out, int s0, int s1, double *sc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -651,7 +673,10 @@ struct Solver {
       snprintf(b, sizeof b, "%g", x);  // std::ostream's default for doubles
       return std::string(b);
     };
-    auto ksp = [](int t) { return std::string(t == LIGMG_KSP_FCG ? "fcg" : "cg"); };
+    auto ksp = [](int t) {
+      static const char *names[] = {"cg", "fcg", "pipefcg", "fgmres"};
+      return std::string(names[std::min(std::max(t, 0), 3)]);
+    };
     static const char *aff[] = {"None", "combblas::Row", "Column", "RowColumn", "Sym", "Max"};
     static const char *sm[] = {"affinity", "algebraic-distance", "none"};
     std::string r;
@@ -822,11 +847,12 @@ struct Solver {
     LevelWork<T> &w = work<T>(l);
     const int64_t n = H[l].n;
     const int m = o.pre_smooth;
-    if (m <= 0 || o.smoother_type == LIGMG_SMOOTHER_JACOBI) {
+    const bool jac = o.smoother_type == LIGMG_SMOOTHER_JACOBI;
+    if (m <= 0 || jac) {
       TailInstr I = ti(TI_ZERO, n);
       I.p[0] = w.B[0].p;
       c.push_back(I);
-      if (m <= 0) return 0;
+      if (!jac) return 0;
       rec_cheb<T>(c, l, dinv, b, w.B[0].p, nullptr, w.B[1].p, 0.0, 2.0 / 3.0);
       return 1;
     }
@@ -846,12 +872,12 @@ struct Solver {
   int rec_smooth_from(std::vector<TailInstr> &c, size_t l, const T *dinv, const T *b, int s0) {
     LevelWork<T> &w = work<T>(l);
     const int m = o.post_smooth;
-    if (m <= 0) return s0;
     if (o.smoother_type == LIGMG_SMOOTHER_JACOBI) {
       const int s1 = (s0 + 1) % 3;
       rec_cheb<T>(c, l, dinv, b, w.B[s0].p, nullptr, w.B[s1].p, 0.0, 2.0 / 3.0);
       return s1;
     }
+    if (m <= 0) return s0;
     for (int k = 0; k < m; k++) {
       const T *xp = (k == 0) ? nullptr : w.B[(s0 + k - 1) % 3].p;
       rec_cheb<T>(c, l, dinv, b, w.B[(s0 + k) % 3].p, xp, w.B[(s0 + k + 1) % 3].p, al_post[l][k],
@@ -926,7 +952,7 @@ struct Solver {
   void rec_krylov(std::vector<TailInstr> &c, size_t lvl, T *x, const T *b) {
     KWork<T> &K = work<T>(lvl).kw;
     const int64_t n = H[lvl].A.n;
-    const bool fcg = o.kcycle_ksp_type == LIGMG_KSP_FCG;
+    const bool fcg = fcg_like(o.kcycle_ksp_type);
     TailInstr I = ti(TI_KR_INIT);
     I.lvl = (int32_t)lvl;
     c.push_back(I);
@@ -1055,7 +1081,7 @@ struct Solver {
     }
     make_tail<double>(tail_ops64, tail_lv64);
     if (mixed) make_tail<float>(tail_ops32, tail_lv32);
-    tail_p.fcg = o.kcycle_ksp_type == LIGMG_KSP_FCG;
+    tail_p.fcg = fcg_like(o.kcycle_ksp_type);
     tail_p.rtol = o.tol;
     LG_HIP(hipStreamSynchronize(s));
     if (o.verbose)
@@ -1138,13 +1164,15 @@ struct Solver {
 
   template <class T>
   int smooth_from_zero(DLevel &L, LevelWork<T> &w, const T *dinv, const T *b, int m, size_t l) {
-    if (m <= 0) {
-      fill_zero<T>(w.B[0].p, L.n);
-      return 0;
-    }
+    // the reference's Jacobi smoother is one sweep whatever the step count
+    // (create_smoother, smoothing.hpp:206-211)
     if (o.smoother_type == LIGMG_SMOOTHER_JACOBI) {
       fill_zero<T>(w.B[0].p, L.n);
       return jacobi<T>(L, w, dinv, b, 0);
+    }
+    if (m <= 0) {
+      fill_zero<T>(w.B[0].p, L.n);
+      return 0;
     }
     // x_1 = beta_0 D^-1 b in B[1]; x_{k+1} in B[(k+1) % 3]
     op_cheb_step<T>(L.A, dinv, b, nullptr, nullptr, w.B[1].p, 0.0, be_pre[l][0], s);
@@ -1158,8 +1186,8 @@ struct Solver {
 
   template <class T>
   int smooth_from(DLevel &L, LevelWork<T> &w, const T *dinv, const T *b, int s0, int m, size_t l) {
-    if (m <= 0) return s0;
     if (o.smoother_type == LIGMG_SMOOTHER_JACOBI) return jacobi<T>(L, w, dinv, b, s0);
+    if (m <= 0) return s0;
     for (int k = 0; k < m; k++) {
       const T *xp = (k == 0) ? nullptr : w.B[(s0 + k - 1) % 3].p;
       op_cheb_step<T>(L.A, dinv, b, w.B[(s0 + k) % 3].p, xp, w.B[(s0 + k + 1) % 3].p, al_post[l][k],
@@ -1207,7 +1235,7 @@ struct Solver {
     const RedPlan &R = red[lvl];
     const int nb = R.nb;
     const int64_t n = A.n;
-    const bool fcg = type == LIGMG_KSP_FCG;
+    const bool fcg = fcg_like(type);
     hipLaunchKernelGGL(k_kr_init, dim3((nb + 63) / 64), dim3(64), 0, s, nb, K.sc.p);
     // r = b - A x
     op_residual<T>(A, x, b, K.r.p, s);
@@ -1270,8 +1298,182 @@ struct Solver {
   }
 
   int krylov_outer(size_t base, const double *rhs, std::vector<double> &h, int *status) {
+    if (o.outer_ksp_type == LIGMG_KSP_FGMRES) return fgmres(base, outer.x.p, rhs, &h, status);
     return krylov<double>(base, H[base].A, outer.x.p, rhs, outer, o.outer_ksp_type, o.tol, o.iters,
                           true, &h, status, true);
+  }
+
+  // FGMRES(m) outer solve: restarted flexible GMRES with PETSc KSPFGMRES's
+  // defaults (restart 30, classical Gram-Schmidt, right preconditioning,
+  // unpreconditioned residual norm).  The Arnoldi process, Givens rotations
+  // and convergence test are per block; the Hessenberg matrices live on the
+  // host, the basis V (m + 1 vectors) and the preconditioned basis Z (m) on
+  // the device.  The history logs the true residual at each cycle start and
+  // the Givens estimate after each iteration, as PETSc's KSPGMRESCycle.
+  static constexpr int FGMRES_RESTART = 30;
+  DevBuf<double> gm_V, gm_Z, gm_coef;
+
+  int fgmres(size_t lvl, double *x, const double *b, std::vector<double> *hist, int *status) {
+    const RedPlan &R = red[lvl];
+    const DCsr &A = H[lvl].A;
+    const int nb = R.nb, m = FGMRES_RESTART;
+    const int64_t n = A.n;
+    const double rtol = o.tol;
+    const int maxit = o.iters;
+    gm_V.ensure((size_t)(m + 1) * n);
+    gm_Z.ensure((size_t)m * n);
+    gm_coef.ensure((size_t)nb * (m + 1));
+    KWork<double> &K = outer;
+    const int32_t *segblk = R.bseg.p + R.nb + 1;
+    auto V = [&](int j) { return gm_V.p + (size_t)j * n; };
+    auto Z = [&](int j) { return gm_Z.p + (size_t)j * n; };
+    auto dots = [&](int np, const double *const *a, const double *const *bb) {
+      R.dots<double>(np, a, bb, K.out.p, s);
+      return download(K.out.p, (size_t)nb * np, s);
+    };
+    auto upload = [&](const std::vector<double> &c) {
+      to_device(gm_coef.p, c.data(), c.size(), s);
+      LG_HIP(hipStreamSynchronize(s));
+    };
+    auto lincomb = [&](double *y, const double *base, int k, const std::vector<double> &coef, int kk) {
+      upload(coef);
+      hipLaunchKernelGGL(k_blk_lincomb, dim3(R.nseg), dim3(256), 0, s, y, base, n, k, (const double *)gm_coef.p,
+                         kk, R.seg_lo.p, R.seg_hi.p, segblk);
+      LG_LAUNCH_CHECK();
+    };
+    auto scale = [&](double *y, const std::vector<double> &sc) {
+      upload(sc);
+      hipLaunchKernelGGL(k_blk_scale, dim3(R.nseg), dim3(256), 0, s, y, (const double *)y,
+                         (const double *)gm_coef.p, R.seg_lo.p, R.seg_hi.p, segblk);
+      LG_LAUNCH_CHECK();
+    };
+    auto none = [](const std::vector<char> &f) { return !std::any_of(f.begin(), f.end(), [](char a) { return a; }); };
+    const double *bb1[1] = {b};
+    const std::vector<double> bn = dots(1, bb1, bb1);
+    if (status) *status = LIGMG_CONVERGED;
+    int it = 0;
+    std::vector<char> active(nb, 1);
+    while (true) {
+      op_residual<double>(A, x, b, V(0), s);
+      const double *v0[1] = {V(0)};
+      const std::vector<double> rn = dots(1, v0, v0);
+      double t = 0;
+      bool nan = false;
+      for (int c = 0; c < nb; c++) {
+        t += rn[c];
+        nan |= rn[c] != rn[c];
+        if (!(std::sqrt(rn[c]) > rtol * std::sqrt(bn[c]))) active[c] = 0;
+      }
+      if (hist) hist->push_back(std::sqrt(t));
+      if (nan) {
+        if (status) *status = LIGMG_DIVERGED_NAN;
+        return it;
+      }
+      if (none(active)) return it;
+      if (it >= maxit) {
+        if (status) *status = LIGMG_DIVERGED_ITS;
+        return it;
+      }
+      std::vector<double> Hh((size_t)nb * (m + 1) * m, 0.0), g((size_t)nb * (m + 1), 0.0), cs((size_t)nb * m, 0.0),
+          sn((size_t)nb * m, 0.0), est(nb), sc(nb);
+      std::vector<int> cols(nb, 0);
+      std::vector<char> live = active;
+      for (int c = 0; c < nb; c++) {
+        const double beta = std::sqrt(rn[c]);
+        est[c] = active[c] ? beta : 0.0;
+        g[(size_t)c * (m + 1)] = beta;
+        sc[c] = (live[c] && beta > 0) ? 1.0 / beta : 0.0;
+      }
+      scale(V(0), sc);
+      bool done = false;
+      for (int j = 0; j < m; j++) {
+        d2d(K.r.p, V(j), n, s);
+        apply_prec<double>(lvl, K.r.p, K.z.p, true);
+        d2d(Z(j), K.z.p, n, s);
+        op_spmv<double>(A, Z(j), V(j + 1), s);
+        // classical Gram-Schmidt: h_i = <w, v_i>, i <= j, four at a time
+        std::vector<double> h((size_t)(j + 1) * nb);
+        for (int i0 = 0; i0 <= j; i0 += 4) {
+          const int np = std::min(4, j + 1 - i0);
+          const double *a[4] = {V(j + 1), V(j + 1), V(j + 1), V(j + 1)};
+          const double *vv[4] = {nullptr, nullptr, nullptr, nullptr};
+          for (int q = 0; q < np; q++) vv[q] = V(i0 + q);
+          const std::vector<double> out = dots(np, a, vv);
+          for (int q = 0; q < np; q++)
+            for (int c = 0; c < nb; c++) h[(size_t)(i0 + q) * nb + c] = out[(size_t)c * np + q];
+        }
+        std::vector<double> coef((size_t)nb * (j + 1), 0.0);
+        for (int c = 0; c < nb; c++)
+          if (live[c])
+            for (int i = 0; i <= j; i++) coef[(size_t)c * (j + 1) + i] = -h[(size_t)i * nb + c];
+        lincomb(V(j + 1), V(0), j + 1, coef, j + 1);
+        const double *w1[1] = {V(j + 1)};
+        const std::vector<double> hn = dots(1, w1, w1);
+        for (int c = 0; c < nb; c++) {
+          const double nrm = std::sqrt(hn[c]);
+          sc[c] = (live[c] && nrm > 0) ? 1.0 / nrm : 0.0;
+        }
+        scale(V(j + 1), sc);
+        for (int c = 0; c < nb; c++) {
+          if (!live[c]) continue;
+          const double nrm = std::sqrt(hn[c]);
+          double *col = Hh.data() + ((size_t)c * m + j) * (m + 1);  // column j, rows 0..j+1
+          double *gc = g.data() + (size_t)c * (m + 1);
+          double *csc = cs.data() + (size_t)c * m, *snc = sn.data() + (size_t)c * m;
+          for (int i = 0; i <= j; i++) col[i] = h[(size_t)i * nb + c];
+          col[j + 1] = nrm;
+          for (int i = 0; i < j; i++) {
+            const double a = col[i], bq = col[i + 1];
+            col[i] = csc[i] * a + snc[i] * bq;
+            col[i + 1] = -snc[i] * a + csc[i] * bq;
+          }
+          const double a = col[j], bq = col[j + 1];
+          const double d = std::hypot(a, bq);
+          csc[j] = d > 0 ? a / d : 1.0;
+          snc[j] = d > 0 ? bq / d : 0.0;
+          col[j] = d;
+          col[j + 1] = 0.0;
+          gc[j + 1] = -snc[j] * gc[j];
+          gc[j] = csc[j] * gc[j];
+          cols[c] = j + 1;
+          est[c] = std::fabs(gc[j + 1]);
+          if (!(est[c] > rtol * std::sqrt(bn[c]))) {
+            live[c] = 0;
+            active[c] = 0;
+          } else if (nrm == 0) {
+            live[c] = 0;
+          }
+        }
+        it++;
+        double t2 = 0;
+        for (int c = 0; c < nb; c++) t2 += est[c] * est[c];
+        done = none(active) || it >= maxit;
+        if (j + 1 == m && !done) break;
+        if (hist) hist->push_back(std::sqrt(t2));
+        if (done || none(live)) break;
+      }
+      // x += Z y with H y = g, per block
+      int kmax = 0;
+      for (int c = 0; c < nb; c++) kmax = std::max(kmax, cols[c]);
+      std::vector<double> y((size_t)nb * m, 0.0);
+      for (int c = 0; c < nb; c++) {
+        const int k = cols[c];
+        const double *Hc = Hh.data() + (size_t)c * m * (m + 1);
+        const double *gc = g.data() + (size_t)c * (m + 1);
+        double *yc = y.data() + (size_t)c * m;
+        for (int i = k - 1; i >= 0; i--) {
+          double sacc = gc[i];
+          for (int q = i + 1; q < k; q++) sacc -= Hc[(size_t)q * (m + 1) + i] * yc[q];
+          const double d = Hc[(size_t)i * (m + 1) + i];
+          yc[i] = d != 0 ? sacc / d : 0.0;
+        }
+      }
+      if (kmax > 0) lincomb(x, Z(0), kmax, y, m);
+      if (done) {
+        if (status) *status = none(active) ? LIGMG_CONVERGED : LIGMG_DIVERGED_ITS;
+        return it;
+      }
+    }
   }
 
   static double total_norm(const std::vector<double> &hsc, int nb) {
@@ -1426,10 +1628,11 @@ struct Solver {
     const T *dinv = dinvv<T>(L);
     int cur;
     const int m = o.pre_smooth;
-    if (o.smoother_type == LIGMG_SMOOTHER_JACOBI || m <= 0) {
+    const bool jac = o.smoother_type == LIGMG_SMOOTHER_JACOBI;  // one sweep whatever m
+    if (jac || m <= 0) {
       fill_zero<T>(w.B[0].p, L.n * kp);
       cur = 0;
-      if (m > 0) {
+      if (jac) {
         op_bcheb_step<T>(L.A, dinv, b, w.B[0].p, nullptr, w.B[1].p, 0.0, 2.0 / 3.0, kp, bparts, s);
         cur = 1;
       }
@@ -1449,11 +1652,9 @@ struct Solver {
     op_bprolong_add<T>(L.agg.p, L.n, xc, w.B[cur].p, kp, s);
     const int mp = o.post_smooth;
     if (o.smoother_type == LIGMG_SMOOTHER_JACOBI) {
-      if (mp > 0) {
-        int s1 = (cur + 1) % 3;
-        op_bcheb_step<T>(L.A, dinv, b, w.B[cur].p, nullptr, w.B[s1].p, 0.0, 2.0 / 3.0, kp, bparts, s);
-        cur = s1;
-      }
+      int s1 = (cur + 1) % 3;
+      op_bcheb_step<T>(L.A, dinv, b, w.B[cur].p, nullptr, w.B[s1].p, 0.0, 2.0 / 3.0, kp, bparts, s);
+      cur = s1;
     } else {
       for (int k = 0; k < mp; k++) {
         const T *xp = (k == 0) ? nullptr : w.B[(cur + k - 1) % 3].p;
@@ -1515,7 +1716,7 @@ struct Solver {
               bool outer_prec) {
     const RedPlan &R = red[lvl];
     const int ns = R.nb * kp, lk = ilog2(kp);
-    const bool fcg = type == LIGMG_KSP_FCG;
+    const bool fcg = fcg_like(type);
     auto grid1 = dim3((ns + 63) / 64);
     hipLaunchKernelGGL(k_kr_init, grid1, dim3(64), 0, s, ns, K.sc.p);
     op_bresidual<T>(A, x, b, K.r.p, kp, bparts, s);
@@ -1593,6 +1794,8 @@ BLevelWork<float> &Solver::bwork<float>(size_t l) {
 int Solver::solve_batch(int k, const double *Bin, double *Xout, int mem, ligmg_info *info,
                         int32_t *iters_out) {
   LG_CHECK(k >= 1 && k <= 64, LIGMG_ERR_INVALID, "solve_batch: k must be in [1, 64]");
+  LG_CHECK(o.outer_ksp_type != LIGMG_KSP_FGMRES, LIGMG_ERR_UNSUPPORTED,
+           "solve_batch: batched solves use CG / FCG / PIPEFCG");
   int kp = 1;
   while (kp < k) kp <<= 1;
   const DLevel &L0 = H[0];
@@ -1900,6 +2103,10 @@ const char *ligmg_last_error(void) { return lg::g_last_error.c_str(); }
 
 static int check_options(const ligmg_options &o) {
   if (o.elim_iters != 1) return LIGMG_ERR_UNSUPPORTED;
+  if (o.outer_ksp_type < LIGMG_KSP_CG || o.outer_ksp_type > LIGMG_KSP_FGMRES) return LIGMG_ERR_INVALID;
+  if (o.kcycle_ksp_type < LIGMG_KSP_CG || o.kcycle_ksp_type > LIGMG_KSP_FGMRES) return LIGMG_ERR_INVALID;
+  // inner K-cycle solves run on the device without host steps
+  if (o.kcycle_ksp_type == LIGMG_KSP_FGMRES) return LIGMG_ERR_UNSUPPORTED;
   if (o.num_tvs < 1 || o.num_tvs > 16) return LIGMG_ERR_UNSUPPORTED;
   if (o.aff_scaling < LIGMG_AFF_NONE || o.aff_scaling > LIGMG_AFF_MAX) return LIGMG_ERR_INVALID;
   if (o.pre_smooth < 0 || o.post_smooth < 0 || o.iters < 0 || o.kcycle_iters < 1 || o.tail_nnz < 0)

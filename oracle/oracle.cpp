@@ -939,7 +939,7 @@ int krylov(const Csr &A, std::vector<double> &x, const std::vector<double> &b, c
     spmv(A, p.data(), q.data());
     bdot(p, q, delta);
     std::vector<double> alpha(nb, 0.0);
-    if (type == LIGMG_KSP_FCG) {
+    if (type == LIGMG_KSP_FCG || type == LIGMG_KSP_PIPEFCG) {
       bdot(p, r, t1);
       for (size_t c = 0; c < nb; c++) alpha[c] = active[c] && delta[c] != 0 ? t1[c] / delta[c] : 0;
     } else {
@@ -964,7 +964,7 @@ int krylov(const Csr &A, std::vector<double> &x, const std::vector<double> &b, c
     }
     if (!std::any_of(active.begin(), active.end(), [](char a) { return a; })) return it;
     prec(r, z);
-    if (type == LIGMG_KSP_FCG) {
+    if (type == LIGMG_KSP_FCG || type == LIGMG_KSP_PIPEFCG) {
       bdot(z, q, t1);
       for (size_t c = 0; c < nb; c++) {
         double beta = active[c] && delta[c] != 0 ? -t1[c] / delta[c] : 0;
@@ -982,6 +982,150 @@ int krylov(const Csr &A, std::vector<double> &x, const std::vector<double> &b, c
   }
   if (status) *status = LIGMG_DIVERGED_ITS;
   return it;
+}
+
+// FGMRES(m) (Saad, Iterative Methods for Sparse Linear Systems, 2nd ed.,
+// Alg. 9.6; PETSc KSPFGMRES defaults: restart 30, classical Gram-Schmidt,
+// right preconditioning, unpreconditioned residual norm), with an Arnoldi
+// process, Givens rotations and a convergence test per block.  The history
+// holds the norm at the start of every restart cycle (the true residual, as
+// PETSc logs it) and the Givens estimate after every iteration.
+constexpr int FGMRES_RESTART = 30;
+
+template <class Prec>
+int fgmres(const Csr &A, std::vector<double> &x, const std::vector<double> &b, const Blocks &B, Prec prec,
+           double rtol, int maxit, std::vector<double> *hist, int *status) {
+  const int64_t n = A.n;
+  const size_t nb = B.ptr.size() - 1;
+  const int m = FGMRES_RESTART;
+  auto bdot = [&](const std::vector<double> &u, const std::vector<double> &v, std::vector<double> &out) {
+    out.assign(nb, 0.0);
+    for (size_t c = 0; c < nb; c++) {
+      double s = 0;
+      for (int64_t i = B.ptr[c]; i < B.ptr[c + 1]; i++) s += u[i] * v[i];
+      out[c] = s;
+    }
+  };
+  auto none = [](const std::vector<char> &f) { return !std::any_of(f.begin(), f.end(), [](char a) { return a; }); };
+  std::vector<double> bn, rn, r(n), w(n), tmp(n);
+  bdot(b, b, bn);
+  if (status) *status = LIGMG_CONVERGED;
+  int it = 0;
+  std::vector<char> active(nb, 1);
+  while (true) {
+    // cycle start: the true residual, logged (KSPGMRESCycle)
+    spmv(A, x.data(), tmp.data());
+    for (int64_t i = 0; i < n; i++) r[i] = b[i] - tmp[i];
+    bdot(r, r, rn);
+    double t = 0;
+    bool nan = false;
+    for (size_t c = 0; c < nb; c++) {
+      t += rn[c];
+      nan |= rn[c] != rn[c];
+      if (!(std::sqrt(rn[c]) > rtol * std::sqrt(bn[c]))) active[c] = 0;
+    }
+    if (hist) hist->push_back(std::sqrt(t));
+    if (nan) {
+      if (status) *status = LIGMG_DIVERGED_NAN;
+      return it;
+    }
+    if (none(active)) return it;
+    if (it >= maxit) {
+      if (status) *status = LIGMG_DIVERGED_ITS;
+      return it;
+    }
+    std::vector<std::vector<double>> V(1, std::vector<double>(n, 0.0)), Z;
+    std::vector<std::vector<double>> Hc(nb, std::vector<double>((size_t)(m + 1) * m, 0.0)), g(nb), cs(nb), sn(nb);
+    std::vector<int> cols(nb, 0);
+    std::vector<char> live = active;
+    std::vector<double> est(nb);
+    for (size_t c = 0; c < nb; c++) {
+      const double beta = std::sqrt(rn[c]);
+      est[c] = active[c] ? beta : 0.0;
+      g[c].assign(m + 1, 0.0);
+      g[c][0] = beta;
+      cs[c].assign(m, 0.0);
+      sn[c].assign(m, 0.0);
+      const double s = (live[c] && beta > 0) ? 1.0 / beta : 0.0;
+      for (int64_t i = B.ptr[c]; i < B.ptr[c + 1]; i++) V[0][i] = r[i] * s;
+    }
+    bool done = false;
+    for (int j = 0; j < m; j++) {
+      Z.emplace_back(n, 0.0);
+      prec(V[j], Z[j]);
+      spmv(A, Z[j].data(), w.data());
+      std::vector<std::vector<double>> h(j + 1);
+      for (int i = 0; i <= j; i++) bdot(w, V[i], h[i]);  // classical Gram-Schmidt
+      for (size_t c = 0; c < nb; c++) {
+        if (!live[c]) continue;
+        for (int64_t k = B.ptr[c]; k < B.ptr[c + 1]; k++) {
+          double s = w[k];
+          for (int i = 0; i <= j; i++) s -= h[i][c] * V[i][k];
+          w[k] = s;
+        }
+      }
+      std::vector<double> hn;
+      bdot(w, w, hn);
+      V.emplace_back(n, 0.0);
+      for (size_t c = 0; c < nb; c++) {
+        if (!live[c]) continue;
+        const double nrm = std::sqrt(hn[c]);
+        const double s = nrm > 0 ? 1.0 / nrm : 0.0;
+        for (int64_t k = B.ptr[c]; k < B.ptr[c + 1]; k++) V[j + 1][k] = w[k] * s;
+        double *col = Hc[c].data() + (size_t)j * (m + 1);  // column j, rows 0..j+1
+        for (int i = 0; i <= j; i++) col[i] = h[i][c];
+        col[j + 1] = nrm;
+        for (int i = 0; i < j; i++) {  // the previous rotations
+          const double a = col[i], bb = col[i + 1];
+          col[i] = cs[c][i] * a + sn[c][i] * bb;
+          col[i + 1] = -sn[c][i] * a + cs[c][i] * bb;
+        }
+        const double a = col[j], bb = col[j + 1];
+        const double d = std::hypot(a, bb);
+        cs[c][j] = d > 0 ? a / d : 1.0;
+        sn[c][j] = d > 0 ? bb / d : 0.0;
+        col[j] = d;
+        col[j + 1] = 0.0;
+        g[c][j + 1] = -sn[c][j] * g[c][j];
+        g[c][j] = cs[c][j] * g[c][j];
+        cols[c] = j + 1;
+        est[c] = std::fabs(g[c][j + 1]);
+        if (!(est[c] > rtol * std::sqrt(bn[c]))) {  // converged (includes a happy breakdown)
+          live[c] = 0;
+          active[c] = 0;
+        } else if (nrm == 0) {
+          live[c] = 0;
+        }
+      }
+      it++;
+      double t2 = 0;
+      for (size_t c = 0; c < nb; c++) t2 += est[c] * est[c];
+      done = none(active) || it >= maxit;
+      if (j + 1 == m && !done) break;  // restart: the next cycle logs the true residual
+      if (hist) hist->push_back(std::sqrt(t2));
+      if (done || none(live)) break;
+    }
+    // x += Z y, H y = g (upper triangular, per block)
+    for (size_t c = 0; c < nb; c++) {
+      const int k = cols[c];
+      std::vector<double> y(k, 0.0);
+      for (int i = k - 1; i >= 0; i--) {
+        double s = g[c][i];
+        for (int q = i + 1; q < k; q++) s -= Hc[c][(size_t)q * (m + 1) + i] * y[q];
+        const double d = Hc[c][(size_t)i * (m + 1) + i];
+        y[i] = d != 0 ? s / d : 0.0;
+      }
+      for (int64_t e = B.ptr[c]; e < B.ptr[c + 1]; e++) {
+        double s = x[e];
+        for (int q = 0; q < k; q++) s += y[q] * Z[q][e];
+        x[e] = s;
+      }
+    }
+    if (done) {
+      if (status) *status = none(active) ? LIGMG_CONVERGED : LIGMG_DIVERGED_ITS;
+      return it;
+    }
+  }
 }
 
 Blocks level_blocks(const Solver &S, size_t l) {
@@ -1114,15 +1258,15 @@ int solve(const Solver &S, const double *bin, double *xout, ligmg_info *info, do
   std::vector<double> x(rhs.size(), 0.0), h;
   int status = 0;
   int outer = S.opts.outer_ksp_type;
-  int iters = krylov(
-      S.H[base].A, x, rhs, B,
-      [&](const std::vector<double> &r, std::vector<double> &z) {
-        std::vector<double> xx(r.size(), 0.0);
-        cycle(S, base, xx, r);
-        z = xx;
-        project_mean(z, B);
-      },
-      outer, S.opts.tol, S.opts.iters, &h, &status);
+  auto prec = [&](const std::vector<double> &r, std::vector<double> &z) {
+    std::vector<double> xx(r.size(), 0.0);
+    cycle(S, base, xx, r);
+    z = xx;
+    project_mean(z, B);
+  };
+  int iters = outer == LIGMG_KSP_FGMRES
+                  ? fgmres(S.H[base].A, x, rhs, B, prec, S.opts.tol, S.opts.iters, &h, &status)
+                  : krylov(S.H[base].A, x, rhs, B, prec, outer, S.opts.tol, S.opts.iters, &h, &status);
   std::vector<double> sol(n);
   if (do_once) {
     spmv(L0.P, x.data(), sol.data());

@@ -175,6 +175,8 @@ VARIANTS = {
     "jacobi": {"smoother_type": _ffi.SMOOTHER_JACOBI},
     "no_top_elim": {"top_elim": 0},
     "fcg_outer": {"outer_ksp_type": _ffi.KSP_FCG},
+    "fgmres_outer": {"outer_ksp_type": _ffi.KSP_FGMRES},
+    "pipefcg_kcycle": {"kcycle": 1, "outer_ksp_type": _ffi.KSP_PIPEFCG, "kcycle_ksp_type": _ffi.KSP_PIPEFCG},
 }
 
 
@@ -224,6 +226,52 @@ def test_mixed_precision(case, my_matrix):
     assert np.linalg.norm((x - x.mean()) - (xr - xr.mean())) <= 1e-6 * np.linalg.norm(xr - xr.mean())
     print(f"mixed iters={info.iters()} fp64 oracle iters={rinfo.iters}")
     assert info.iters() <= rinfo.iters + 5
+
+
+def test_fgmres_restarts_match_oracle():
+    """FGMRES(30) past a restart (one-sweep Jacobi, tolerance 1e-13):
+    histories, iteration count and solution against the oracle."""
+    g = graphs.grid2d(96, 80)
+    kw = dict(outer_ksp_type=_ffi.KSP_FGMRES, smoother_type=_ffi.SMOOTHER_JACOBI, pre_smooth=1, post_smooth=0,
+              tol=1e-13)
+    do, oo = opts_pair(**kw)
+    b = graphs.rhs(g.n, 1, 0)
+    info = ligmg_amd.Solver(g, do).solve(b)
+    xr, rinfo, rhist = O.OracleSolver(g, oo).solve(b)
+    print(f"iters {info.iters()} oracle {rinfo.iters}")
+    assert info.status == _ffi.CONVERGED and info.iters() > 30
+    assert abs(info.iters() - rinfo.iters) <= 2
+    h = np.asarray(info.residuals)
+    assert len(h) == info.iters() + 1
+    assert np.allclose(h[:20], rhist[:20], rtol=1e-6)
+    x = info.solution
+    assert np.linalg.norm((x - x.mean()) - (xr - xr.mean())) <= 1e-9 * np.linalg.norm(xr - xr.mean())
+
+
+@pytest.mark.parametrize("prec", [_ffi.PREC_FP64, _ffi.PREC_MIXED])
+def test_fgmres_blocks_and_precisions(prec):
+    gs = [graphs.barabasi_albert(n, 4, seed=s) for s, n in enumerate([3000, 800, 5000])]
+    u = graphs.union(gs)
+    b = graphs.rhs(u.n, 1, 5, comp_ptr=u.comp_ptr)
+    d = ligmg_amd.Solver(u, ligmg_amd.Options(outer_ksp_type=_ffi.KSP_FGMRES, precision=prec), comp_ptr=u.comp_ptr)
+    info = d.solve(b)
+    assert info.status == _ffi.CONVERGED and info.rel_residual() <= 1e-8
+    xr, _, _ = O.OracleSolver(u, O.default_options(outer_ksp_type=_ffi.KSP_FGMRES), comp_ptr=u.comp_ptr).solve(b)
+    for c in range(len(gs)):
+        lo, hi = u.comp_ptr[c], u.comp_ptr[c + 1]
+        x, y = info.solution[lo:hi], xr[lo:hi]
+        assert np.linalg.norm((x - x.mean()) - (y - y.mean())) <= 1e-6 * np.linalg.norm(y - y.mean())
+
+
+def test_unsupported_krylov_combinations():
+    g = graphs.grid2d(20, 20)
+    with pytest.raises(ligmg_amd.LigmgError) as e:
+        ligmg_amd.Solver(g, ligmg_amd.Options(kcycle=1, kcycle_ksp_type=_ffi.KSP_FGMRES))
+    assert e.value.code == _ffi.ERR_UNSUPPORTED
+    s = ligmg_amd.Solver(g, ligmg_amd.Options(outer_ksp_type=_ffi.KSP_FGMRES))
+    with pytest.raises(ligmg_amd.LigmgError) as e:
+        s.solve_batch(np.asfortranarray(graphs.rhs(g.n, 2, 0).reshape(g.n, 2, order="F")))
+    assert e.value.code == _ffi.ERR_UNSUPPORTED
 
 
 def test_determinism():

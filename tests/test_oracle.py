@@ -404,7 +404,7 @@ def check_solution(g, x, b, info, tol=1e-6):
 
 
 @pytest.mark.parametrize("case", ["my_matrix", "grid", "rmat_pend", "ba", "rgg"])
-@pytest.mark.parametrize("variant", ["vcycle_cg", "kcycle_fcg"])
+@pytest.mark.parametrize("variant", ["vcycle_cg", "kcycle_fcg", "vcycle_fgmres", "kcycle_pipefcg"])
 def test_solve_against_sparse_direct(case, variant, my_matrix):
     g = {
         "my_matrix": lambda: my_matrix,
@@ -414,9 +414,12 @@ def test_solve_against_sparse_direct(case, variant, my_matrix):
         "rgg": lambda: graphs.rgg(3000, math.sqrt(12 / (math.pi * 3000)), seed=1),
     }[case]()
     opts = O.default_options()
-    if variant == "kcycle_fcg":
+    if variant in ("kcycle_fcg", "kcycle_pipefcg"):
         opts.kcycle = 1
-        opts.outer_ksp_type = _ffi.KSP_FCG
+        opts.outer_ksp_type = _ffi.KSP_FCG if variant == "kcycle_fcg" else _ffi.KSP_PIPEFCG
+        opts.kcycle_ksp_type = opts.outer_ksp_type
+    if variant == "vcycle_fgmres":
+        opts.outer_ksp_type = _ffi.KSP_FGMRES
     s = O.OracleSolver(g, opts)
     b = graphs.rhs(g.n, 1, 0)
     x, info, hist = s.solve(b)
@@ -424,6 +427,43 @@ def test_solve_against_sparse_direct(case, variant, my_matrix):
     assert len(hist) == info.iters + 1
     assert info.rel_residual == pytest.approx(hist[-1] / hist[0])
     assert info.rel_residual <= 1e-8
+
+
+def test_fgmres_restarts_and_residual_estimates():
+    """FGMRES(30): a weak (one-sweep Jacobi) preconditioner and a tight
+    tolerance force restarts; the logged norms never increase within a cycle
+    (a GMRES property), each restart logs the true residual, the history has
+    iterations + 1 entries as PETSc's, and the solution matches the sparse
+    direct solve."""
+    g = graphs.grid2d(96, 80)
+    b = graphs.rhs(g.n, 1, 0)
+    kw = dict(smoother_type=_ffi.SMOOTHER_JACOBI, pre_smooth=1, post_smooth=0, tol=1e-13)
+    x, info, hist = O.OracleSolver(g, O.default_options(outer_ksp_type=_ffi.KSP_FGMRES, **kw)).solve(b)
+    assert info.status == _ffi.CONVERGED and info.iters > 30  # at least one restart
+    assert len(hist) == info.iters + 1
+    for c0 in range(0, len(hist), 30):  # one cycle: its first norm and 29 estimates
+        assert np.all(np.diff(hist[c0:c0 + 30]) <= 1e-12 * hist[0])
+    xr = grounded_solve(g, b)
+    assert np.linalg.norm((x - x.mean()) - xr) <= 1e-9 * np.linalg.norm(xr)
+    assert hist[-1] <= 1e-13 * hist[0]
+    # PIPEFCG runs as FCG
+    _, i2, h2 = O.OracleSolver(g, O.default_options(outer_ksp_type=_ffi.KSP_FCG, **kw)).solve(b)
+    _, i3, h3 = O.OracleSolver(g, O.default_options(outer_ksp_type=_ffi.KSP_PIPEFCG, **kw)).solve(b)
+    assert np.array_equal(h2, h3)  # PIPEFCG runs as FCG
+
+
+def test_fgmres_per_block():
+    gs = [graphs.barabasi_albert(n, 4, seed=s) for s, n in enumerate([900, 300, 1500])]
+    u = graphs.union(gs)
+    b = graphs.rhs(u.n, 1, 2, comp_ptr=u.comp_ptr)
+    x, info, hist = O.OracleSolver(u, O.default_options(outer_ksp_type=_ffi.KSP_FGMRES),
+                                   comp_ptr=u.comp_ptr).solve(b)
+    assert info.status == _ffi.CONVERGED
+    for c, gc in enumerate(gs):
+        lo, hi = u.comp_ptr[c], u.comp_ptr[c + 1]
+        xr = grounded_solve(gc, b[lo:hi])
+        xc = x[lo:hi] - x[lo:hi].mean()
+        assert np.linalg.norm(xc - xr) <= 1e-6 * np.linalg.norm(xr)
 
 
 def test_vcycle_is_symmetric_positive():
