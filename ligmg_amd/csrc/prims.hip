@@ -92,10 +92,10 @@ int64_t exclusive_scan_impl(const T *in, T *out, int64_t n, hipStream_t s) {
 
 // ---------------------------------------------------------------- radix sort
 constexpr int RS_BLOCK = 256;
-constexpr int RS_ITEMS = 16;  // per lane
+constexpr int RS_ITEMS = 8;  // per lane
 constexpr int RS_WAVES = RS_BLOCK / WAVE;
-constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;  // 4096
-constexpr int RS_WTILE = WAVE * RS_ITEMS;     // 1024 items per wave
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;  // 2048
+constexpr int RS_WTILE = WAVE * RS_ITEMS;     // 512 items per wave
 
 __global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const uint64_t *keys, int64_t n, int shift,
                                                      int32_t *hist, int64_t nblocks) {
@@ -111,15 +111,24 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const uint64_t *keys, int6
   hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// One tile per workgroup.  Each wavefront ranks its items by digit with
+// ballots (stable: wave order, then item order); the tile is then reordered
+// by digit in LDS and written with consecutive lanes on consecutive
+// addresses of each digit's run, so the scattered writes coalesce.
 __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const uint64_t *kin, const uint64_t *vin,
                                                         uint64_t *kout, uint64_t *vout, int64_t n,
                                                         int shift, const int32_t *hoff,
                                                         int64_t nblocks) {
   __shared__ int32_t cnt[RS_WAVES][256];
+  __shared__ int32_t tstart[256];
+  __shared__ int32_t wsum[RS_WAVES];
+  __shared__ uint64_t stage[RS_TILE];
+  __shared__ uint8_t sdig[RS_TILE];
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
   for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_BLOCK) (&cnt[0][0])[i] = 0;
   __syncthreads();
-  const int64_t wbase = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * RS_WTILE;
+  const int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
+  const int64_t wbase = tbase + (int64_t)w * RS_WTILE;
   uint64_t k[RS_ITEMS], v[RS_ITEMS];
   int32_t loc[RS_ITEMS];
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -146,24 +155,54 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const uint64_t *kin, co
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
-  // per-digit offsets of each wave inside the tile
-  __shared__ int32_t woff[RS_WAVES][256];
+  // cnt becomes each wave's offset inside its digit's run; tstart the
+  // digit's first position in the tile (exclusive scan over digits)
   {
-    int d = threadIdx.x;
+    const int d = threadIdx.x;
     int32_t acc = 0;
     for (int ww = 0; ww < RS_WAVES; ww++) {
-      woff[ww][d] = acc;
-      acc += cnt[ww][d];
+      const int32_t c = cnt[ww][d];
+      cnt[ww][d] = acc;
+      acc += c;
     }
+    int32_t x = acc;
+    for (int off = 1; off < WAVE; off <<= 1) {
+      const int32_t y = __shfl_up(x, off, WAVE);
+      if (lane >= off) x += y;
+    }
+    if (lane == WAVE - 1) wsum[w] = x;
+    __syncthreads();
+    int32_t add = 0;
+    for (int ww = 0; ww < w; ww++) add += wsum[ww];
+    tstart[d] = x - acc + add;
+  }
+  __syncthreads();
+  const int tn = (int)min<int64_t>(RS_TILE, n - tbase);
+  int32_t lp[RS_ITEMS];
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; it++) {
+    if (loc[it] < 0) {
+      lp[it] = -1;
+      continue;
+    }
+    const uint32_t d = (uint32_t)((k[it] >> shift) & 255);
+    lp[it] = tstart[d] + cnt[w][d] + loc[it];
+    stage[lp[it]] = k[it];
+    sdig[lp[it]] = (uint8_t)d;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < tn; i += RS_BLOCK) {
+    const int d = sdig[i];
+    kout[(int64_t)hoff[(int64_t)d * nblocks + blockIdx.x] + (i - tstart[d])] = stage[i];
   }
   __syncthreads();
 #pragma unroll
-  for (int it = 0; it < RS_ITEMS; it++) {
-    if (loc[it] < 0) continue;
-    uint32_t d = (uint32_t)((k[it] >> shift) & 255);
-    int64_t pos = (int64_t)hoff[(int64_t)d * nblocks + blockIdx.x] + woff[w][d] + loc[it];
-    kout[pos] = k[it];
-    vout[pos] = v[it];
+  for (int it = 0; it < RS_ITEMS; it++)
+    if (lp[it] >= 0) stage[lp[it]] = v[it];
+  __syncthreads();
+  for (int i = threadIdx.x; i < tn; i += RS_BLOCK) {
+    const int d = sdig[i];
+    vout[(int64_t)hoff[(int64_t)d * nblocks + blockIdx.x] + (i - tstart[d])] = stage[i];
   }
 }
 
