@@ -210,6 +210,12 @@ int ligmg_destroy(ligmg_handle h);
  * receives, for every internal position, the caller's vertex index. */
 int ligmg_ordering(ligmg_handle h, int32_t *perm);
 
+/* (new) Device memory the library keeps for reuse: temporaries of setup and
+ * of freed solvers are cached by size class (at most 32 GiB) instead of
+ * returned to HIP, which makes repeated setups in one process as fast as
+ * the first.  This returns all of it to HIP; the library never needs it. */
+int ligmg_trim_memory(void);
+
 /* Thread-local message of the last failed call ("" if none). */
 const char *ligmg_last_error(void);
 

@@ -8,4 +8,4 @@ from . import graphs  # noqa: F401
 from ._ffi import (KSP_CG, KSP_FCG, PREC_FP64, PREC_MIXED, SMOOTHER_CHEBYSHEV,  # noqa: F401
                    SMOOTHER_JACOBI)
 from .solver import (ConvergenceInfo, LigmgError, Options, Solver,  # noqa: F401
-                     aggregate_with_strength, default_options, ligmg)
+                     aggregate_with_strength, default_options, ligmg, trim_memory)

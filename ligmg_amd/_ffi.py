@@ -91,6 +91,7 @@ SOLVER_PROTOS = {
     "ligmg_last_error": (C.c_char_p, []),
     "ligmg_num_levels": (C.c_int, [HANDLE, P(i32)]),
     "ligmg_tail_start": (C.c_int, [HANDLE, P(i32)]),
+    "ligmg_trim_memory": (C.c_int, []),
     "ligmg_dump_hierarchy": (C.c_int, [HANDLE, C.c_char_p]),
     "ligmg_load_hierarchy": (C.c_int, [C.c_char_p, P(ligmg_options), P(HANDLE)]),
     "ligmg_dump_csv": (C.c_int, [HANDLE, C.c_char_p, P(ligmg_info), C.c_char_p, i64]),

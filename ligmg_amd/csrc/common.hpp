@@ -36,7 +36,14 @@ struct Error : std::runtime_error {
 
 constexpr int WAVE = 64;
 
-// Owning device allocation.
+// Pooled device memory (pool.hip): blocks come back from dev_free as
+// pending and are reused after the queued work that may read them is done;
+// dev_pool_trim returns every cached block to HIP.
+void *dev_alloc(size_t bytes);
+void dev_free(void *p);
+void dev_pool_trim();
+
+// Owning device allocation (pooled).
 template <class T>
 struct DevBuf {
   T *p = nullptr;
@@ -64,11 +71,12 @@ struct DevBuf {
     release();
     n = count;
     if (count) {
-      hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
-      if (e != hipSuccess) {
+      try {
+        p = (T *)dev_alloc(count * sizeof(T));
+      } catch (...) {
         p = nullptr;
         n = 0;
-        throw Error(4, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        throw;
       }
     }
   }
@@ -76,7 +84,7 @@ struct DevBuf {
     if (count > n) alloc(count);
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p);
     p = nullptr;
     n = 0;
   }

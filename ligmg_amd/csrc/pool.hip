@@ -1,0 +1,140 @@
+// Device memory pool behind DevBuf.
+//
+// Setup allocates and frees thousands of temporaries (sort buffers, maps,
+// per-level scratch).  hipFree synchronises and unmaps; once a process has
+// released a large solver, each such free costs up to tens of milliseconds,
+// which doubled the second setup's time.  Released blocks are kept here by
+// size class instead.  A released block may still be read by work already
+// queued on some stream, so it is parked as pending and becomes reusable
+// only after one device-wide synchronisation - the guarantee hipFree's own
+// synchronisation gave.  No allocation happens inside a captured graph.
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../include/ligmg_amd.h"
+#include "common.hpp"
+
+namespace lg {
+
+namespace {
+
+// bytes kept for reuse at most; above it released blocks go back to HIP
+constexpr size_t CACHE_MAX = (size_t)32 << 30;
+
+struct Pool {
+  std::mutex m;
+  using Key = std::pair<int, size_t>;  // (device, class bytes)
+  std::map<Key, std::vector<void *>> ready, pending;
+  std::unordered_map<void *, Key> live;
+  size_t cached = 0;
+};
+
+// never destroyed: blocks live until the process ends, after the HIP
+// runtime may already be gone
+Pool &pool() {
+  static Pool *p = new Pool();
+  return *p;
+}
+
+// classes 2^k and 3 * 2^(k-2): at most a third of a block unused
+size_t size_class(size_t b) {
+  size_t p2 = 256;
+  while (p2 < b) p2 <<= 1;
+  const size_t three_quarters = (p2 >> 1) + (p2 >> 2);
+  return (p2 > 256 && b <= three_quarters) ? three_quarters : p2;
+}
+
+void flush_locked(Pool &P) {
+  (void)hipDeviceSynchronize();
+  for (auto *lists : {&P.ready, &P.pending})
+    for (auto &kv : *lists)
+      for (void *q : kv.second) (void)hipFree(q);
+  P.ready.clear();
+  P.pending.clear();
+  P.cached = 0;
+}
+
+}  // namespace
+
+void *dev_alloc(size_t bytes) {
+  if (bytes == 0) return nullptr;
+  int dev = 0;
+  LG_HIP(hipGetDevice(&dev));
+  const size_t cls = size_class(bytes);
+  const Pool::Key key{dev, cls};
+  Pool &P = pool();
+  std::lock_guard<std::mutex> g(P.m);
+  auto it = P.ready.find(key);
+  if (it == P.ready.end() || it->second.empty()) {
+    auto pt = P.pending.find(key);
+    if (pt != P.pending.end() && !pt->second.empty()) {
+      // this device's pending blocks are free once its queued work is done
+      LG_HIP(hipDeviceSynchronize());
+      for (auto q = P.pending.begin(); q != P.pending.end();) {
+        if (q->first.first != dev) {
+          ++q;
+          continue;
+        }
+        auto &dst = P.ready[q->first];
+        dst.insert(dst.end(), q->second.begin(), q->second.end());
+        q = P.pending.erase(q);
+      }
+      it = P.ready.find(key);
+    }
+  }
+  void *p = nullptr;
+  if (it != P.ready.end() && !it->second.empty()) {
+    p = it->second.back();
+    it->second.pop_back();
+    P.cached -= cls;
+  } else {
+    hipError_t e = hipMalloc(&p, cls);
+    if (e != hipSuccess) {  // give the cache back and try once more
+      (void)hipGetLastError();
+      flush_locked(P);
+      e = hipMalloc(&p, cls);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(LIGMG_ERR_NOMEM, "hipMalloc of " + std::to_string(cls) + " bytes failed");
+    }
+  }
+  P.live[p] = key;
+  return p;
+}
+
+void dev_free(void *p) {
+  if (!p) return;
+  Pool &P = pool();
+  std::lock_guard<std::mutex> g(P.m);
+  auto it = P.live.find(p);
+  if (it == P.live.end()) return;
+  const Pool::Key key = it->second;
+  P.live.erase(it);
+  if (P.cached + key.second > CACHE_MAX) {
+    (void)hipFree(p);
+    return;
+  }
+  P.pending[key].push_back(p);
+  P.cached += key.second;
+}
+
+void dev_pool_trim() {
+  Pool &P = pool();
+  std::lock_guard<std::mutex> g(P.m);
+  flush_locked(P);
+}
+
+}  // namespace lg
+
+int ligmg_trim_memory(void) {
+  try {
+    lg::dev_pool_trim();
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return e.code;
+  }
+}

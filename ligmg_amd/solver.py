@@ -373,3 +373,9 @@ def aggregate_with_strength(g, S, opts=None):
         ind.ctypes.data_as(P(C.c_int32)), st.ctypes.data_as(P(C.c_int32)), C.byref(nc))
     _check(rc, "ligmg_aggregate_with_strength")
     return agg, ind, st, nc.value
+
+
+def trim_memory():
+    """Return the device memory the library caches for reuse (temporaries of
+    setup, buffers of closed solvers) to HIP."""
+    _check(_ffi.solver_lib().ligmg_trim_memory(), "ligmg_trim_memory")
