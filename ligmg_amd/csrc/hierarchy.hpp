@@ -21,6 +21,20 @@ namespace lg {
 constexpr int UNIT_NNZ = 2048;
 constexpr int UNIT_ROWS = 512;
 constexpr int SPMV_BLOCK = 256;
+// units per SpMV workgroup: the colind/value loads of the next unit are in
+// flight while the rows of the current one are reduced
+constexpr int SPMV_UPW = 2;
+
+// Lanes that reduce one row of a complete-row unit: the smallest power of
+// two K <= 64 with 4K >= the unit's mean row length.  K = 1 sums a row left
+// to right; otherwise lane g of the row's group sums positions g, g + K, ...
+// left to right and the K partials meet in an xor butterfly (off = K/2..1).
+__host__ __device__ inline int row_lanes(int64_t cnt, int64_t nr) {
+  const int64_t avg = (cnt + nr - 1) / nr;
+  int K = 1;
+  while (K < 64 && 4 * (int64_t)K < avg) K <<= 1;
+  return K;
+}
 
 struct Units {
   int64_t nunits = 0, nlong = 0, nparts = 0;

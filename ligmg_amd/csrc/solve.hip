@@ -30,40 +30,30 @@ struct EpiAccFinal {
   __device__ __forceinline__ void operator()(int64_t i, double s) const { e(i, acc[i] + s); }
 };
 
-template <class T>
-__device__ __forceinline__ T block_sum(T v) {
-  __shared__ T ws[SPMV_BLOCK / WAVE];
-  v = wave_sum(v);
-  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-  __syncthreads();
-  if (lane == 0) ws[w] = v;
-  __syncthreads();
-  T s = 0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < SPMV_BLOCK / WAVE; i++) s += ws[i];
-  return s;
-}
-
-// One workgroup per unit.  Complete-row units stage every product of the
-// unit in LDS with coalesced loads of colind/values (the x gathers are the
-// only scattered accesses), then reduce rows from LDS: one thread per row
-// when rows average at most 8 entries, one wavefront per row otherwise.
-// Long-row chunks write a partial sum that k_spmv_long combines.
+// Each workgroup takes SPMV_UPW consecutive units.  A complete-row unit
+// stages every product in LDS with coalesced loads of colind/values (the x
+// gathers are the only scattered accesses); its rows are then reduced from
+// LDS by groups of row_lanes() lanes while the next unit's colind/values
+// loads are in flight.  A long-row chunk writes a partial sum (8 products
+// per thread added in order, butterfly per wavefront, the wavefront sums
+// added from 0) that k_spmv_long combines.
 template <class T, class Val, class Epi>
 __global__ void __launch_bounds__(SPMV_BLOCK)
     k_spmv_units(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, Val val,
                  const T *__restrict__ x, const int32_t *__restrict__ urow0,
-                 const int32_t *__restrict__ unrows, const int32_t *__restrict__ ukoff,
+                 const int32_t *__restrict__ unrows, const int32_t *__restrict__ ukoff, int64_t nunits,
                  double *__restrict__ parts, Epi epi) {
   constexpr int IT = UNIT_NNZ / SPMV_BLOCK;
   __shared__ double prod[UNIT_NNZ];
-  const int u = blockIdx.x;
-  const int row0 = urow0[u], nr = unrows[u], k0 = ukoff[u];
-  // all column/value loads of a lane first, then all gathers, then products:
-  // IT independent gathers in flight per lane
-  auto stage = [&](int cnt, double *out) {
-    int col[IT];
-    T vv[IT];
+  __shared__ double ws[SPMV_BLOCK / WAVE];
+  const int64_t ubeg = (int64_t)blockIdx.x * SPMV_UPW;
+  const int64_t uend = min(ubeg + SPMV_UPW, nunits);
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  int col[IT];
+  T vv[IT];
+  auto fetch = [&](int64_t u) {
+    const int row0 = urow0[u], nr = unrows[u], k0 = ukoff[u];
+    const int cnt = nr > 0 ? rp[row0 + nr] - k0 : min(k0 + UNIT_NNZ, rp[row0 + 1]) - k0;
 #pragma unroll
     for (int j = 0; j < IT; j++) {
       const int t = threadIdx.x + j * SPMV_BLOCK;
@@ -74,51 +64,66 @@ __global__ void __launch_bounds__(SPMV_BLOCK)
       const int t = threadIdx.x + j * SPMV_BLOCK;
       vv[j] = (t < cnt) ? val(k0 + t) : (T)0;
     }
-    double xx[IT];
-#pragma unroll
-    for (int j = 0; j < IT; j++) xx[j] = (col[j] >= 0) ? (double)x[col[j]] : 0.0;
-#pragma unroll
-    for (int j = 0; j < IT; j++) out[j] = (double)vv[j] * xx[j];
   };
-  if (nr > 0) {
-    const int cnt = rp[row0 + nr] - k0;
+  fetch(ubeg);
+  for (int64_t u = ubeg; u < uend; u++) {
+    const int row0 = urow0[u], nr = unrows[u], k0 = ukoff[u];
+    const int cnt = nr > 0 ? rp[row0 + nr] - k0 : min(k0 + UNIT_NNZ, rp[row0 + 1]) - k0;
+    // all gathers of a lane in flight before any product
+    double pr[IT];
     {
-      double pr[IT];
-      stage(cnt, pr);
+      double xx[IT];
+#pragma unroll
+      for (int j = 0; j < IT; j++) xx[j] = (col[j] >= 0) ? (double)x[col[j]] : 0.0;
+#pragma unroll
+      for (int j = 0; j < IT; j++) pr[j] = (double)vv[j] * xx[j];
+    }
+    if (nr > 0) {
+      if (u > ubeg) __syncthreads();  // the previous unit's rows are read
 #pragma unroll
       for (int j = 0; j < IT; j++) {
         const int t = threadIdx.x + j * SPMV_BLOCK;
         if (t < cnt) prod[t] = pr[j];
       }
-    }
-    __syncthreads();
-    if (cnt <= 8 * nr) {
-      for (int r = threadIdx.x; r < nr; r += SPMV_BLOCK) {
-        const int a = rp[row0 + r] - k0, b = rp[row0 + r + 1] - k0;
-        double s = 0;
-        for (int t = a; t < b; t++) s += prod[t];
-        epi(row0 + r, s);
+      if (u + 1 < uend) fetch(u + 1);
+      __syncthreads();
+      const int K = row_lanes(cnt, nr);
+      if (K == 1) {
+        for (int r = threadIdx.x; r < nr; r += SPMV_BLOCK) {
+          const int a = rp[row0 + r] - k0, b = rp[row0 + r + 1] - k0;
+          double s = 0;
+          for (int t = a; t < b; t++) s += prod[t];
+          epi(row0 + r, s);
+        }
+      } else {
+        // the whole workgroup runs the same trip count: butterflies stay converged
+        const int g = threadIdx.x & (K - 1), ng = SPMV_BLOCK / K;
+        for (int r0 = 0; r0 < nr; r0 += ng) {
+          const int r = r0 + threadIdx.x / K;
+          double s = 0;
+          if (r < nr) {
+            const int a = rp[row0 + r] - k0, b = rp[row0 + r + 1] - k0;
+            for (int t = a + g; t < b; t += K) s += prod[t];
+          }
+          s = group_sum(s, K);
+          if (g == 0 && r < nr) epi(row0 + r, s);
+        }
       }
     } else {
-      const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-      for (int r = w; r < nr; r += SPMV_BLOCK / WAVE) {
-        const int a = rp[row0 + r] - k0, b = rp[row0 + r + 1] - k0;
-        double s = 0;
-        for (int t = a + lane; t < b; t += WAVE) s += prod[t];
-        s = wave_sum(s);
-        if (lane == 0) epi(row0 + r, s);
+      double s = 0;
+#pragma unroll
+      for (int j = 0; j < IT; j++) s += pr[j];
+      if (u + 1 < uend) fetch(u + 1);
+      s = wave_sum(s);
+      __syncthreads();
+      if (lane == 0) ws[w] = s;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0;
+        for (int i = 0; i < SPMV_BLOCK / WAVE; i++) t += ws[i];
+        parts[-nr] = t;
       }
     }
-  } else {
-    const int row = row0;
-    const int cnt = min(k0 + UNIT_NNZ, rp[row + 1]) - k0;
-    double pr[IT];
-    stage(cnt, pr);
-    double s = 0;
-#pragma unroll
-    for (int j = 0; j < IT; j++) s += pr[j];
-    s = block_sum(s);
-    if (threadIdx.x == 0) parts[-nr] = s;
   }
 }
 
@@ -137,8 +142,9 @@ template <class T, class Val, class Epi>
 void launch_spmv(const DCsr &A, Val val, const T *x, Epi epi, hipStream_t s) {
   if (A.U.nunits == 0) return;
   double *parts = A.U.parts.p;
-  hipLaunchKernelGGL((k_spmv_units<T, Val, Epi>), dim3((unsigned)A.U.nunits), dim3(SPMV_BLOCK), 0, s,
-                     A.rp.p, A.ci.p, val, x, A.U.row0.p, A.U.nrows.p, A.U.koff.p, parts, epi);
+  const int64_t grid = (A.U.nunits + SPMV_UPW - 1) / SPMV_UPW;
+  hipLaunchKernelGGL((k_spmv_units<T, Val, Epi>), dim3((unsigned)grid), dim3(SPMV_BLOCK), 0, s, A.rp.p,
+                     A.ci.p, val, x, A.U.row0.p, A.U.nrows.p, A.U.koff.p, A.U.nunits, parts, epi);
   LG_LAUNCH_CHECK();
   if (A.U.nlong) {
     hipLaunchKernelGGL((k_spmv_long<Epi>), dim3(grid_for(A.U.nlong, 64)), dim3(64), 0, s, A.U.lrow.p,

@@ -86,6 +86,12 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// xor butterfly over aligned groups of K lanes (K a power of two <= 64)
+__device__ __forceinline__ double group_sum(double v, int K) {
+  for (int off = K >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+
 // ------------------------------------------------- CG / FCG scalar logic
 // One set of scalars per (block, column): the Krylov coefficients of each
 // independent graph are kept apart.

@@ -788,10 +788,10 @@ struct Solver {
       op.d = A.d32.p;
     }
     op.trow = S.trow.p;
-    op.wrow = S.wrow.p;
+    op.grow = S.grow.p;
+    for (int c = 0; c <= TAIL_GCLASSES; c++) op.gofs[c] = S.gofs[c];
     op.chunk = S.chunk.p;
     op.ntrow = S.ntrow;
-    op.nwrow = S.nwrow;
     op.nchunk = S.nchunk;
     op.lrow = A.U.lrow.p;
     op.lpart = A.U.lpart.p;

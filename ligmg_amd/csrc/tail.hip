@@ -21,11 +21,11 @@ static_assert(SPMV_BLOCK == 256, "the replayed trees are those of 256-thread wor
 
 // ------------------------------------------------------------------ SpMV
 // k_spmv_units + k_spmv_long.  Thread rows: one thread sums the row left to
-// right (entries loaded four at a time, added in order).  Wavefront rows:
-// 64 strided partials, xor butterfly.  Long rows: each UNIT_NNZ-chunk is the
-// tree of a 256-thread workgroup (8 products per thread added in order,
-// butterfly per wavefront, the 4 wavefront sums added from 0) on one group;
-// the chunk sums are then added in order.
+// right (entries loaded four at a time, added in order).  Group rows: K
+// strided partials, xor butterfly over the K lanes.  Long rows: each
+// UNIT_NNZ-chunk is the tree of a 256-thread workgroup (8 products per
+// thread added in order, butterfly per wavefront, the 4 wavefront sums
+// added from 0) on one group; the chunk sums are then added in order.
 template <class T, class Val, class Epi>
 __device__ __forceinline__ void t_spmv(const TailOp<T> &A, Val val, const T *__restrict__ x, Epi epi,
                                        double *ws) {
@@ -48,14 +48,23 @@ __device__ __forceinline__ void t_spmv(const TailOp<T> &A, Val val, const T *__r
     for (; k < b; k++) s += (double)val(k) * (double)x[ci[k]];
     epi(r, s);
   }
-  const int lane = threadIdx.x & (WAVE - 1);
-  for (int j = threadIdx.x / WAVE; j < A.nwrow; j += NW) {
-    const int r = A.wrow[j];
-    double s = 0;
-    for (int k = rp[r] + lane; k < rp[r + 1]; k += WAVE) s += (double)val(k) * (double)x[ci[k]];
-    s = wave_sum(s);
-    if (lane == 0) epi(r, s);
+  for (int c = 0; c < TAIL_GCLASSES; c++) {
+    const int K = 2 << c, lo = A.gofs[c], hi = A.gofs[c + 1];
+    const int g = threadIdx.x & (K - 1), ng = TAIL_THREADS / K;
+    // uniform trip count: the butterflies stay converged
+    for (int j0 = lo; j0 < hi; j0 += ng) {
+      const int j = j0 + threadIdx.x / K;
+      double s = 0;
+      int r = 0;
+      if (j < hi) {
+        r = A.grow[j];
+        for (int k = rp[r] + g; k < rp[r + 1]; k += K) s += (double)val(k) * (double)x[ci[k]];
+      }
+      s = group_sum(s, K);
+      if (g == 0 && j < hi) epi(r, s);
+    }
   }
+  const int lane = threadIdx.x & (WAVE - 1);
   if (A.nlong == 0) return;
   const int grp = threadIdx.x / SPMV_BLOCK, vt = threadIdx.x % SPMV_BLOCK, w = vt / WAVE;
   for (int j0 = 0; j0 < A.nchunk; j0 += NG) {
@@ -330,13 +339,16 @@ void tail_schedule(const DCsr &A, TailSched &S, hipStream_t s) {
   const std::vector<int32_t> row0 = download(U.row0.p, U.nunits, s);
   const std::vector<int32_t> nrows = download(U.nrows.p, U.nunits, s);
   const std::vector<int32_t> koff = download(U.koff.p, U.nunits, s);
-  std::vector<int32_t> tr, wr, ch;
+  std::vector<int32_t> tr, ch;
+  std::vector<int32_t> gr[TAIL_GCLASSES];
   for (int64_t u = 0; u < U.nunits; u++) {
     const int32_t nr = nrows[u];
     if (nr > 0) {
-      // k_spmv_units: thread per row when the unit averages <= 8 entries
-      const int64_t cnt = rp[row0[u] + nr] - koff[u];
-      auto &dst = (cnt <= 8 * (int64_t)nr) ? tr : wr;
+      // the lanes k_spmv_units gives each row of the unit
+      const int K = row_lanes(rp[row0[u] + nr] - koff[u], nr);
+      int c = 0;
+      while ((2 << c) < K) c++;
+      auto &dst = K == 1 ? tr : gr[c];
       for (int32_t r = 0; r < nr; r++) dst.push_back(row0[u] + r);
     } else {
       ch.push_back(row0[u]);
@@ -344,14 +356,19 @@ void tail_schedule(const DCsr &A, TailSched &S, hipStream_t s) {
       ch.push_back(-nr);
     }
   }
+  std::vector<int32_t> all;
+  for (int c = 0; c < TAIL_GCLASSES; c++) {
+    S.gofs[c] = (int32_t)all.size();
+    all.insert(all.end(), gr[c].begin(), gr[c].end());
+  }
+  S.gofs[TAIL_GCLASSES] = (int32_t)all.size();
   S.ntrow = (int32_t)tr.size();
-  S.nwrow = (int32_t)wr.size();
   S.nchunk = (int32_t)(ch.size() / 3);
   S.trow.alloc(tr.size());
-  S.wrow.alloc(wr.size());
+  S.grow.alloc(all.size());
   S.chunk.alloc(ch.size());
   to_device(S.trow.p, tr.data(), tr.size(), s);
-  to_device(S.wrow.p, wr.data(), wr.size(), s);
+  to_device(S.grow.p, all.data(), all.size(), s);
   to_device(S.chunk.p, ch.data(), ch.size(), s);
   LG_HIP(hipStreamSynchronize(s));
 }

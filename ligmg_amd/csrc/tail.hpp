@@ -28,6 +28,8 @@
 namespace lg {
 
 constexpr int TAIL_THREADS = 1024;
+// classes of rows reduced by K = 2, 4, ..., 64 lanes
+constexpr int TAIL_GCLASSES = 6;
 
 // One CSR operator with the row schedule of its SpMV units: rows of units
 // reduced by one thread, rows reduced by one wavefront, and the
@@ -37,11 +39,14 @@ struct TailOp {
   const int32_t *rp = nullptr, *ci = nullptr;
   const T *v = nullptr;       // null: unit values
   const double *d = nullptr;  // FP64 diagonal term of an FP32 Laplacian-form operator
-  const int32_t *trow = nullptr, *wrow = nullptr;
+  // rows reduced by one thread, and rows reduced by groups of K = 2 << c
+  // lanes (grow[gofs[c] .. gofs[c + 1]]), as row_lanes() assigns them
+  const int32_t *trow = nullptr, *grow = nullptr;
   const int32_t *chunk = nullptr;  // (row, first position, slot) per chunk
   const int32_t *lrow = nullptr, *lpart = nullptr;
   double *parts = nullptr;
-  int32_t ntrow = 0, nwrow = 0, nchunk = 0, nlong = 0;
+  int32_t ntrow = 0, nchunk = 0, nlong = 0;
+  int32_t gofs[TAIL_GCLASSES + 1] = {};
 };
 
 // Per-level data of the Krylov instructions (RedPlan + KWork).
@@ -90,8 +95,9 @@ struct TailParams {
 
 // Host-side owner of an operator's row schedule.
 struct TailSched {
-  DevBuf<int32_t> trow, wrow, chunk;
-  int32_t ntrow = 0, nwrow = 0, nchunk = 0;
+  DevBuf<int32_t> trow, grow, chunk;
+  int32_t ntrow = 0, nchunk = 0;
+  int32_t gofs[TAIL_GCLASSES + 1] = {};
 };
 
 // Row schedule of A from its SpMV units (build_units must have run).
