@@ -46,6 +46,8 @@ struct Units {
   DevBuf<double> parts;   // partial sums (also used as float storage)
 };
 
+struct HugeRows;  // rows too long for one wavefront of the setup row kernels
+
 struct DCsr {
   int64_t n = 0, ncols = 0, nnz = 0;
   DevBuf<int32_t> rp, ci;
@@ -62,6 +64,9 @@ struct DCsr {
   // stays in L2.  Empty when the operator is used unsliced.
   std::vector<std::unique_ptr<DCsr>> slices[2];
   DevBuf<double> acc;  // FP64 row accumulator across slice passes
+  // long-row table of the setup row kernels, built on first use from rp
+  // (a DCsr's pattern does not change once built)
+  mutable std::shared_ptr<HugeRows> huge;
 };
 
 struct DLevel {

@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const T *in, int64_t
 
 template <class T>
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const T *in, T *out, int64_t n,
-                                                          const T *boff) {
+                                                          const T *boff, int64_t *total) {
   int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
   T v[SCAN_ITEMS];
   T s = 0;
@@ -60,34 +60,41 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const T *in, T *out, 
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; i++) {
     if (base + i < n) out[base + i] = pre;
+    if (total && base + i == n - 1) *total = (int64_t)pre + (int64_t)v[i];  // inclusive total
     pre += v[i];
   }
+}
+
+// Stream-ordered scan; the element that holds in[n-1] also writes the
+// inclusive total to *total when total is not null.
+template <class T>
+void exclusive_scan_async(const T *in, T *out, int64_t n, int64_t *total, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb == 1) {
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(SCAN_BLOCK), 0, s, in, out, n, (const T *)nullptr,
+                       total);
+    LG_LAUNCH_CHECK();
+    return;
+  }
+  DevBuf<T> bsum(nb);
+  hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, bsum.p);
+  LG_LAUNCH_CHECK();
+  exclusive_scan_async<T>(bsum.p, bsum.p, nb, nullptr, s);
+  hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, out, n,
+                     (const T *)bsum.p, total);
+  LG_LAUNCH_CHECK();
 }
 
 template <class T>
 int64_t exclusive_scan_impl(const T *in, T *out, int64_t n, hipStream_t s) {
   if (n <= 0) return 0;
-  int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-  // last element kept to form the total (out may alias in)
-  T last_in = 0;
-  LG_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(T), hipMemcpyDeviceToHost, s));
-  if (nb == 1) {
-    hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(SCAN_BLOCK), 0, s, in, out, n, (const T *)nullptr);
-    LG_LAUNCH_CHECK();
-  } else {
-    DevBuf<T> bsum(nb);
-    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, bsum.p);
-    LG_LAUNCH_CHECK();
-    exclusive_scan_impl<T>(bsum.p, bsum.p, nb, s);
-    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, out, n,
-                       (const T *)bsum.p);
-    LG_LAUNCH_CHECK();
-    LG_HIP(hipStreamSynchronize(s));
-  }
-  T last_out = 0;
-  LG_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(T), hipMemcpyDeviceToHost, s));
+  DevBuf<int64_t> tot(1);
+  exclusive_scan_async<T>(in, out, n, tot.p, s);
+  int64_t h = 0;
+  to_host(&h, tot.p, 1, s);
   LG_HIP(hipStreamSynchronize(s));
-  return (int64_t)last_out + (int64_t)last_in;
+  return h;
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -297,7 +304,7 @@ void radix_sort_pairs(uint64_t *keys, uint64_t *vals, int64_t n, int bits, hipSt
     int shift = 8 * p;
     hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nb), dim3(RS_BLOCK), 0, s, ka, n, shift, hist.p, nb);
     LG_LAUNCH_CHECK();
-    exclusive_scan_i32(hist.p, hist.p, 256 * nb, s);
+    exclusive_scan_async<int32_t>(hist.p, hist.p, 256 * nb, nullptr, s);
     hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(RS_BLOCK), 0, s, ka, va, kb, vb, n,
                        shift, hist.p, nb);
     LG_LAUNCH_CHECK();
@@ -308,7 +315,8 @@ void radix_sort_pairs(uint64_t *keys, uint64_t *vals, int64_t n, int bits, hipSt
     d2d(keys, ka, n, s);
     d2d(vals, va, n, s);
   }
-  LG_HIP(hipStreamSynchronize(s));
+  // no synchronisation: the temporaries go back to the device pool, which
+  // reuses them only after the device has finished this work
 }
 
 int64_t segmented_canon_sum(const uint64_t *keys, const uint64_t *vals, int64_t n,
@@ -327,7 +335,6 @@ int64_t segmented_canon_sum(const uint64_t *keys, const uint64_t *vals, int64_t 
   hipLaunchKernelGGL(k_seg_sum, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, keys, vals,
                      start.p, nseg, n, out_keys, out_sums);
   LG_LAUNCH_CHECK();
-  LG_HIP(hipStreamSynchronize(s));
   return nseg;
 }
 
@@ -337,7 +344,6 @@ int64_t compact_flags(const int32_t *flags, int64_t n, int32_t *out, hipStream_t
   int64_t cnt = exclusive_scan_i32(flags, pos.p, n, s);
   hipLaunchKernelGGL(k_compact_write, dim3(grid_for(n, 256)), dim3(256), 0, s, flags, pos.p, n, out);
   LG_LAUNCH_CHECK();
-  LG_HIP(hipStreamSynchronize(s));
   return cnt;
 }
 

@@ -1,6 +1,6 @@
 // Deterministic device primitives: scan, stable LSD radix sort, canonical
 // segmented sums.  All run on one stream; functions that return a host value
-// synchronise that stream.
+// synchronise that stream, the others return with their work queued.
 #pragma once
 #include "common.hpp"
 

@@ -103,12 +103,16 @@ __global__ void k_rows_huge(int64_t nrows, const int32_t *row, const int32_t *cp
   op.finish(row[j], acc);
 }
 
+}  // namespace
+
 // Rows longer than CHUNK of one matrix and their chunks.
 struct HugeRows {
   int64_t nrows = 0, nchunks = 0;
   DevBuf<int32_t> row, cptr, cstart, cend, crow;
   DevBuf<uint8_t> cacc;
 };
+
+namespace {
 
 void find_huge(int64_t n, const int32_t *rp, HugeRows &Hr, hipStream_t s) {
   std::vector<int32_t> h = download(rp, n + 1, s);
@@ -139,6 +143,16 @@ void find_huge(int64_t n, const int32_t *rp, HugeRows &Hr, hipStream_t s) {
   to_device(Hr.crow.p, crow.data(), Hr.nchunks, s);
   Hr.cacc.alloc((size_t)Hr.nchunks * 192);  // room for the largest Acc
   LG_HIP(hipStreamSynchronize(s));
+}
+
+// the long-row table of A, downloaded and built once per matrix
+const HugeRows &huge_of(const DCsr &A, hipStream_t s) {
+  if (!A.huge) {
+    auto h = std::make_shared<HugeRows>();
+    find_huge(A.n, A.rp.p, *h, s);
+    A.huge = std::move(h);
+  }
+  return *A.huge;
 }
 
 template <class Op>
@@ -907,7 +921,6 @@ void row_sums_canonical(int64_t n, const int32_t *rp, const double *v, double *o
   HugeRows Hr;
   find_huge(n, rp, Hr, s);
   run_rows(n, rp, OpRowSum{{}, v, out}, Hr, s);
-  LG_HIP(hipStreamSynchronize(s));
 }
 
 // ----------------------------------------------------------------------------
@@ -954,7 +967,6 @@ void transpose_p(const DCsr &P, int64_t ncols, DCsr &R, hipStream_t s) {
       LG_LAUNCH_CHECK();
     }
   }
-  LG_HIP(hipStreamSynchronize(s));
 }
 
 // Member lists of the aggregates (nc x n, unit values): row c lists the
@@ -973,7 +985,6 @@ void build_members(const int32_t *agg, int64_t n, int64_t nc, DCsr &M, hipStream
   LG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_members_unpack, dim3(g1(n)), dim3(256), 0, s, (int)n, vals.p, M.ci.p);
   LG_LAUNCH_CHECK();
-  LG_HIP(hipStreamSynchronize(s));
 }
 
 // ----------------------------------------------------------------------------
@@ -1003,7 +1014,6 @@ void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, 
     LG_LAUNCH_CHECK();
   }
   d2d(out.v.p, us.p, nu, s);
-  LG_HIP(hipStreamSynchronize(s));
 }
 
 // ----------------------------------------------------------------------------
@@ -1012,8 +1022,7 @@ void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, 
 bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &coarse,
                        hipStream_t s) {
   const int64_t n = A.n;
-  HugeRows Hr;
-  find_huge(n, A.rp.p, Hr, s);
+  const HugeRows &Hr = huge_of(A, s);
   DevBuf<int32_t> deg(n), elim(n);
   run_rows(n, A.rp.p, OpDeg{{}, A.v.p, deg.p}, Hr, s);
   run_rows(n, A.rp.p, OpElimMin{A.ci.p, deg.p, A.v.p, o.elim_degree, elim.p}, Hr, s);
@@ -1107,7 +1116,6 @@ void make_elim32(DLevel &L, hipStream_t s) {
                        L.R.v32.p);
     LG_LAUNCH_CHECK();
   }
-  LG_HIP(hipStreamSynchronize(s));
 }
 
 // ----------------------------------------------------------------------------
@@ -1138,8 +1146,7 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
                       DevBuf<double> &cs, hipStream_t s) {
   const int64_t n = A.n, nnz = A.nnz;
   const int T = o.num_tvs;
-  HugeRows Hr;
-  find_huge(n, A.rp.p, Hr, s);
+  const HugeRows &Hr = huge_of(A, s);
   DevBuf<double> X((size_t)n * T), Y((size_t)n * T), nrm(n), diag(n), raw(nnz), rowmax(n), colmax(n);
   DevBuf<uint8_t> hasd(n);
   DevBuf<int32_t> tpos(nnz), bad(1);
@@ -1159,7 +1166,6 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
     }
     hipLaunchKernelGGL(k_tv_norm, dim3(g1(n)), dim3(256), 0, s, (int)n, T, X.p, nrm.p);
     LG_LAUNCH_CHECK();
-    LG_HIP(hipStreamSynchronize(s));
   }
   run_rows(n, A.rp.p, OpAffinity{{}, A.ci.p, A.v.p, X.p, nrm.p, T, o.strength_metric, raw.p, rowmax.p},
            Hr, s);
@@ -1197,7 +1203,6 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
            s);
   cs.alloc(n);
   run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S.p, cs.p}, Hr, s);
-  LG_HIP(hipStreamSynchronize(s));
 }
 
 // ----------------------------------------------------------------------------
@@ -1206,8 +1211,7 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
 int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg_options &o,
                    DevBuf<int32_t> &agg, DevBuf<int32_t> &ind, DevBuf<int32_t> &state, hipStream_t s) {
   const int64_t n = A.n;
-  HugeRows Hr;
-  find_huge(n, A.rp.p, Hr, s);
+  const HugeRows &Hr = huge_of(A, s);
   state.alloc(n);
   ind.alloc(n);
   DevBuf<int32_t> nstate(n), nind(n), vind(n), vf(n), flag(n), idx(n);
@@ -1237,7 +1241,6 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
       hipLaunchKernelGGL(k_scatter_sums, dim3(grid_for(nu, 256, 8192)), dim3(256), 0, s, nu, uk.p, us.p,
                          nv.p);
       LG_LAUNCH_CHECK();
-      LG_HIP(hipStreamSynchronize(s));
     }
     hipLaunchKernelGGL(k_add_votes, dim3(g1(n)), dim3(256), 0, s, (int)n, nv.p, votes.p);
     LG_LAUNCH_CHECK();
@@ -1264,7 +1267,6 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
       hipLaunchKernelGGL(k_conductance, dim3(g1(n)), dim3(256), 0, s, (int)n, ext.p, in.p, o.rejection_ratio,
                          o.rejection_to_seed ? 1 : 0, state.p, ind.p);
       LG_LAUNCH_CHECK();
-      LG_HIP(hipStreamSynchronize(s));
     }
   }
   hipLaunchKernelGGL(k_self_unassigned, dim3(g1(n)), dim3(256), 0, s, (int)n, ind.p);
@@ -1277,14 +1279,12 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
   agg.alloc(n);
   hipLaunchKernelGGL(k_agg_ids, dim3(g1(n)), dim3(256), 0, s, (int)n, ind.p, cid.p, agg.p);
   LG_LAUNCH_CHECK();
-  LG_HIP(hipStreamSynchronize(s));
   return nc;
 }
 
 void col_sums_of(const DCsr &A, const double *S, DevBuf<double> &cs, hipStream_t s) {
   const int64_t n = A.n;
-  HugeRows Hr;
-  find_huge(n, A.rp.p, Hr, s);
+  const HugeRows &Hr = huge_of(A, s);
   DevBuf<int32_t> tpos(A.nnz), bad(1);
   LG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
   run_rows(n, A.rp.p, OpTpos{{}, A.rp.p, A.ci.p, tpos.p, bad.p}, Hr, s);
@@ -1294,7 +1294,6 @@ void col_sums_of(const DCsr &A, const double *S, DevBuf<double> &cs, hipStream_t
   LG_CHECK(hb == 0, LIGMG_ERR_INVALID, "matrix pattern is not structurally symmetric");
   cs.alloc(n);
   run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S, cs.p}, Hr, s);
-  LG_HIP(hipStreamSynchronize(s));
 }
 
 // ----------------------------------------------------------------------------
@@ -1313,8 +1312,7 @@ void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLe
   {
     DevBuf<uint64_t> keys(A.nnz), vals(A.nnz);
     const int cb = std::max(1, bits_for((uint64_t)nc));
-    HugeRows Hr;
-    find_huge(n, A.rp.p, Hr, s);
+    const HugeRows &Hr = huge_of(A, s);
     run_rows(n, A.rp.p, OpGalAgg{{}, A.ci.p, L.agg.p, A.v.p, cb, keys.p, vals.p}, Hr, s);
     contract_to_csr(keys, vals, A.nnz, nc, nc, coarse, s);
   }
@@ -1330,7 +1328,6 @@ void setup_dinv(const DCsr &A, DLevel &L, bool fp32, hipStream_t s) {
   hipLaunchKernelGGL(k_dinv_rows, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p, A.ci.p,
                      A.v.p, L.dinv.p, fp32 ? L.dinv32.p : nullptr);
   LG_LAUNCH_CHECK();
-  LG_HIP(hipStreamSynchronize(s));
 }
 
 // ----------------------------------------------------------------------------

@@ -421,7 +421,6 @@ struct Solver {
     dsc.alloc((size_t)8 * maxnb);
     setup_tail();
     LG_HIP(hipStreamSynchronize(s));
-    LG_HIP(hipStreamSynchronize(s));
   }
 
   // ---------------------------------------------------------- dump / load
