@@ -517,11 +517,16 @@ __global__ void k_tv_fill(int n, uint64_t seed, int level, int T, const unsigned
 
 // ---- a8: Jacobi on the test vectors, IgnoreDiagSR, b = 0
 //      (smoothing.hpp:11-32, 78-98)
+// accumulators of N test vectors (N >= T; lanes t >= T stay unused)
+template <int N>
 struct TVAcc {
-  double v[MAXT];
+  double v[N];
 };
+// One weighted Jacobi sweep on the T test vectors, N = the accumulator width
+// chosen for T at the launch (each vector's sum has the same order for any N).
+template <int N>
 struct OpJacobiTV {
-  using Acc = TVAcc;
+  using Acc = TVAcc<N>;
   const int32_t *ci;
   const double *v, *X, *diag;
   const uint8_t *hasd;
@@ -530,38 +535,40 @@ struct OpJacobiTV {
   __device__ Acc zero() const {
     Acc a;
 #pragma unroll
-    for (int t = 0; t < MAXT; t++) a.v[t] = 0.0;
+    for (int t = 0; t < N; t++) a.v[t] = 0.0;
     return a;
   }
   __device__ Acc merge(const Acc &a, const Acc &b) const {
     Acc r;
 #pragma unroll
-    for (int t = 0; t < MAXT; t++) r.v[t] = a.v[t] + b.v[t];
+    for (int t = 0; t < N; t++) r.v[t] = a.v[t] + b.v[t];
     return r;
   }
   __device__ Acc shfl_down(const Acc &a, int off) const {
     Acc r;
 #pragma unroll
-    for (int t = 0; t < MAXT; t++) r.v[t] = __shfl_down(a.v[t], off, WAVE);
+    for (int t = 0; t < N; t++) r.v[t] = __shfl_down(a.v[t], off, WAVE);
     return r;
   }
   __device__ Acc shfl0(const Acc &a) const {
     Acc r;
 #pragma unroll
-    for (int t = 0; t < MAXT; t++) r.v[t] = __shfl(a.v[t], 0, WAVE);
+    for (int t = 0; t < N; t++) r.v[t] = __shfl(a.v[t], 0, WAVE);
     return r;
   }
   __device__ void add(Acc &acc, int, int k) const {
     const double a = v[k];
     const double *xj = X + (int64_t)ci[k] * T;
 #pragma unroll
-    for (int t = 0; t < MAXT; t++)
+    for (int t = 0; t < N; t++)
       if (t < T) acc.v[t] += (a < 0) ? a * xj[t] : 0.0;
   }
   __device__ void finish(int row, const Acc &acc) const {
     const double w = 2.0 / 3.0;
     const double d = diag[row];
-    for (int t = 0; t < T; t++) {
+#pragma unroll
+    for (int t = 0; t < N; t++) {
+      if (t >= T) break;
       double x = X[(int64_t)row * T + t];
       Y[(int64_t)row * T + t] = hasd[row] ? w * (0.0 - acc.v[t]) / d + (1 - w) * x : x;
     }
@@ -1142,6 +1149,18 @@ __global__ void k_sym_half(int64_t nnz, const int32_t *tpos, const double *v, do
 }
 }  // namespace
 
+namespace {
+// the narrowest accumulator that holds T test vectors (10 is the default)
+void jacobi_tv_sweep(int64_t n, const DCsr &A, const double *X, const double *diag, const uint8_t *hasd,
+                     double *Y, int T, const HugeRows &Hr, hipStream_t s) {
+  if (T <= 4) run_rows(n, A.rp.p, OpJacobiTV<4>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
+  else if (T <= 8) run_rows(n, A.rp.p, OpJacobiTV<8>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
+  else if (T <= 10) run_rows(n, A.rp.p, OpJacobiTV<10>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
+  else if (T <= 12) run_rows(n, A.rp.p, OpJacobiTV<12>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
+  else run_rows(n, A.rp.p, OpJacobiTV<MAXT>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
+}
+}  // namespace
+
 void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevBuf<double> &S,
                       DevBuf<double> &cs, hipStream_t s) {
   const int64_t n = A.n, nnz = A.nnz;
@@ -1161,7 +1180,7 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
     hipLaunchKernelGGL(k_tv_fill, dim3(g1(n)), dim3(256), 0, s, (int)n, o.seed, level_idx, T, sums.p, X.p);
     LG_LAUNCH_CHECK();
     for (int sw = 0; sw < o.tvs_smooth; sw++) {
-      run_rows(n, A.rp.p, OpJacobiTV{A.ci.p, A.v.p, X.p, diag.p, hasd.p, Y.p, T}, Hr, s);
+      jacobi_tv_sweep(n, A, X.p, diag.p, hasd.p, Y.p, T, Hr, s);
       std::swap(X.p, Y.p);
     }
     hipLaunchKernelGGL(k_tv_norm, dim3(g1(n)), dim3(256), 0, s, (int)n, T, X.p, nrm.p);

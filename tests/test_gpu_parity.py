@@ -106,6 +106,19 @@ def test_hierarchy_bit_exact_aggregation_variants(case, variant, my_matrix):
     assert info.rel_residual() <= 1e-8
 
 
+@pytest.mark.parametrize("tvs", [1, 3, 8, 12, 16])
+def test_hierarchy_bit_exact_test_vector_counts(tvs):
+    """Test-vector counts across the accumulator widths of the Jacobi
+    sweep (4, 8, 10, 12 and 16 lanes): same strengths, aggregates and coarse
+    matrices as the oracle."""
+    g = graphs.rmat_config(12, "lognormal", pendants=True)
+    print(f"num_tvs={tvs}", flush=True)
+    do, oo = opts_pair(num_tvs=tvs)
+    d = ligmg_amd.Solver(g, do)
+    assert_same_hierarchy(d, O.OracleSolver(g, oo))
+    assert d.solve(graphs.rhs(g.n, 1, 0)).rel_residual() <= 1e-8
+
+
 def test_stalled_coarsening_is_reported():
     """Rejection that removes nearly every member stalls coarsening until
     max_levels; a coarse level too large for the dense solve is an error,
