@@ -274,7 +274,19 @@ def main():
     cal = torch.ones(1 << 28, dtype=torch.float64, device=dev)
     float(cal.sum())
     torch.cuda.synchronize()
-    del cal
+    # achievable HBM bandwidth beside the vendor peak: a 2 GiB device copy
+    # (2 GiB read + 2 GiB written per copy), HIP events around 5 copies
+    dst = torch.empty_like(cal)
+    dst.copy_(cal)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(cal)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 5 * 2 * cal.numel() * 8 / (e0.elapsed_time(e1) / 1e3) / 1e9
+    print(f"# 2 GiB device copy {copy_gbs:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
+    del cal, dst
 
     # FP64-only solve (the plain method, the drop-in): V-cycle + CG
     verbose = int(os.environ.get("LIGMG_VERBOSE", "0"))
@@ -392,9 +404,10 @@ def main():
             "unit": "GB/s",
             "frac": cheb32["gbs"] / HBM_PEAK_GBS,
             "traffic": traffic_of("smoother_fp32"),
-            "kernel": ("k_spmv_units<float, ValArr<float>, EpiDiag<float, EpiCheb<float>>> "
+            "kernel": ("k_spmv_units<float, ValArr<float>, EpiDiag<float, EpiCheb<float>>, true> "
                        "(base-level FP32 Chebyshev step)"),
         },
+        "hbm_copy_gbs": copy_gbs,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_scale, g.nnz)

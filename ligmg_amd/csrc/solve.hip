@@ -30,6 +30,17 @@ struct EpiAccFinal {
   __device__ __forceinline__ void operator()(int64_t i, double s) const { e(i, acc[i] + s); }
 };
 
+// x[col] through a buffer resource of the vector's bytes: one 32-bit offset
+// per lane instead of a 64-bit address, and col = -1 (a masked lane) falls
+// outside the buffer and reads 0.
+template <class T>
+__device__ __forceinline__ double buf_gather(__amdgpu_buffer_rsrc_t r, int col) {
+  if constexpr (sizeof(T) == 4)
+    return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (uint32_t)col * 4u, 0, 0));
+  else
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (uint32_t)col * 8u, 0, 0));
+}
+
 // Each workgroup takes SPMV_UPW consecutive units.  A complete-row unit
 // stages every product in LDS with coalesced loads of colind/values (the x
 // gathers are the only scattered accesses); its rows are then reduced from
@@ -37,10 +48,11 @@ struct EpiAccFinal {
 // loads are in flight.  A long-row chunk writes a partial sum (8 products
 // per thread added in order, butterfly per wavefront, the wavefront sums
 // added from 0) that k_spmv_long combines.
-template <class T, class Val, class Epi>
+// BUF: x is read through a buffer resource of xbytes (< 2^31) bytes.
+template <class T, class Val, class Epi, bool BUF>
 __global__ void __launch_bounds__(SPMV_BLOCK)
     k_spmv_units(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, Val val,
-                 const T *__restrict__ x, const int32_t *__restrict__ urow0,
+                 const T *__restrict__ x, int32_t xbytes, const int32_t *__restrict__ urow0,
                  const int32_t *__restrict__ unrows, const int32_t *__restrict__ ukoff, int64_t nunits,
                  double *__restrict__ parts, Epi epi) {
   constexpr int IT = UNIT_NNZ / SPMV_BLOCK;
@@ -73,8 +85,16 @@ __global__ void __launch_bounds__(SPMV_BLOCK)
     double pr[IT];
     {
       double xx[IT];
+      if constexpr (BUF) {
+        // word 3 = 0x00020000: the raw-buffer format of gfx9 targets
+        const __amdgpu_buffer_rsrc_t xr =
+            __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, xbytes, 0x00020000);
 #pragma unroll
-      for (int j = 0; j < IT; j++) xx[j] = (col[j] >= 0) ? (double)x[col[j]] : 0.0;
+        for (int j = 0; j < IT; j++) xx[j] = buf_gather<T>(xr, col[j]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < IT; j++) xx[j] = (col[j] >= 0) ? (double)x[col[j]] : 0.0;
+      }
 #pragma unroll
       for (int j = 0; j < IT; j++) pr[j] = (double)vv[j] * xx[j];
     }
@@ -143,8 +163,15 @@ void launch_spmv(const DCsr &A, Val val, const T *x, Epi epi, hipStream_t s) {
   if (A.U.nunits == 0) return;
   double *parts = A.U.parts.p;
   const int64_t grid = (A.U.nunits + SPMV_UPW - 1) / SPMV_UPW;
-  hipLaunchKernelGGL((k_spmv_units<T, Val, Epi>), dim3((unsigned)grid), dim3(SPMV_BLOCK), 0, s, A.rp.p,
-                     A.ci.p, val, x, A.U.row0.p, A.U.nrows.p, A.U.koff.p, A.U.nunits, parts, epi);
+  const int64_t xbytes = A.ncols * (int64_t)sizeof(T);
+  if (xbytes < ((int64_t)1 << 31))
+    hipLaunchKernelGGL((k_spmv_units<T, Val, Epi, true>), dim3((unsigned)grid), dim3(SPMV_BLOCK), 0, s, A.rp.p,
+                       A.ci.p, val, x, (int32_t)xbytes, A.U.row0.p, A.U.nrows.p, A.U.koff.p, A.U.nunits, parts,
+                       epi);
+  else
+    hipLaunchKernelGGL((k_spmv_units<T, Val, Epi, false>), dim3((unsigned)grid), dim3(SPMV_BLOCK), 0, s,
+                       A.rp.p, A.ci.p, val, x, (int32_t)0, A.U.row0.p, A.U.nrows.p, A.U.koff.p, A.U.nunits,
+                       parts, epi);
   LG_LAUNCH_CHECK();
   if (A.U.nlong) {
     hipLaunchKernelGGL((k_spmv_long<Epi>), dim3(grid_for(A.U.nlong, 64)), dim3(64), 0, s, A.U.lrow.p,
