@@ -1150,14 +1150,13 @@ __global__ void k_sym_half(int64_t nnz, const int32_t *tpos, const double *v, do
 }  // namespace
 
 namespace {
-// the narrowest accumulator that holds T test vectors (10 is the default)
+// One accumulator width for every T <= MAXT.  Narrower ones (N = T) lower
+// the register count and raise occupancy, but measured slower (7.6% at
+// T = 10 on R-MAT 23): more gathers of the test-vector array (n * T * 8
+// bytes, larger than the last-level cache) in flight thrash the caches.
 void jacobi_tv_sweep(int64_t n, const DCsr &A, const double *X, const double *diag, const uint8_t *hasd,
                      double *Y, int T, const HugeRows &Hr, hipStream_t s) {
-  if (T <= 4) run_rows(n, A.rp.p, OpJacobiTV<4>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
-  else if (T <= 8) run_rows(n, A.rp.p, OpJacobiTV<8>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
-  else if (T <= 10) run_rows(n, A.rp.p, OpJacobiTV<10>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
-  else if (T <= 12) run_rows(n, A.rp.p, OpJacobiTV<12>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
-  else run_rows(n, A.rp.p, OpJacobiTV<MAXT>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
+  run_rows(n, A.rp.p, OpJacobiTV<MAXT>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
 }
 }  // namespace
 

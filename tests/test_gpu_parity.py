@@ -108,8 +108,8 @@ def test_hierarchy_bit_exact_aggregation_variants(case, variant, my_matrix):
 
 @pytest.mark.parametrize("tvs", [1, 3, 8, 12, 16])
 def test_hierarchy_bit_exact_test_vector_counts(tvs):
-    """Test-vector counts across the accumulator widths of the Jacobi
-    sweep (4, 8, 10, 12 and 16 lanes): same strengths, aggregates and coarse
+    """Test-vector counts from 1 to the maximum of 16 (lanes of the Jacobi
+    sweep's accumulator left unused): same strengths, aggregates and coarse
     matrices as the oracle."""
     g = graphs.rmat_config(12, "lognormal", pendants=True)
     print(f"num_tvs={tvs}", flush=True)
