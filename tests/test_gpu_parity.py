@@ -375,6 +375,58 @@ def test_level_spmv_matches_scipy(case, width, my_matrix):
             assert np.all(np.abs(c) <= 1e-12 * (abs(A) @ np.ones(n)) + 1e-300) or l > 0
 
 
+def banded_degree_graph(seed=5):
+    """A path through n vertices plus, for each target length L, a band of
+    vertices joined to about L random others: units of every mean row length
+    (each lane-group width 1..64 of the SpMV) and hub rows of several
+    2048-entry chunks."""
+    rng = np.random.default_rng(seed)
+    n = 30000
+    u = [np.arange(n - 1)]
+    v = [np.arange(1, n)]
+    first = 0
+    for L, count in [(3, 600), (6, 500), (11, 400), (20, 300), (37, 200), (70, 120), (130, 80), (250, 40),
+                     (500, 20), (1000, 10), (2000, 6), (4500, 4), (9000, 2)]:
+        for r in range(first, first + count):
+            nb = rng.choice(n, size=L, replace=False)
+            nb = nb[nb != r]
+            u.append(np.full(len(nb), r))
+            v.append(nb)
+        first += count
+    u, v = np.concatenate(u).astype(np.int32), np.concatenate(v).astype(np.int32)
+    return graphs.from_edges(n, u, v, rng.uniform(0.5, 1.5, len(u)))
+
+
+@pytest.mark.parametrize("ordering", [_ffi.ORDER_NATURAL, _ffi.ORDER_DEGREE])
+def test_level_spmv_row_length_classes(ordering):
+    """Every lane-group width of the row reduction, two units per workgroup
+    and hub-row chunks, against scipy on every level, FP64 and FP32."""
+    g = banded_degree_graph()
+    d = ligmg_amd.Solver(g, ligmg_amd.Options(precision=_ffi.PREC_MIXED, ordering=ordering))
+    import scipy.sparse as sp
+    rng = np.random.default_rng(1)
+    widths = set()
+    for l in range(d.num_levels()):
+        rp, ci, v = d.level_matrix(l)
+        n = len(rp) - 1
+        deg = np.diff(rp)
+        widths |= {int(min(64, 1 << max(0, int(np.ceil(np.log2(max(1, k / 4))))))) for k in np.unique(deg)}
+        A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+        x = rng.standard_normal(n)
+        y_ref = A @ x
+        scale = abs(A) @ abs(x) + 1e-300
+        y64 = d.level_spmv(l, x, _ffi.PREC_FP64)
+        assert np.all(np.abs(y64 - y_ref) <= 1e-13 * scale), f"level {l} fp64"
+        if d.level_info(l)[0] != _ffi.LEVEL_ELIM or l == 0:
+            y32 = d.level_spmv(l, x, _ffi.PREC_MIXED)
+            assert np.all(np.abs(y32 - y_ref) <= 2e-6 * scale), f"level {l} fp32"
+        if l == 0:
+            assert deg.max() > 2 * 2048  # hub rows of several chunks
+    assert widths >= {1, 2, 4, 8, 16, 32, 64}
+    info = d.solve(graphs.rhs(g.n, 1, 0))
+    assert info.rel_residual() <= 1e-8
+
+
 @pytest.mark.parametrize("prec", [_ffi.PREC_FP64, _ffi.PREC_MIXED])
 def test_sliced_solve_matches_unsliced(prec):
     g = make_case("rmat13_unif")
