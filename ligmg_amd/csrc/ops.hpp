@@ -59,6 +59,12 @@ struct RedPlan {
   // 1), j < np (np <= 4), accumulated in FP64.
   template <class T>
   void dots(int np, const T *const *a, const T *const *b, double *out, hipStream_t s) const;
+  // Krylov form (np <= 2): block c's results go to the scalar slots
+  // sc[c * SC_N + s0] (and s1), then the scalar step `post` (KrPost) runs on
+  // them in the same launch.  Same bits as dots + the separate scalar kernels.
+  template <class T>
+  void dots_sc(int np, const T *const *a, const T *const *b, double *sc, int s0, int s1, int post, int fcg,
+               double rtol, hipStream_t s) const;
   // out[c] = sum over block c of a[i]
   template <class T> void sums(const T *a, double *out, hipStream_t s) const;
   // block vectors (n x kp row-major): out[(blk * np + j) * kp + c] = sum over

@@ -317,14 +317,86 @@ __global__ void __launch_bounds__(256) k_red_stage1(int np, const T *a0, const T
   }
 }
 
-__global__ void k_red_stage2(int np, int nb, const int32_t *bseg, const double *part, double *out) {
+// Where a block's dot results go: out[c * np + j], or the Krylov scalar
+// slots of block c followed by its scalar step.
+struct RedSink {
+  double *out = nullptr;
+  double *sc = nullptr;
+  int s0 = 0, s1 = 0, post = KP_NONE, fcg = 0;
+  double rtol = 0;
+  __device__ __forceinline__ void put(int c, int np, const double *v) const {
+    if (!sc) {
+      for (int j = 0; j < np; j++) out[c * np + j] = v[j];
+      return;
+    }
+    double *S = sc + c * SC_N;
+    S[s0] = v[0];
+    if (np > 1) S[s1] = v[1];
+    kr_post(S, post, fcg, rtol);
+  }
+};
+
+// Both stages in one workgroup for plans of at most RED_FUSE_SEGS segments:
+// four 256-thread groups reduce segments g, g + 4, ... with the tree of
+// k_red_stage1, then one thread per block adds its segment sums in order as
+// k_red_stage2 does.  Same bits as the two launches.
+constexpr int RED_FUSE_SEGS = 16;
+template <class T>
+__global__ void __launch_bounds__(1024) k_red_fused(int np, int nseg, int nb, const T *a0, const T *b0,
+                                                    const T *a1, const T *b1, const T *a2, const T *b2,
+                                                    const T *a3, const T *b3, const int64_t *lo,
+                                                    const int64_t *hi, const int32_t *bseg, RedSink sink) {
+  __shared__ double part[RED_FUSE_SEGS * 4];
+  __shared__ double ws[4][4][4];  // group, product, wavefront
+  const int grp = threadIdx.x / 256, vt = threadIdx.x % 256;
+  const int lane = threadIdx.x & (WAVE - 1), w = vt / WAVE;
+  const T *A[4] = {a0, a1, a2, a3};
+  const T *B[4] = {b0, b1, b2, b3};
+  for (int s0 = 0; s0 < nseg; s0 += 4) {
+    const int s = s0 + grp;
+    double acc[4] = {0, 0, 0, 0};
+    if (s < nseg) {
+      const int64_t l = lo[s], h = hi[s];
+      for (int64_t i = l + vt; i < h; i += 256) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (j < np) acc[j] += (double)A[j][i] * (B[j] ? (double)B[j][i] : 1.0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      double v = wave_sum(acc[j]);
+      if (lane == 0) ws[grp][j][w] = v;
+    }
+    __syncthreads();
+    if (vt < 4 && vt < np && s < nseg) {
+      const int j = vt;
+      part[s * 4 + j] = ((ws[grp][j][0] + ws[grp][j][1]) + ws[grp][j][2]) + ws[grp][j][3];
+    }
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < nb) {
+    const int c = threadIdx.x;
+    double v[4];
+    for (int j = 0; j < np; j++) {
+      double s = 0;
+      for (int g = bseg[c]; g < bseg[c + 1]; g++) s += part[g * 4 + j];
+      v[j] = s;
+    }
+    sink.put(c, np, v);
+  }
+}
+
+__global__ void k_red_stage2(int np, int nb, const int32_t *bseg, const double *part, RedSink sink) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nb) return;
+  double v[4];
   for (int j = 0; j < np; j++) {
     double s = 0;
     for (int g = bseg[c]; g < bseg[c + 1]; g++) s += part[g * 4 + j];
-    out[c * np + j] = s;
+    v[j] = s;
   }
+  sink.put(c, np, v);
 }
 
 template <class T>
@@ -593,19 +665,50 @@ void RedPlan::build(int64_t n_, const std::vector<int64_t> &blocks, hipStream_t 
   LG_HIP(hipStreamSynchronize(s));
 }
 
+namespace {
 template <class T>
-void RedPlan::dots(int np, const T *const *a, const T *const *b, double *out, hipStream_t s) const {
-  if (nseg == 0) return;
+void run_dots(const RedPlan &R, int np, const T *const *a, const T *const *b, const RedSink &sink,
+              hipStream_t s) {
+  if (R.nseg == 0) return;
   const T *A[4] = {nullptr, nullptr, nullptr, nullptr}, *B[4] = {nullptr, nullptr, nullptr, nullptr};
   for (int j = 0; j < np; j++) {
     A[j] = a[j];
     B[j] = b[j];
   }
-  hipLaunchKernelGGL(k_red_stage1<T>, dim3(nseg), dim3(256), 0, s, np, A[0], B[0], A[1], B[1], A[2],
-                     B[2], A[3], B[3], seg_lo.p, seg_hi.p, part.p);
+  if (R.nseg <= RED_FUSE_SEGS && R.nb <= 1024) {  // one launch; a thread per block for the second stage
+    hipLaunchKernelGGL(k_red_fused<T>, dim3(1), dim3(1024), 0, s, np, R.nseg, R.nb, A[0], B[0], A[1], B[1],
+                       A[2], B[2], A[3], B[3], R.seg_lo.p, R.seg_hi.p, R.bseg.p, sink);
+    LG_LAUNCH_CHECK();
+    return;
+  }
+  hipLaunchKernelGGL(k_red_stage1<T>, dim3(R.nseg), dim3(256), 0, s, np, A[0], B[0], A[1], B[1], A[2],
+                     B[2], A[3], B[3], R.seg_lo.p, R.seg_hi.p, R.part.p);
   LG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_red_stage2, dim3((nb + 63) / 64), dim3(64), 0, s, np, nb, bseg.p, part.p, out);
+  hipLaunchKernelGGL(k_red_stage2, dim3((R.nb + 63) / 64), dim3(64), 0, s, np, R.nb, R.bseg.p, R.part.p,
+                     sink);
   LG_LAUNCH_CHECK();
+}
+}  // namespace
+
+template <class T>
+void RedPlan::dots(int np, const T *const *a, const T *const *b, double *out, hipStream_t s) const {
+  RedSink sink;
+  sink.out = out;
+  run_dots<T>(*this, np, a, b, sink, s);
+}
+
+template <class T>
+void RedPlan::dots_sc(int np, const T *const *a, const T *const *b, double *sc, int s0, int s1, int post,
+                      int fcg, double rtol, hipStream_t s) const {
+  LG_CHECK(np >= 1 && np <= 2, 1, "dots_sc: one or two products");
+  RedSink sink;
+  sink.sc = sc;
+  sink.s0 = s0;
+  sink.s1 = s1;
+  sink.post = post;
+  sink.fcg = fcg;
+  sink.rtol = rtol;
+  run_dots<T>(*this, np, a, b, sink, s);
 }
 
 template <class T>
@@ -626,6 +729,10 @@ template void RedPlan::dots<double>(int, const double *const *, const double *co
                                     hipStream_t) const;
 template void RedPlan::dots<float>(int, const float *const *, const float *const *, double *,
                                    hipStream_t) const;
+template void RedPlan::dots_sc<double>(int, const double *const *, const double *const *, double *, int, int,
+                                       int, int, double, hipStream_t) const;
+template void RedPlan::dots_sc<float>(int, const float *const *, const float *const *, double *, int, int, int,
+                                      int, double, hipStream_t) const;
 template void RedPlan::sums<double>(const double *, double *, hipStream_t) const;
 template void RedPlan::sums<float>(const float *, double *, hipStream_t) const;
 template void op_sub_block_mean<double>(double *, const RedPlan &, const double *, hipStream_t);

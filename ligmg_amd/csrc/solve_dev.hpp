@@ -117,5 +117,13 @@ __device__ __forceinline__ void kr_beta(double *S, int fcg) {
   }
 }
 
+// the scalar step that follows a dot product in CG / FCG
+enum KrPost { KP_NONE = 0, KP_ALPHA, KP_CHECK, KP_BETA };
+__device__ __forceinline__ void kr_post(double *S, int post, int fcg, double rtol) {
+  if (post == KP_ALPHA) kr_alpha(S, fcg);
+  else if (post == KP_CHECK) kr_check(S, rtol);
+  else if (post == KP_BETA) kr_beta(S, fcg);
+}
+
 }  // namespace sdev
 }  // namespace lg

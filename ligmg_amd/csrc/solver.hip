@@ -53,13 +53,6 @@ __global__ void k_kr_init(int nb, double *sc) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < nb) kr_init(sc + c * SC_N);
 }
-// scatter a packed dot result out[c*np + j] into scalar slots
-__global__ void k_kr_store(int nb, int np, const double *out, int s0, int s1, double *sc) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nb) return;
-  sc[c * SC_N + s0] = out[c * np];
-  if (np > 1) sc[c * SC_N + s1] = out[c * np + 1];
-}
 
 // x += alpha_c p ; r -= alpha_c q   (segment-per-workgroup, block-aware)
 template <class T>
@@ -1238,8 +1231,7 @@ struct Solver {
     hipLaunchKernelGGL(k_kr_init, dim3((nb + 63) / 64), dim3(64), 0, s, nb, K.sc.p);
     // r = b - A x
     op_residual<T>(A, x, b, K.r.p, s);
-    dot2<T>(R, b, b, K.r.p, K.r.p, K, SC_BN2, SC_RN2);
-    hipLaunchKernelGGL(k_kr_check, dim3((nb + 63) / 64), dim3(64), 0, s, nb, rtol, K.sc.p);
+    dot2<T>(R, b, b, K.r.p, K.r.p, K, SC_BN2, SC_RN2, KP_CHECK, 0, rtol);
     std::vector<double> hsc;
     if (status) *status = LIGMG_CONVERGED;
     if (host_check) {
@@ -1253,15 +1245,13 @@ struct Solver {
     int it = 0;
     while (it < maxit) {
       op_spmv<T>(A, K.p.p, K.q.p, s);
-      if (fcg) dot2<T>(R, K.p.p, K.q.p, K.p.p, K.r.p, K, SC_DELTA, SC_T1);
-      else dot1<T>(R, K.p.p, K.q.p, K, SC_DELTA);
-      hipLaunchKernelGGL(k_kr_alpha, dim3((nb + 63) / 64), dim3(64), 0, s, nb, fcg ? 1 : 0, K.sc.p);
+      if (fcg) dot2<T>(R, K.p.p, K.q.p, K.p.p, K.r.p, K, SC_DELTA, SC_T1, KP_ALPHA, 1);
+      else dot1<T>(R, K.p.p, K.q.p, K, SC_DELTA, KP_ALPHA, 0);
       hipLaunchKernelGGL(k_kr_update<T>, dim3(R.nseg), dim3(256), 0, s, x, K.r.p, K.p.p, K.q.p,
                          R.seg_lo.p, R.seg_hi.p, R.bseg.p + R.nb + 1, K.sc.p);
       LG_LAUNCH_CHECK();
       it++;
-      dot1<T>(R, K.r.p, K.r.p, K, SC_RN2);
-      hipLaunchKernelGGL(k_kr_check, dim3((nb + 63) / 64), dim3(64), 0, s, nb, rtol, K.sc.p);
+      dot1<T>(R, K.r.p, K.r.p, K, SC_RN2, KP_CHECK, 0, rtol);
       if (host_check) {
         hsc = download(K.sc.p, (size_t)nb * SC_N, s);
         double tn = total_norm(hsc, nb);
@@ -1274,9 +1264,8 @@ struct Solver {
       }
       if (it >= maxit) break;
       apply_prec<T>(lvl, K.r.p, K.z.p, outer_prec);
-      if (fcg) dot1<T>(R, K.z.p, K.q.p, K, SC_T1);
-      else dot1<T>(R, K.r.p, K.z.p, K, SC_GN);
-      hipLaunchKernelGGL(k_kr_beta, dim3((nb + 63) / 64), dim3(64), 0, s, nb, fcg ? 1 : 0, K.sc.p);
+      if (fcg) dot1<T>(R, K.z.p, K.q.p, K, SC_T1, KP_BETA, 1);
+      else dot1<T>(R, K.r.p, K.z.p, K, SC_GN, KP_BETA, 0);
       hipLaunchKernelGGL(k_kr_pupdate<T>, dim3(R.nseg), dim3(256), 0, s, K.p.p, K.z.p, R.seg_lo.p,
                          R.seg_hi.p, R.bseg.p + R.nb + 1, K.sc.p);
       LG_LAUNCH_CHECK();
@@ -1487,19 +1476,18 @@ struct Solver {
   }
 
   template <class T>
-  void dot1(const RedPlan &R, const T *a, const T *b, KWork<T> &K, int slot) {
+  // dot products into the scalar slots, followed in the same launch by the
+  // scalar step post (KrPost)
+  void dot1(const RedPlan &R, const T *a, const T *b, KWork<T> &K, int slot, int post = KP_NONE, int fcg = 0,
+            double rtol = 0) {
     const T *A_[1] = {a}, *B_[1] = {b};
-    R.dots<T>(1, A_, B_, K.out.p, s);
-    hipLaunchKernelGGL(k_kr_store, dim3((R.nb + 63) / 64), dim3(64), 0, s, R.nb, 1, K.out.p, slot, slot,
-                       K.sc.p);
+    R.dots_sc<T>(1, A_, B_, K.sc.p, slot, slot, post, fcg, rtol, s);
   }
   template <class T>
   void dot2(const RedPlan &R, const T *a0, const T *b0, const T *a1, const T *b1, KWork<T> &K, int s0,
-            int s1) {
+            int s1, int post = KP_NONE, int fcg = 0, double rtol = 0) {
     const T *A_[2] = {a0, a1}, *B_[2] = {b0, b1};
-    R.dots<T>(2, A_, B_, K.out.p, s);
-    hipLaunchKernelGGL(k_kr_store, dim3((R.nb + 63) / 64), dim3(64), 0, s, R.nb, 2, K.out.p, s0, s1,
-                       K.sc.p);
+    R.dots_sc<T>(2, A_, B_, K.sc.p, s0, s1, post, fcg, rtol, s);
   }
 
   // z = M r with the constant removed per block (PETSc removes the attached

@@ -110,8 +110,9 @@ __device__ __forceinline__ void t_spmv_any(const TailOp<T> &A, const T *x, Epi e
 }
 
 // ------------------------------------------------------------- reductions
-// k_red_stage1 (one chunk per 256-thread group) + k_red_stage2 (chunk sums
-// of a block added in order) + k_kr_store and the scalar kernel after it.
+// RedPlan::dots_sc: k_red_stage1 (one chunk per 256-thread group) +
+// k_red_stage2 (chunk sums of a block added in order, stored into the
+// scalar slots, then the scalar step).
 // s0 < 0: the block sums go to bsum (nullspace projection).
 template <class T>
 __device__ __forceinline__ void t_dots(const TailLevel<T> &L, const TailParams &P, int np, const T *a0,
