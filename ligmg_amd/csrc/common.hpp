@@ -42,6 +42,15 @@ constexpr int WAVE = 64;
 void *dev_alloc(size_t bytes);
 void dev_free(void *p);
 void dev_pool_trim();
+// bracket a graph capture: the pool then never synchronises the device
+void pool_capture_begin();
+void pool_capture_end();
+struct PoolCaptureScope {
+  PoolCaptureScope() { pool_capture_begin(); }
+  ~PoolCaptureScope() { pool_capture_end(); }
+  PoolCaptureScope(const PoolCaptureScope &) = delete;
+  PoolCaptureScope &operator=(const PoolCaptureScope &) = delete;
+};
 
 // Owning device allocation (pooled).
 template <class T>

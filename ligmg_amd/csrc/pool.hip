@@ -7,7 +7,10 @@
 // size class instead.  A released block may still be read by work already
 // queued on some stream, so it is parked as pending and becomes reusable
 // only after one device-wide synchronisation - the guarantee hipFree's own
-// synchronisation gave.  No allocation happens inside a captured graph.
+// synchronisation gave.  No allocation happens inside a captured graph, and
+// while any handle captures one (on any thread) the pool takes fresh blocks
+// instead of synchronising the device, which would end that capture.
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -31,6 +34,9 @@ struct Pool {
   std::unordered_map<void *, Key> live;
   size_t cached = 0;
 };
+
+// graph captures of the library in progress (all threads)
+std::atomic<int> g_capturing{0};
 
 // never destroyed: blocks live until the process ends, after the HIP
 // runtime may already be gone
@@ -70,7 +76,7 @@ void *dev_alloc(size_t bytes) {
   auto it = P.ready.find(key);
   if (it == P.ready.end() || it->second.empty()) {
     auto pt = P.pending.find(key);
-    if (pt != P.pending.end() && !pt->second.empty()) {
+    if (pt != P.pending.end() && !pt->second.empty() && g_capturing.load() == 0) {
       // this device's pending blocks are free once its queued work is done
       LG_HIP(hipDeviceSynchronize());
       for (auto q = P.pending.begin(); q != P.pending.end();) {
@@ -92,7 +98,7 @@ void *dev_alloc(size_t bytes) {
     P.cached -= cls;
   } else {
     hipError_t e = hipMalloc(&p, cls);
-    if (e != hipSuccess) {  // give the cache back and try once more
+    if (e != hipSuccess && g_capturing.load() == 0) {  // give the cache back and try once more
       (void)hipGetLastError();
       flush_locked(P);
       e = hipMalloc(&p, cls);
@@ -121,6 +127,9 @@ void dev_free(void *p) {
   P.pending[key].push_back(p);
   P.cached += key.second;
 }
+
+void pool_capture_begin() { g_capturing.fetch_add(1); }
+void pool_capture_end() { g_capturing.fetch_sub(1); }
 
 void dev_pool_trim() {
   Pool &P = pool();

@@ -1514,14 +1514,17 @@ struct Solver {
           return;
         }
       hipGraph_t graph;
-      LG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      try {
-        precond_outer_eager(lvl, r, z);
-      } catch (...) {
-        (void)hipStreamEndCapture(s, &graph);
-        throw;
+      {
+        PoolCaptureScope cap;
+        LG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          precond_outer_eager(lvl, r, z);
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &graph);
+          throw;
+        }
+        LG_HIP(hipStreamEndCapture(s, &graph));
       }
-      LG_HIP(hipStreamEndCapture(s, &graph));
       hipGraphExec_t exec;
       LG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
       (void)hipGraphDestroy(graph);

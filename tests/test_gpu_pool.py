@@ -53,3 +53,44 @@ def test_solver_kept_while_others_come_and_go():
         run(graphs.barabasi_albert(4000 + 1000 * rep, 4, seed=rep), {}, graphs.rhs(4000 + 1000 * rep, 1, 0))
         ligmg_amd.trim_memory() if rep == 1 else None
         assert np.array_equal(keep.solve(rhs).solution, ref)
+
+
+def test_concurrent_handles_with_graph_capture():
+    """Setups on one thread (allocating from the pool) while another thread's
+    handle captures and replays its preconditioner graph: both get the bits
+    of a sequential run."""
+    import threading
+    ga = graphs.rmat_config(14, "lognormal", pendants=True)
+    gb = graphs.grid2d(120, 110, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=4)
+    ra, rb = graphs.rhs(ga.n, 1, 2), graphs.rhs(gb.n, 1, 3)
+    ref_a = ligmg_amd.Solver(ga).solve(ra).solution
+    ref_b = ligmg_amd.Solver(gb, ligmg_amd.Options(use_graphs=1)).solve(rb).solution
+    errors = []
+
+    def setups():
+        try:
+            for _ in range(4):
+                s = ligmg_amd.Solver(ga)
+                if not np.array_equal(s.solve(ra).solution, ref_a):
+                    errors.append("setup thread: different bits")
+                s.close()
+        except Exception as e:  # noqa: BLE001
+            errors.append(repr(e))
+
+    def solves():
+        try:
+            s = ligmg_amd.Solver(gb, ligmg_amd.Options(use_graphs=1))
+            for _ in range(6):
+                if not np.array_equal(s.solve(rb).solution, ref_b):
+                    errors.append("graph thread: different bits")
+            s.close()
+        except Exception as e:  # noqa: BLE001
+            errors.append(repr(e))
+
+    ts = [threading.Thread(target=setups), threading.Thread(target=solves)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in ts)
+    assert not errors, errors
