@@ -114,38 +114,60 @@ struct HugeRows {
 
 namespace {
 
-void find_huge(int64_t n, const int32_t *rp, HugeRows &Hr, hipStream_t s) {
-  std::vector<int32_t> h = download(rp, n + 1, s);
-  std::vector<int32_t> row, cptr, cstart, cend, crow;
-  for (int64_t i = 0; i < n; i++) {
-    if (h[i + 1] - h[i] <= CHUNK) continue;
-    row.push_back((int32_t)i);
-    cptr.push_back((int32_t)cstart.size());
-    for (int64_t k = h[i]; k < h[i + 1]; k += CHUNK) {
-      cstart.push_back((int32_t)k);
-      cend.push_back((int32_t)std::min<int64_t>(k + CHUNK, h[i + 1]));
-      crow.push_back((int32_t)i);
+__global__ void k_huge_flags(int64_t n, const int32_t *rp, int32_t *flag) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    flag[i] = (rp[i + 1] - rp[i] > CHUNK) ? 1 : 0;
+}
+__global__ void k_huge_nchunks(int64_t nh, const int32_t *row, const int32_t *rp, int32_t *cnt) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t len = rp[row[j] + 1] - rp[row[j]];
+    cnt[j] = (len + CHUNK - 1) / CHUNK;
+  }
+}
+// the chunks of huge row j: [rp[r] + c * CHUNK, min(.. + CHUNK, rp[r + 1]))
+__global__ void k_huge_fill(int64_t nh, const int32_t *row, const int32_t *rp, const int32_t *cptr,
+                            int32_t *cstart, int32_t *cend, int32_t *crow) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t r = row[j];
+    const int64_t b = rp[r + 1];
+    int32_t c = cptr[j];
+    for (int64_t k = rp[r]; k < b; k += CHUNK, c++) {
+      cstart[c] = (int32_t)k;
+      cend[c] = (int32_t)(b - k > CHUNK ? k + CHUNK : b);
+      crow[c] = r;
     }
   }
-  cptr.push_back((int32_t)cstart.size());
-  Hr.nrows = (int64_t)row.size();
-  Hr.nchunks = (int64_t)cstart.size();
-  if (Hr.nrows == 0) return;
-  Hr.row.alloc(Hr.nrows);
-  Hr.cptr.alloc(Hr.nrows + 1);
+}
+
+// Rows longer than CHUNK, in ascending order, and their chunks, built on the
+// device (two counts come back to size the tables).
+void find_huge(int64_t n, const int32_t *rp, HugeRows &Hr, hipStream_t s) {
+  Hr.nrows = 0;
+  Hr.nchunks = 0;
+  if (n <= 0) return;
+  DevBuf<int32_t> flag(n), idx(n);
+  hipLaunchKernelGGL(k_huge_flags, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, rp, flag.p);
+  LG_LAUNCH_CHECK();
+  const int64_t nh = compact_flags(flag.p, n, idx.p, s);
+  if (nh == 0) return;
+  Hr.nrows = nh;
+  Hr.row.alloc(nh);
+  d2d(Hr.row.p, idx.p, nh, s);
+  Hr.cptr.alloc(nh + 1);
+  hipLaunchKernelGGL(k_huge_nchunks, dim3(grid_for(nh, 256, 16384)), dim3(256), 0, s, nh, Hr.row.p, rp, Hr.cptr.p);
+  LG_LAUNCH_CHECK();
+  fill_i32(Hr.cptr.p + nh, 0, 1, s);
+  Hr.nchunks = exclusive_scan_i32(Hr.cptr.p, Hr.cptr.p, nh + 1, s);
   Hr.cstart.alloc(Hr.nchunks);
   Hr.cend.alloc(Hr.nchunks);
   Hr.crow.alloc(Hr.nchunks);
-  to_device(Hr.row.p, row.data(), Hr.nrows, s);
-  to_device(Hr.cptr.p, cptr.data(), Hr.nrows + 1, s);
-  to_device(Hr.cstart.p, cstart.data(), Hr.nchunks, s);
-  to_device(Hr.cend.p, cend.data(), Hr.nchunks, s);
-  to_device(Hr.crow.p, crow.data(), Hr.nchunks, s);
+  hipLaunchKernelGGL(k_huge_fill, dim3(grid_for(nh, 256, 16384)), dim3(256), 0, s, nh, Hr.row.p, rp, Hr.cptr.p,
+                     Hr.cstart.p, Hr.cend.p, Hr.crow.p);
+  LG_LAUNCH_CHECK();
   Hr.cacc.alloc((size_t)Hr.nchunks * 192);  // room for the largest Acc
-  LG_HIP(hipStreamSynchronize(s));
 }
 
-// the long-row table of A, downloaded and built once per matrix
+// the long-row table of A, built once per matrix
 const HugeRows &huge_of(const DCsr &A, hipStream_t s) {
   if (!A.huge) {
     auto h = std::make_shared<HugeRows>();

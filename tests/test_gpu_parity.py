@@ -427,6 +427,17 @@ def test_level_spmv_row_length_classes(ordering):
     assert info.rel_residual() <= 1e-8
 
 
+@pytest.mark.parametrize("ordering", [_ffi.ORDER_NATURAL, _ffi.ORDER_DEGREE])
+def test_hierarchy_bit_exact_with_rows_beyond_one_chunk(ordering):
+    """Rows of 4500 and 9000 entries take the chunked path of every setup
+    row kernel (chunks of 4096 combined in order): hierarchy bit-exact."""
+    g = banded_degree_graph()
+    assert np.diff(g.rowptr).max() > 2 * 4096
+    do, oo = opts_pair(ordering=ordering)
+    d = ligmg_amd.Solver(g, do)
+    assert_same_hierarchy(d, O.OracleSolver(g, oo))
+
+
 @pytest.mark.parametrize("prec", [_ffi.PREC_FP64, _ffi.PREC_MIXED])
 def test_sliced_solve_matches_unsliced(prec):
     g = make_case("rmat13_unif")
