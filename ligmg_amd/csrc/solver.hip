@@ -332,6 +332,28 @@ struct Solver {
   // layout, SpMV units, smoothers and eigen-estimates, FP32 copies, work
   // vectors, slicing, the fused tail.  Shared by build and load.
   void finish_setup() {
+    // with o.verbose, the time of each solve-preparation phase
+    auto t_last = std::chrono::steady_clock::now();
+    auto phase = [&](const char *what) {
+      if (!o.verbose) return;
+      LG_HIP(hipStreamSynchronize(s));
+      const auto now = std::chrono::steady_clock::now();
+      fprintf(stderr, "[ligmg]   solve-prep %-14s %.3f s\n", what,
+              std::chrono::duration<double>(now - t_last).count());
+      t_last = now;
+    };
+    double t_units = 0, t_eig = 0, t_red = 0;
+    auto timed = [&](double &acc, auto &&f) {
+      if (!o.verbose) {
+        f();
+        return;
+      }
+      LG_HIP(hipStreamSynchronize(s));
+      const auto a = std::chrono::steady_clock::now();
+      f();
+      LG_HIP(hipStreamSynchronize(s));
+      acc += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    };
     // blocks per level: level-0 blocks mapped through kept ids / roots
     for (size_t l = 0; l + 1 < H.size(); l++) {
       const DLevel &L = H[l];
@@ -349,7 +371,9 @@ struct Solver {
       }
       H[l + 1].blocks = nb;
     }
+    phase("blocks");
     if (o.solve_layout && o.spmv_slice == 0) apply_solve_layout();
+    phase("layout");
     // caller's vertex of each level-0 solve position
     if (H[0].sperm.p && perm_input.p) {
       perm.alloc(H[0].A.n);
@@ -371,21 +395,25 @@ struct Solver {
     be_post.resize(H.size());
     for (size_t l = 0; l < H.size(); l++) {
       DLevel &L = H[l];
-      red[l].build(L.A.n, L.blocks, s);
-      if (L.kind != LIGMG_LEVEL_ELIM || l == 0) build_units(L.A, s);
+      timed(t_red, [&] { red[l].build(L.A.n, L.blocks, s); });
+      if (L.kind != LIGMG_LEVEL_ELIM || l == 0) timed(t_units, [&] { build_units(L.A, s); });
       if (L.kind == LIGMG_LEVEL_AGG) {
-        build_units(L.M, s);
+        timed(t_units, [&] { build_units(L.M, s); });
         setup_dinv(L.A, L, mixed, s);
         if (o.smoother_type == LIGMG_SMOOTHER_CHEBYSHEV) {
-          L.lam = eig_estimate(L.A, L.dinv.p, o.seed, (int)l, o.cheby_eig_iters, L.sperm.p, s);
+          timed(t_eig, [&] {
+            L.lam = eig_estimate(L.A, L.dinv.p, o.seed, (int)l, o.cheby_eig_iters, L.sperm.p, s);
+          });
           L.cheb_lo = o.cheby_lower * L.lam;
           L.cheb_hi = o.cheby_upper * L.lam;
           al_pre[l] = cheb_alpha(L.cheb_lo, L.cheb_hi, o.pre_smooth, be_pre[l]);
           al_post[l] = cheb_alpha(L.cheb_lo, L.cheb_hi, o.post_smooth, be_post[l]);
         }
       } else if (L.kind == LIGMG_LEVEL_ELIM) {
-        build_units(L.P, s);
-        build_units(L.R, s);
+        timed(t_units, [&] {
+          build_units(L.P, s);
+          build_units(L.R, s);
+        });
       }
       if (mixed) {
         make_v32(L.A, s, true);
@@ -395,6 +423,10 @@ struct Solver {
       alloc_work<double>(l);
       if (mixed) alloc_work<float>(l);
     }
+    if (o.verbose)
+      fprintf(stderr, "[ligmg]   solve-prep of which: SpMV units %.3f s, eigen estimates %.3f s, "
+              "reduction plans %.3f s\n", t_units, t_eig, t_red);
+    phase("levels");
     // SpMV layout: operators applied in the solve get column slices when
     // their vector does not fit in L2 (tuned or fixed by o.spmv_slice)
     const size_t base = (o.top_elim && H[0].kind == LIGMG_LEVEL_ELIM) ? 1 : 0;
@@ -412,8 +444,10 @@ struct Solver {
     int maxnb = 1;
     for (auto &r : red) maxnb = std::max(maxnb, r.nb);
     dsc.alloc((size_t)8 * maxnb);
+    phase("slices");
     setup_tail();
     LG_HIP(hipStreamSynchronize(s));
+    phase("tail");
   }
 
   // ---------------------------------------------------------- dump / load
