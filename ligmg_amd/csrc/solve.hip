@@ -339,8 +339,10 @@ struct RedSink {
 // Both stages in one workgroup for plans of at most RED_FUSE_SEGS segments:
 // four 256-thread groups reduce segments g, g + 4, ... with the tree of
 // k_red_stage1, then one thread per block adds its segment sums in order as
-// k_red_stage2 does.  Same bits as the two launches.
-constexpr int RED_FUSE_SEGS = 16;
+// k_red_stage2 does.  Same bits as the two launches.  Only for small
+// vectors: one workgroup reads at one CU's bandwidth, which loses to
+// nseg parallel workgroups beyond two segments (measured on the K-cycle).
+constexpr int RED_FUSE_SEGS = 2;
 template <class T>
 __global__ void __launch_bounds__(1024) k_red_fused(int np, int nseg, int nb, const T *a0, const T *b0,
                                                     const T *a1, const T *b1, const T *a2, const T *b2,
