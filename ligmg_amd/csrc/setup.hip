@@ -672,6 +672,31 @@ struct OpTpos : NoAcc {
   __device__ void finish(int, int) const {}
 };
 
+// (column, row | index) of every stored entry, for transpose_positions
+struct OpTposKeys : NoAcc {
+  const int32_t *ci;
+  uint64_t *keys, *vals;
+  __device__ void add(int &, int row, int k) const {
+    keys[k] = (uint64_t)(uint32_t)ci[k];
+    vals[k] = ((uint64_t)(uint32_t)row << 32) | (uint32_t)k;
+  }
+  __device__ void finish(int, int) const {}
+};
+
+// Sorted by column (stable), the entries of column c sit at rp[c] ..
+// rp[c + 1] in ascending row order - the positions of their transposes -
+// when the pattern is structurally symmetric; anything else sets *bad.
+__global__ void k_tpos_scatter(int64_t m, const int32_t *rp, const int32_t *ci, const uint64_t *keys,
+                               const uint64_t *vals, int32_t *tpos, int32_t *bad) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < m; p += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c = (int32_t)keys[p];
+    const int32_t row = (int32_t)(vals[p] >> 32);
+    const int32_t k = (int32_t)(vals[p] & 0xffffffffu);
+    tpos[k] = (int32_t)p;
+    if (p < rp[c] || p >= rp[c + 1] || ci[p] != row) atomicOr(bad, 1);
+  }
+}
+
 struct OpColMax : DblMax {
   const int32_t *tpos;
   const double *raw;
@@ -1172,6 +1197,30 @@ __global__ void k_sym_half(int64_t nnz, const int32_t *tpos, const double *v, do
 }  // namespace
 
 namespace {
+// tpos[k] = position of the transposed entry of k.  Large matrices: one
+// stable sort of the entries by column (a few passes) instead of a binary
+// search per entry; both give the same table.  Returns false when the
+// pattern is not structurally symmetric.
+bool transpose_positions(const DCsr &A, const HugeRows &Hr, int32_t *tpos, hipStream_t s) {
+  const int64_t n = A.n, m = A.nnz;
+  DevBuf<int32_t> bad(1);
+  fill_i32(bad.p, 0, 1, s);
+  if (m >= ((int64_t)1 << 20)) {
+    DevBuf<uint64_t> keys(m), vals(m);
+    run_rows(n, A.rp.p, OpTposKeys{{}, A.ci.p, keys.p, vals.p}, Hr, s);
+    radix_sort_pairs(keys.p, vals.p, m, std::max(1, bits_for((uint64_t)n)), s);
+    hipLaunchKernelGGL(k_tpos_scatter, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, m, A.rp.p, A.ci.p,
+                       keys.p, vals.p, tpos, bad.p);
+    LG_LAUNCH_CHECK();
+  } else {
+    run_rows(n, A.rp.p, OpTpos{{}, A.rp.p, A.ci.p, tpos, bad.p}, Hr, s);
+  }
+  int32_t hb = 0;
+  to_host(&hb, bad.p, 1, s);
+  LG_HIP(hipStreamSynchronize(s));
+  return hb == 0;
+}
+
 // One accumulator width for every T <= MAXT.  Narrower ones (N = T) lower
 // the register count and raise occupancy, but measured slower (7.6% at
 // T = 10 on R-MAT 23): more gathers of the test-vector array (n * T * 8
@@ -1189,7 +1238,7 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
   const HugeRows &Hr = huge_of(A, s);
   DevBuf<double> X((size_t)n * T), Y((size_t)n * T), nrm(n), diag(n), raw(nnz), rowmax(n), colmax(n);
   DevBuf<uint8_t> hasd(n);
-  DevBuf<int32_t> tpos(nnz), bad(1);
+  DevBuf<int32_t> tpos(nnz);
   hipLaunchKernelGGL(k_diag, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p, diag.p, hasd.p);
   LG_LAUNCH_CHECK();
   if (o.strength_metric != LIGMG_STRENGTH_NONE) {
@@ -1209,12 +1258,8 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
   }
   run_rows(n, A.rp.p, OpAffinity{{}, A.ci.p, A.v.p, X.p, nrm.p, T, o.strength_metric, raw.p, rowmax.p},
            Hr, s);
-  LG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
-  run_rows(n, A.rp.p, OpTpos{{}, A.rp.p, A.ci.p, tpos.p, bad.p}, Hr, s);
-  int32_t hb = 0;
-  to_host(&hb, bad.p, 1, s);
-  LG_HIP(hipStreamSynchronize(s));
-  LG_CHECK(hb == 0, LIGMG_ERR_INVALID, "matrix pattern is not structurally symmetric");
+  LG_CHECK(transpose_positions(A, Hr, tpos.p, s), LIGMG_ERR_INVALID,
+           "matrix pattern is not structurally symmetric");
   // Row / Column / RowColumn / Sym scaling (aggregation.hpp:350-366): row
   // sums, then column sums (ascending row), then (S^T + S) / 2; a zero sum
   // leaves its (zero) entries at 0
@@ -1325,13 +1370,8 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
 void col_sums_of(const DCsr &A, const double *S, DevBuf<double> &cs, hipStream_t s) {
   const int64_t n = A.n;
   const HugeRows &Hr = huge_of(A, s);
-  DevBuf<int32_t> tpos(A.nnz), bad(1);
-  LG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
-  run_rows(n, A.rp.p, OpTpos{{}, A.rp.p, A.ci.p, tpos.p, bad.p}, Hr, s);
-  int32_t hb = 0;
-  to_host(&hb, bad.p, 1, s);
-  LG_HIP(hipStreamSynchronize(s));
-  LG_CHECK(hb == 0, LIGMG_ERR_INVALID, "matrix pattern is not structurally symmetric");
+  DevBuf<int32_t> tpos(A.nnz);
+  LG_CHECK(transpose_positions(A, Hr, tpos.p, s), LIGMG_ERR_INVALID, "matrix pattern is not structurally symmetric");
   cs.alloc(n);
   run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S, cs.p}, Hr, s);
 }

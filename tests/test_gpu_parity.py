@@ -337,6 +337,34 @@ def test_zero_rhs_and_tiny_graphs():
         assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("ordering", [_ffi.ORDER_NATURAL, _ffi.ORDER_DEGREE])
+def test_hierarchy_bit_exact_sorted_transpose(ordering):
+    """Level matrices of more than 2^20 entries find transposed entries by
+    one sort by column instead of a binary search per entry: same
+    strengths and hierarchy as the oracle."""
+    g = graphs.rmat_config(16, "lognormal", pendants=True)
+    do, oo = opts_pair(ordering=ordering)
+    d = ligmg_amd.Solver(g, do)
+    big = [l for l in range(d.num_levels()) if d.level_info(l)[0] == _ffi.LEVEL_AGG and d.level_info(l)[2] >= 1 << 20]
+    assert big
+    assert_same_hierarchy(d, O.OracleSolver(g, oo))
+
+
+def test_large_asymmetric_pattern_is_rejected():
+    """The sorted-transpose check rejects a missing mirror entry too."""
+    g = graphs.grid2d(600, 600)
+    assert g.nnz >= 1 << 20
+    rp, ci, v = g.rowptr.copy(), g.colind.copy(), g.vals.copy()
+    r = 300 * 600 + 7
+    k = rp[r] + 1 if ci[rp[r]] != r else rp[r] + 2  # an off-diagonal entry of row r
+    ci = np.delete(ci, k)
+    v = np.delete(v, k)
+    rp[r + 1:] -= 1
+    with pytest.raises(ligmg_amd.LigmgError) as e:
+        ligmg_amd.Solver((rp, ci, v), ligmg_amd.Options(always_elim=0))
+    assert e.value.code == _ffi.ERR_INVALID
+
+
 def test_asymmetric_pattern_is_rejected():
     g = graphs.grid2d(10, 10)
     # drop one off-diagonal entry of row 0 -> structurally asymmetric
