@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <type_traits>
 
 #include "../../include/ligmg_amd.h"
 #include "ops.hpp"
@@ -35,12 +36,24 @@ constexpr int MAXT = 16;
 // --------------------------------------------------------------------------
 constexpr int CHUNK = 4096;
 
+// An Op with skip(row) leaves such rows out entirely (no add, no finish):
+// for operators whose results are only read for the other rows.
+template <class Op, class = void>
+struct HasSkip : std::false_type {};
+template <class Op>
+struct HasSkip<Op, std::void_t<decltype(std::declval<const Op &>().skip(0))>> : std::true_type {};
+template <class Op>
+__device__ __forceinline__ bool row_skip(const Op &op, int row) {
+  if constexpr (HasSkip<Op>::value) return op.skip(row);
+  else return false;
+}
+
 template <class Op>
 __global__ void __launch_bounds__(256) k_rows(int n, const int32_t *__restrict__ rp, Op op) {
   using Acc = typename Op::Acc;
   const int lane = threadIdx.x & (WAVE - 1);
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = row < n;
+  const bool valid = row < n && !row_skip(op, row);
   int a = 0, b = 0;
   if (valid) {
     a = rp[row];
@@ -78,6 +91,7 @@ __global__ void __launch_bounds__(256) k_rows_chunks(int64_t nchunks, const int3
   const int64_t c = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
   if (c >= nchunks) return;
   const int la = cstart[c], lb = cend[c], lr = crow[c];
+  if (row_skip(op, lr)) return;  // the whole wavefront: one chunk per wavefront
   Acc p = op.zero();
   if (lb - la <= WAVE) {
     if (lane == 0)
@@ -97,7 +111,7 @@ template <class Op>
 __global__ void k_rows_huge(int64_t nrows, const int32_t *row, const int32_t *cptr, Op op,
                             const typename Op::Acc *cacc) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (j >= nrows) return;
+  if (j >= nrows || row_skip(op, row[j])) return;
   typename Op::Acc acc = op.zero();
   for (int c = cptr[j]; c < cptr[j + 1]; c++) acc = op.merge(acc, cacc[c]);
   op.finish(row[j], acc);
@@ -748,7 +762,7 @@ struct OpColSum : DblSum {
 //      ties to the later operand equals the maximum of (priority, weight,
 //      position), which any reduction order reproduces.
 struct VoteAcc {
-  int prio, pos, ind;
+  int prio, pos;  // pos: the stored entry of the best neighbour so far
   double w;
 };
 struct OpVote {
@@ -760,7 +774,10 @@ struct OpVote {
   int8_t *vst;
   int32_t *vind;
   double *vw;
-  __device__ Acc zero() const { return {-1, -1, -1, -1.0}; }
+  __device__ Acc zero() const { return {-1, -1, -1.0}; }
+  // only Undecided vertices vote; the vote of any other row is never read
+  // (k_voters, k_mark) in the iteration that computes it
+  __device__ bool skip(int row) const { return state[row] != ST_UNDECIDED; }
   __device__ static bool better(const Acc &b, const Acc &a) {  // b beats a
     if (b.prio != a.prio) return b.prio > a.prio;
     if (b.w != a.w) return b.w > a.w;
@@ -768,12 +785,10 @@ struct OpVote {
   }
   __device__ Acc merge(const Acc &a, const Acc &b) const { return better(b, a) ? b : a; }
   __device__ Acc shfl_down(const Acc &a, int off) const {
-    return {__shfl_down(a.prio, off, WAVE), __shfl_down(a.pos, off, WAVE),
-            __shfl_down(a.ind, off, WAVE), __shfl_down(a.w, off, WAVE)};
+    return {__shfl_down(a.prio, off, WAVE), __shfl_down(a.pos, off, WAVE), __shfl_down(a.w, off, WAVE)};
   }
   __device__ Acc shfl0(const Acc &a) const {
-    return {__shfl(a.prio, 0, WAVE), __shfl(a.pos, 0, WAVE), __shfl(a.ind, 0, WAVE),
-            __shfl(a.w, 0, WAVE)};
+    return {__shfl(a.prio, 0, WAVE), __shfl(a.pos, 0, WAVE), __shfl(a.w, 0, WAVE)};
   }
   __device__ void add(Acc &acc, int, int k) const {
     const double s = S[k];
@@ -784,7 +799,7 @@ struct OpVote {
     if (st == ST_SEED || (nn && st == ST_SEEDNEIGHBOR)) p = (s > thresh) ? PRIO_SEED : PRIO_DECIDED;
     else if (st == ST_UNDECIDED) p = PRIO_UNDECIDED;
     else p = PRIO_DECIDED;
-    Acc m = {p, k, ind[j], s};
+    Acc m = {p, k, s};
     if (better(m, acc)) acc = m;
   }
   __device__ void finish(int row, const Acc &acc) const {
@@ -794,7 +809,7 @@ struct OpVote {
       vw[row] = -1.0;
     } else {
       vst[row] = (int8_t)acc.prio;
-      vind[row] = acc.ind;
+      vind[row] = ind[ci[acc.pos]];  // the winner's aggregate index, read once
       vw[row] = acc.w;
     }
   }
