@@ -541,13 +541,16 @@ __global__ void k_tv_sum(int n, uint64_t seed, int level, int T, unsigned long l
   }
 }
 
-__global__ void k_tv_fill(int n, uint64_t seed, int level, int T, const unsigned long long *sums,
+// Test vectors are stored vertex-major with a row stride TS (the power of
+// two >= T): a vertex's T values never straddle a cache line, so each
+// gather of a neighbour's row costs one line.  Slots t >= T stay unused.
+__global__ void k_tv_fill(int n, uint64_t seed, int level, int T, int TS, const unsigned long long *sums,
                           double *X) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int t = 0; t < T; t++) {
     double mean = ((double)sums[t] / (double)n) * 0x1p-32;
-    X[(int64_t)i * T + t] = (double)ligmg_rng::tv_bits(seed, level, t, i) * 0x1p-32 - mean;
+    X[(int64_t)i * TS + t] = (double)ligmg_rng::tv_bits(seed, level, t, i) * 0x1p-32 - mean;
   }
 }
 
@@ -567,7 +570,7 @@ struct OpJacobiTV {
   const double *v, *X, *diag;
   const uint8_t *hasd;
   double *Y;
-  int T;
+  int T, TS;  // vector count, row stride
   __device__ Acc zero() const {
     Acc a;
 #pragma unroll
@@ -594,7 +597,7 @@ struct OpJacobiTV {
   }
   __device__ void add(Acc &acc, int, int k) const {
     const double a = v[k];
-    const double *xj = X + (int64_t)ci[k] * T;
+    const double *xj = X + (int64_t)ci[k] * TS;
 #pragma unroll
     for (int t = 0; t < N; t++)
       if (t < T) acc.v[t] += (a < 0) ? a * xj[t] : 0.0;
@@ -605,8 +608,8 @@ struct OpJacobiTV {
 #pragma unroll
     for (int t = 0; t < N; t++) {
       if (t >= T) break;
-      double x = X[(int64_t)row * T + t];
-      Y[(int64_t)row * T + t] = hasd[row] ? w * (0.0 - acc.v[t]) / d + (1 - w) * x : x;
+      double x = X[(int64_t)row * TS + t];
+      Y[(int64_t)row * TS + t] = hasd[row] ? w * (0.0 - acc.v[t]) / d + (1 - w) * x : x;
     }
   }
 };
@@ -621,11 +624,11 @@ __global__ void k_diag(int n, const int32_t *rp, const int32_t *ci, const double
   hasd[row] = f ? 1 : 0;
 }
 
-__global__ void k_tv_norm(int n, int T, const double *X, double *nrm) {
+__global__ void k_tv_norm(int n, int T, int TS, const double *X, double *nrm) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0;
-  for (int t = 0; t < T; t++) s += X[(int64_t)i * T + t] * X[(int64_t)i * T + t];
+  for (int t = 0; t < T; t++) s += X[(int64_t)i * TS + t] * X[(int64_t)i * TS + t];
   nrm[i] = s;
 }
 
@@ -633,7 +636,7 @@ __global__ void k_tv_norm(int n, int T, const double *X, double *nrm) {
 struct OpAffinity : DblMax {
   const int32_t *ci;
   const double *v, *X, *nrm;
-  int T, metric;
+  int T, TS, metric;  // vector count, row stride
   double *raw, *rowmax;
   __device__ void add(double &acc, int row, int k) const {
     const int c = ci[k];
@@ -641,12 +644,12 @@ struct OpAffinity : DblMax {
     if (c == row) {
       r = (metric == LIGMG_STRENGTH_NONE) ? (v[k] < 0 ? -v[k] : 0.0) : 0.0;
     } else if (metric == LIGMG_STRENGTH_AFFINITY) {
-      const double *xc = X + (int64_t)c * T, *xr = X + (int64_t)row * T;
+      const double *xc = X + (int64_t)c * TS, *xr = X + (int64_t)row * TS;
       double top = 0;
       for (int t = 0; t < T; t++) top += xc[t] * xr[t];
       r = top * top / (nrm[c] * nrm[c] * nrm[row] * nrm[row]);
     } else if (metric == LIGMG_STRENGTH_ALGEBRAIC_DISTANCE) {
-      const double *xc = X + (int64_t)c * T, *xr = X + (int64_t)row * T;
+      const double *xc = X + (int64_t)c * TS, *xr = X + (int64_t)row * TS;
       double m = 0;
       for (int t = 0; t < T; t++) {
         double d = fabs(xc[t] - xr[t]);
@@ -1238,11 +1241,11 @@ bool transpose_positions(const DCsr &A, const HugeRows &Hr, int32_t *tpos, hipSt
 
 // One accumulator width for every T <= MAXT.  Narrower ones (N = T) lower
 // the register count and raise occupancy, but measured slower (7.6% at
-// T = 10 on R-MAT 23): more gathers of the test-vector array (n * T * 8
+// T = 10 on R-MAT 23): more gathers of the test-vector array (n * TS * 8
 // bytes, larger than the last-level cache) in flight thrash the caches.
 void jacobi_tv_sweep(int64_t n, const DCsr &A, const double *X, const double *diag, const uint8_t *hasd,
-                     double *Y, int T, const HugeRows &Hr, hipStream_t s) {
-  run_rows(n, A.rp.p, OpJacobiTV<MAXT>{A.ci.p, A.v.p, X, diag, hasd, Y, T}, Hr, s);
+                     double *Y, int T, int TS, const HugeRows &Hr, hipStream_t s) {
+  run_rows(n, A.rp.p, OpJacobiTV<MAXT>{A.ci.p, A.v.p, X, diag, hasd, Y, T, TS}, Hr, s);
 }
 }  // namespace
 
@@ -1251,7 +1254,9 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
   const int64_t n = A.n, nnz = A.nnz;
   const int T = o.num_tvs;
   const HugeRows &Hr = huge_of(A, s);
-  DevBuf<double> X((size_t)n * T), Y((size_t)n * T), nrm(n), diag(n), raw(nnz), rowmax(n), colmax(n);
+  int TS = 1;
+  while (TS < T) TS <<= 1;
+  DevBuf<double> X((size_t)n * TS), Y((size_t)n * TS), nrm(n), diag(n), raw(nnz), rowmax(n), colmax(n);
   DevBuf<uint8_t> hasd(n);
   DevBuf<int32_t> tpos(nnz);
   hipLaunchKernelGGL(k_diag, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p, diag.p, hasd.p);
@@ -1262,16 +1267,16 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
     hipLaunchKernelGGL(k_tv_sum, dim3(std::min<unsigned>(g1(n), 1024)), dim3(256), 0, s, (int)n, o.seed,
                        level_idx, T, sums.p);
     LG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tv_fill, dim3(g1(n)), dim3(256), 0, s, (int)n, o.seed, level_idx, T, sums.p, X.p);
+    hipLaunchKernelGGL(k_tv_fill, dim3(g1(n)), dim3(256), 0, s, (int)n, o.seed, level_idx, T, TS, sums.p, X.p);
     LG_LAUNCH_CHECK();
     for (int sw = 0; sw < o.tvs_smooth; sw++) {
-      jacobi_tv_sweep(n, A, X.p, diag.p, hasd.p, Y.p, T, Hr, s);
+      jacobi_tv_sweep(n, A, X.p, diag.p, hasd.p, Y.p, T, TS, Hr, s);
       std::swap(X.p, Y.p);
     }
-    hipLaunchKernelGGL(k_tv_norm, dim3(g1(n)), dim3(256), 0, s, (int)n, T, X.p, nrm.p);
+    hipLaunchKernelGGL(k_tv_norm, dim3(g1(n)), dim3(256), 0, s, (int)n, T, TS, X.p, nrm.p);
     LG_LAUNCH_CHECK();
   }
-  run_rows(n, A.rp.p, OpAffinity{{}, A.ci.p, A.v.p, X.p, nrm.p, T, o.strength_metric, raw.p, rowmax.p},
+  run_rows(n, A.rp.p, OpAffinity{{}, A.ci.p, A.v.p, X.p, nrm.p, T, TS, o.strength_metric, raw.p, rowmax.p},
            Hr, s);
   LG_CHECK(transpose_positions(A, Hr, tpos.p, s), LIGMG_ERR_INVALID,
            "matrix pattern is not structurally symmetric");
