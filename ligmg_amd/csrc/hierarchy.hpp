@@ -82,6 +82,9 @@ struct DLevel {
   // aggregation: coarse id per vertex, members CSR (nc x n, unit values)
   DevBuf<int32_t> agg, ind, state;
   DevBuf<double> strength;
+  // positions of A's transposed entries, kept from the strength computation
+  // until the solve layout has used them
+  DevBuf<int32_t> tpos;
   DCsr M;
   DevBuf<double> dinv;
   DevBuf<float> dinv32;

@@ -63,6 +63,34 @@ __global__ void k_order_entries(int64_t n, const int32_t *rp, const int32_t *ci,
   }
 }
 
+// Entries of B^T in row order (row x = A row perm[x]): key = new column in
+// the low bits, x in the high 32; value = position of the transposed entry.
+// A stable sort on the low bits then lists every column y (= row y of B,
+// the pattern being symmetric) with its rows x ascending: B's rows, sorted.
+__global__ void k_order_entries_t(int64_t n, const int32_t *rp, const int32_t *ci, const int32_t *perm,
+                                  const int32_t *iperm, const int32_t *tpos, const int32_t *nrp, uint64_t *keys,
+                                  uint64_t *vals) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
+  for (int64_t r = w0; r < n; r += nw) {
+    const int32_t o = perm[r];
+    const int32_t a = rp[o], len = rp[o + 1] - a, d = nrp[r];
+    for (int32_t t = lane; t < len; t += WAVE) {
+      keys[d + t] = ((uint64_t)r << 32) | (uint64_t)(uint32_t)iperm[ci[a + t]];
+      vals[d + t] = (uint64_t)(uint32_t)tpos[a + t];
+    }
+  }
+}
+
+__global__ void k_order_unpack_t(int64_t m, const uint64_t *keys, const uint64_t *vals, const double *v,
+                                 int32_t *nci, double *nv) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
+    nci[k] = (int32_t)(keys[k] >> 32);
+    nv[k] = v[vals[k]];
+  }
+}
+
 __global__ void k_order_unpack(int64_t m, const uint64_t *keys, const uint64_t *vals, uint64_t mask,
                                const double *v, int32_t *nci, double *nv) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m;
@@ -172,7 +200,8 @@ void relabel_map(int64_t n, const int32_t *perm, const int32_t *map, const int32
   LG_LAUNCH_CHECK();
 }
 
-void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, DCsr &B, hipStream_t s) {
+void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, const int32_t *tpos, DCsr &B,
+                    hipStream_t s) {
   const int64_t n = A.n, m = A.nnz;
   DevBuf<int32_t> len(n + 1);
   hipLaunchKernelGGL(k_perm_len, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, A.rp.p, perm, len.p);
@@ -186,7 +215,17 @@ void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, DC
   B.nnz = m;
   B.ci.alloc(std::max<int64_t>(m, 1));
   B.v.alloc(std::max<int64_t>(m, 1));
-  if (m > 0) {
+  if (m > 0 && tpos) {  // one sort by column of B^T's entries (see k_order_entries_t)
+    const int cb = std::max(1, bits_for((uint64_t)n));
+    DevBuf<uint64_t> keys(m), vals(m);
+    hipLaunchKernelGGL(k_order_entries_t, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p,
+                       A.ci.p, perm, iperm, tpos, B.rp.p, keys.p, vals.p);
+    LG_LAUNCH_CHECK();
+    radix_sort_pairs(keys.p, vals.p, m, cb, s);
+    hipLaunchKernelGGL(k_order_unpack_t, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, m, keys.p, vals.p,
+                       A.v.p, B.ci.p, B.v.p);
+    LG_LAUNCH_CHECK();
+  } else if (m > 0) {  // a sort by (row, column)
     const int cb = bits_for((uint64_t)n);
     DevBuf<uint64_t> keys(m), vals(m);
     hipLaunchKernelGGL(k_order_entries, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p,
@@ -204,7 +243,7 @@ void degree_order(DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32_t> &
   DevBuf<int32_t> iperm;
   degree_perm(A, blocks, perm, iperm, s);
   DCsr B;
-  permute_sorted(A, perm.p, iperm.p, B, s);
+  permute_sorted(A, perm.p, iperm.p, nullptr, B, s);
   A = std::move(B);
 }
 

@@ -1250,7 +1250,7 @@ void jacobi_tv_sweep(int64_t n, const DCsr &A, const double *X, const double *di
 }  // namespace
 
 void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevBuf<double> &S,
-                      DevBuf<double> &cs, hipStream_t s) {
+                      DevBuf<double> &cs, DevBuf<int32_t> &tpos, hipStream_t s) {
   const int64_t n = A.n, nnz = A.nnz;
   const int T = o.num_tvs;
   const HugeRows &Hr = huge_of(A, s);
@@ -1258,7 +1258,7 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
   while (TS < T) TS <<= 1;
   DevBuf<double> X((size_t)n * TS), Y((size_t)n * TS), nrm(n), diag(n), raw(nnz), rowmax(n), colmax(n);
   DevBuf<uint8_t> hasd(n);
-  DevBuf<int32_t> tpos(nnz);
+  tpos.alloc(nnz);
   hipLaunchKernelGGL(k_diag, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p, diag.p, hasd.p);
   LG_LAUNCH_CHECK();
   if (o.strength_metric != LIGMG_STRENGTH_NONE) {
@@ -1405,7 +1405,7 @@ void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLe
   L.kind = LIGMG_LEVEL_AGG;
   L.n = n;
   DevBuf<double> cs;
-  compute_strength(A, o, level_idx, L.strength, cs, s);
+  compute_strength(A, o, level_idx, L.strength, cs, L.tpos, s);
   L.nc = run_voting(A, L.strength.p, cs.p, o, L.agg, L.ind, L.state, s);
   const int64_t nc = L.nc;
   // Galerkin R A P

@@ -24,8 +24,9 @@ void build_members(const int32_t *agg, int64_t n, int64_t nc, DCsr &M, hipStream
 bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &coarse, hipStream_t s);
 
 // Strength (per stored entry, 0 where pruned) and column sums.
+// tpos receives the positions of the transposed entries of A.
 void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevBuf<double> &S,
-                      DevBuf<double> &cs, hipStream_t s);
+                      DevBuf<double> &cs, DevBuf<int32_t> &tpos, hipStream_t s);
 // Column sums of a strength matrix on A's pattern.
 void col_sums_of(const DCsr &A, const double *S, DevBuf<double> &cs, hipStream_t s);
 // Voting aggregation with given strengths; returns the number of aggregates.
@@ -48,8 +49,11 @@ void degree_perm(const DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32
 void permute_rows(const DCsr &A, const int32_t *rperm, const int32_t *ciperm, int64_t ncols, DCsr &B,
                   hipStream_t s);
 // B = P A P^T with P given by perm (new -> old) / iperm (old -> new) and
-// every row's columns sorted; FP64 values only.
-void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, DCsr &B, hipStream_t s);
+// every row's columns sorted; FP64 values only.  tpos (optional): the
+// positions of A's transposed entries (structurally symmetric A), which
+// turn the sort by (row, column) into one by column.
+void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, const int32_t *tpos, DCsr &B,
+                    hipStream_t s);
 // out[i] = niperm[map[perm[i]]] (null: identity; negative map values kept)
 void relabel_map(int64_t n, const int32_t *perm, const int32_t *map, const int32_t *niperm, int32_t *out,
                  hipStream_t s);

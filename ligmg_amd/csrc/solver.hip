@@ -373,6 +373,7 @@ struct Solver {
     }
     phase("blocks");
     if (o.solve_layout && o.spmv_slice == 0) apply_solve_layout();
+    for (auto &L : H) L.tpos.release();
     phase("layout");
     // caller's vertex of each level-0 solve position
     if (H[0].sperm.p && perm_input.p) {
@@ -767,7 +768,7 @@ struct Solver {
       DLevel &V = H[l];
       if (!V.sperm.p) continue;
       DCsr A;
-      if (V.kind == LIGMG_LEVEL_AGG) permute_sorted(V.A, V.sperm.p, V.siperm.p, A, s);
+      if (V.kind == LIGMG_LEVEL_AGG) permute_sorted(V.A, V.sperm.p, V.siperm.p, V.tpos.p, A, s);
       else permute_rows(V.A, V.sperm.p, V.siperm.p, V.A.ncols, A, s);
       V.A = std::move(A);
     }
