@@ -153,8 +153,8 @@ def assert_same_hierarchy(d, r):
 
 @pytest.mark.parametrize("seed", range(4))
 def test_voting_with_injected_strength(seed):
-    """Same strengths in, same aggregates out (the north-star bit-exactness
-    condition), including ties: strengths drawn from a few values."""
+    """Same strengths in, same aggregates out (voting is bit-exact given its
+    strengths), including ties: strengths drawn from a few values."""
     g = graphs.rmat(11, 8, wkind=_ffi.W_UNIT, seed=seed)
     rng = np.random.default_rng(seed)
     rows = np.repeat(np.arange(g.n), np.diff(g.rowptr))
