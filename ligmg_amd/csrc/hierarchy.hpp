@@ -26,18 +26,24 @@ constexpr int SPMV_BLOCK = 256;
 constexpr int SPMV_UPW = 2;
 
 // Lanes that reduce one row of a complete-row unit: the smallest power of
-// two K <= 64 with ROW_LANE_ITEMS * K >= the unit's mean row length.  K = 1
-// sums a row left to right; otherwise lane g of the row's group sums
-// positions g, g + K, ... left to right and the K partials meet in an xor
-// butterfly (off = K/2..1).  16 entries per lane measured best on R-MAT 23
-// (FP32 Chebyshev step: 4 -> 1.353 ms, 8 -> 1.302, 16 -> 1.283, 32 -> 1.296).
-constexpr int ROW_LANE_ITEMS = 16;
-__host__ __device__ inline int row_lanes(int64_t cnt, int64_t nr) {
+// two K <= 64 with items * K >= the unit's mean row length.  K = 1 sums a
+// row left to right; otherwise lane g of the row's group sums positions
+// g, g + K, ... left to right and the K partials meet in an xor butterfly
+// (off = K/2..1).
+__host__ __device__ inline int row_lanes(int64_t cnt, int64_t nr, int items) {
   const int64_t avg = (cnt + nr - 1) / nr;
   int K = 1;
-  while (K < 64 && ROW_LANE_ITEMS * (int64_t)K < avg) K <<= 1;
+  while (K < 64 && items * (int64_t)K < avg) K <<= 1;
   return K;
 }
+
+// Entries per lane of a matrix's row reduction.  Large operators fill the
+// GPU many times over and run fastest with long per-lane runs (R-MAT 23
+// base level, FP32 Chebyshev step: 4 -> 1.353 ms, 8 -> 1.302, 16 -> 1.283,
+// 32 -> 1.296); small ones launch few workgroups and are latency-bound,
+// where short runs win (R-MAT 22 K-cycle: 16 everywhere is 23% slower).
+constexpr int64_t ROW_ITEMS_BIG_NNZ = (int64_t)16 << 20;
+inline int row_items_for(int64_t nnz) { return nnz >= ROW_ITEMS_BIG_NNZ ? 16 : 4; }
 
 struct Units {
   int64_t nunits = 0, nlong = 0, nparts = 0;
@@ -47,6 +53,7 @@ struct Units {
   DevBuf<int32_t> lrow;   // long rows
   DevBuf<int32_t> lpart;  // nlong + 1 slot offsets
   DevBuf<double> parts;   // partial sums (also used as float storage)
+  int32_t row_items = 4;  // entries per lane of the row reduction (row_items_for)
 };
 
 struct HugeRows;  // rows too long for one wavefront of the setup row kernels

@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(SPMV_BLOCK)
     k_spmv_units(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, Val val,
                  const T *__restrict__ x, int32_t xbytes, const int32_t *__restrict__ urow0,
                  const int32_t *__restrict__ unrows, const int32_t *__restrict__ ukoff, int64_t nunits,
-                 double *__restrict__ parts, Epi epi) {
+                 int row_items, double *__restrict__ parts, Epi epi) {
   constexpr int IT = UNIT_NNZ / SPMV_BLOCK;
   __shared__ double prod[UNIT_NNZ];
   __shared__ double ws[SPMV_BLOCK / WAVE];
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(SPMV_BLOCK)
       }
       if (u + 1 < uend) fetch(u + 1);
       __syncthreads();
-      const int K = row_lanes(cnt, nr);
+      const int K = row_lanes(cnt, nr, row_items);
       if (K == 1) {
         for (int r = threadIdx.x; r < nr; r += SPMV_BLOCK) {
           const int a = rp[row0 + r] - k0, b = rp[row0 + r + 1] - k0;
@@ -166,12 +166,12 @@ void launch_spmv(const DCsr &A, Val val, const T *x, Epi epi, hipStream_t s) {
   const int64_t xbytes = A.ncols * (int64_t)sizeof(T);
   if (xbytes < ((int64_t)1 << 31))
     hipLaunchKernelGGL((k_spmv_units<T, Val, Epi, true>), dim3((unsigned)grid), dim3(SPMV_BLOCK), 0, s, A.rp.p,
-                       A.ci.p, val, x, (int32_t)xbytes, A.U.row0.p, A.U.nrows.p, A.U.koff.p, A.U.nunits, parts,
-                       epi);
+                       A.ci.p, val, x, (int32_t)xbytes, A.U.row0.p, A.U.nrows.p, A.U.koff.p, A.U.nunits,
+                       A.U.row_items, parts, epi);
   else
     hipLaunchKernelGGL((k_spmv_units<T, Val, Epi, false>), dim3((unsigned)grid), dim3(SPMV_BLOCK), 0, s,
                        A.rp.p, A.ci.p, val, x, (int32_t)0, A.U.row0.p, A.U.nrows.p, A.U.koff.p, A.U.nunits,
-                       parts, epi);
+                       A.U.row_items, parts, epi);
   LG_LAUNCH_CHECK();
   if (A.U.nlong) {
     hipLaunchKernelGGL((k_spmv_long<Epi>), dim3(grid_for(A.U.nlong, 64)), dim3(64), 0, s, A.U.lrow.p,
@@ -448,6 +448,7 @@ void build_units(DCsr &A, hipStream_t s) {
   lpart.push_back((int32_t)parts);
   Units &U = A.U;
   U.nunits = (int64_t)row0.size();
+  U.row_items = row_items_for(rp[n] - rp[0]);
   U.nlong = (int64_t)lrow.size();
   U.nparts = parts;
   U.row0.alloc(U.nunits);

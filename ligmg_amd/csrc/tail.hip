@@ -346,7 +346,7 @@ void tail_schedule(const DCsr &A, TailSched &S, hipStream_t s) {
     const int32_t nr = nrows[u];
     if (nr > 0) {
       // the lanes k_spmv_units gives each row of the unit
-      const int K = row_lanes(rp[row0[u] + nr] - koff[u], nr);
+      const int K = row_lanes(rp[row0[u] + nr] - koff[u], nr, U.row_items);
       int c = 0;
       while ((2 << c) < K) c++;
       auto &dst = K == 1 ? tr : gr[c];
