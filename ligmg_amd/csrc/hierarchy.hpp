@@ -22,8 +22,11 @@ constexpr int UNIT_NNZ = 2048;
 constexpr int UNIT_ROWS = 512;
 constexpr int SPMV_BLOCK = 256;
 // units per SpMV workgroup: the colind/value loads of the next unit are in
-// flight while the rows of the current one are reduced
+// flight while the rows of the current one are reduced.  Operators of few
+// units take one unit per workgroup, so that they spread over more CUs.
 constexpr int SPMV_UPW = 2;
+constexpr int64_t SPMV_UPW_MIN_UNITS = 8192;
+inline int spmv_upw(int64_t nunits) { return nunits >= SPMV_UPW_MIN_UNITS ? SPMV_UPW : 1; }
 
 // Lanes that reduce one row of a complete-row unit: the smallest power of
 // two K <= 64 with items * K >= the unit's mean row length.  K = 1 sums a

@@ -41,7 +41,7 @@ __device__ __forceinline__ double buf_gather(__amdgpu_buffer_rsrc_t r, int col) 
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (uint32_t)col * 8u, 0, 0));
 }
 
-// Each workgroup takes SPMV_UPW consecutive units.  A complete-row unit
+// Each workgroup takes upw consecutive units (spmv_upw).  A complete-row unit
 // stages every product in LDS with coalesced loads of colind/values (the x
 // gathers are the only scattered accesses); its rows are then reduced from
 // LDS by groups of row_lanes() lanes while the next unit's colind/values
@@ -54,12 +54,12 @@ __global__ void __launch_bounds__(SPMV_BLOCK)
     k_spmv_units(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, Val val,
                  const T *__restrict__ x, int32_t xbytes, const int32_t *__restrict__ urow0,
                  const int32_t *__restrict__ unrows, const int32_t *__restrict__ ukoff, int64_t nunits,
-                 int row_items, double *__restrict__ parts, Epi epi) {
+                 int upw, int row_items, double *__restrict__ parts, Epi epi) {
   constexpr int IT = UNIT_NNZ / SPMV_BLOCK;
   __shared__ double prod[UNIT_NNZ];
   __shared__ double ws[SPMV_BLOCK / WAVE];
-  const int64_t ubeg = (int64_t)blockIdx.x * SPMV_UPW;
-  const int64_t uend = min(ubeg + SPMV_UPW, nunits);
+  const int64_t ubeg = (int64_t)blockIdx.x * upw;
+  const int64_t uend = min(ubeg + upw, nunits);
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
   int col[IT];
   T vv[IT];
@@ -162,16 +162,17 @@ template <class T, class Val, class Epi>
 void launch_spmv(const DCsr &A, Val val, const T *x, Epi epi, hipStream_t s) {
   if (A.U.nunits == 0) return;
   double *parts = A.U.parts.p;
-  const int64_t grid = (A.U.nunits + SPMV_UPW - 1) / SPMV_UPW;
+  const int upw = spmv_upw(A.U.nunits);
+  const int64_t grid = (A.U.nunits + upw - 1) / upw;
   const int64_t xbytes = A.ncols * (int64_t)sizeof(T);
   if (xbytes < ((int64_t)1 << 31))
     hipLaunchKernelGGL((k_spmv_units<T, Val, Epi, true>), dim3((unsigned)grid), dim3(SPMV_BLOCK), 0, s, A.rp.p,
-                       A.ci.p, val, x, (int32_t)xbytes, A.U.row0.p, A.U.nrows.p, A.U.koff.p, A.U.nunits,
+                       A.ci.p, val, x, (int32_t)xbytes, A.U.row0.p, A.U.nrows.p, A.U.koff.p, A.U.nunits, upw,
                        A.U.row_items, parts, epi);
   else
     hipLaunchKernelGGL((k_spmv_units<T, Val, Epi, false>), dim3((unsigned)grid), dim3(SPMV_BLOCK), 0, s,
                        A.rp.p, A.ci.p, val, x, (int32_t)0, A.U.row0.p, A.U.nrows.p, A.U.koff.p, A.U.nunits,
-                       A.U.row_items, parts, epi);
+                       upw, A.U.row_items, parts, epi);
   LG_LAUNCH_CHECK();
   if (A.U.nlong) {
     hipLaunchKernelGGL((k_spmv_long<Epi>), dim3(grid_for(A.U.nlong, 64)), dim3(64), 0, s, A.U.lrow.p,
