@@ -25,7 +25,7 @@ constexpr int SPMV_BLOCK = 256;
 // flight while the rows of the current one are reduced.  Operators of few
 // units take one unit per workgroup, so that they spread over more CUs.
 constexpr int SPMV_UPW = 2;
-constexpr int64_t SPMV_UPW_MIN_UNITS = 8192;
+constexpr int64_t SPMV_UPW_MIN_UNITS = 32768;
 inline int spmv_upw(int64_t nunits) { return nunits >= SPMV_UPW_MIN_UNITS ? SPMV_UPW : 1; }
 
 // Lanes that reduce one row of a complete-row unit: the smallest power of
