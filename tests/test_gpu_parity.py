@@ -466,6 +466,25 @@ def test_hierarchy_bit_exact_with_rows_beyond_one_chunk(ordering):
     assert_same_hierarchy(d, O.OracleSolver(g, oo))
 
 
+def test_level_spmv_large_operator():
+    """Operators of at least 32768 work units take two units per workgroup
+    and 16 entries per lane in their row reduction: the base level of R-MAT
+    22 against scipy (FP64 and the FP32 Laplacian form)."""
+    import scipy.sparse as sp
+    g = graphs.rmat_config(22, "lognormal", pendants=True)
+    d = ligmg_amd.Solver(g, ligmg_amd.Options(precision=_ffi.PREC_MIXED))
+    l = 1 if d.level_info(0)[0] == _ffi.LEVEL_ELIM else 0
+    rp, ci, v = d.level_matrix(l)
+    n = len(rp) - 1
+    assert len(ci) >= 32768 * 2048  # a unit holds at most 2048 entries
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    x = np.random.default_rng(7).standard_normal(n)
+    y_ref = A @ x
+    scale = abs(A) @ abs(x) + 1e-300
+    assert np.all(np.abs(d.level_spmv(l, x, _ffi.PREC_FP64) - y_ref) <= 1e-13 * scale)
+    assert np.all(np.abs(d.level_spmv(l, x, _ffi.PREC_MIXED) - y_ref) <= 2e-6 * scale)
+
+
 @pytest.mark.parametrize("prec", [_ffi.PREC_FP64, _ffi.PREC_MIXED])
 def test_sliced_solve_matches_unsliced(prec):
     g = make_case("rmat13_unif")
