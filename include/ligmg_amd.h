@@ -213,7 +213,10 @@ int ligmg_ordering(ligmg_handle h, int32_t *perm);
 /* (new) Device memory the library keeps for reuse: temporaries of setup and
  * of freed solvers are cached by size class (at most 32 GiB) instead of
  * returned to HIP, which makes repeated setups in one process as fast as
- * the first.  This returns all of it to HIP; the library never needs it. */
+ * the first.  This returns all of it to HIP; the library never needs it.
+ * Returns LIGMG_ERR_INVALID, releasing nothing, while another thread
+ * captures a handle's preconditioner graph (a device synchronisation would
+ * end that capture). */
 int ligmg_trim_memory(void);
 
 /* Thread-local message of the last failed call ("" if none). */

@@ -120,7 +120,9 @@ void dev_free(void *p) {
   if (it == P.live.end()) return;
   const Pool::Key key = it->second;
   P.live.erase(it);
-  if (P.cached + key.second > CACHE_MAX) {
+  // hipFree synchronises; during a capture the block is parked even above
+  // the cap, which the next free outside a capture enforces again
+  if (P.cached + key.second > CACHE_MAX && g_capturing.load() == 0) {
     (void)hipFree(p);
     return;
   }
@@ -128,12 +130,20 @@ void dev_free(void *p) {
   P.cached += key.second;
 }
 
-void pool_capture_begin() { g_capturing.fetch_add(1); }
+// Taken under the pool lock, so a synchronisation or free that the pool has
+// already started ends before the capture begins.
+void pool_capture_begin() {
+  Pool &P = pool();
+  std::lock_guard<std::mutex> g(P.m);
+  g_capturing.fetch_add(1);
+}
 void pool_capture_end() { g_capturing.fetch_sub(1); }
 
 void dev_pool_trim() {
   Pool &P = pool();
   std::lock_guard<std::mutex> g(P.m);
+  if (g_capturing.load() != 0)
+    throw Error(LIGMG_ERR_INVALID, "ligmg_trim_memory: a preconditioner graph capture is in progress");
   flush_locked(P);
 }
 
@@ -144,6 +154,7 @@ int ligmg_trim_memory(void) {
     lg::dev_pool_trim();
     return LIGMG_OK;
   } catch (const lg::Error &e) {
+    lg::g_last_error = e.what();
     return e.code;
   }
 }
