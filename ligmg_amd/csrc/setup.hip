@@ -1449,9 +1449,43 @@ __global__ void k_mul(int n, const double *a, const double *b, double *c) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) c[i] = a[i] * b[i];
 }
-__global__ void k_xpby(int n, const double *x, double beta, double *p) {
+__global__ void k_xpby_dev(int n, const double *x, const double *beta, double *p) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = x[i] + beta * p[i];
+  if (i < n) p[i] = x[i] + beta[0] * p[i];
+}
+
+// Scalars of the estimator.  They stay on the device; the recorded alpha and
+// beta are read once, after the last step.  The stop rules are those of the
+// loop `while gamma > 0: ...; if !(delta > 0) break`.
+enum { EG_GAMMA = 0, EG_DELTA, EG_GN, EG_ALPHA, EG_BETA, EG_ACTIVE, EG_M, EG_N };
+
+// after gamma = (r, z)
+__global__ void k_eig_start(double *sc) {
+  sc[EG_ACTIVE] = sc[EG_GAMMA] > 0 ? 1.0 : 0.0;
+  sc[EG_M] = 0;
+}
+// after delta = (p, A p): alpha = gamma / delta, or stop
+__global__ void k_eig_alpha(double *sc) {
+  double a = 0;
+  if (sc[EG_ACTIVE] != 0) {
+    if (sc[EG_DELTA] > 0) a = sc[EG_GAMMA] / sc[EG_DELTA];
+    else sc[EG_ACTIVE] = 0;
+  }
+  sc[EG_ALPHA] = a;
+}
+// after gn = (r, z): record (alpha, gn / gamma), gamma = gn, stop unless gn > 0
+__global__ void k_eig_beta(double *sc, double *al, double *be) {
+  double b = 0;
+  if (sc[EG_ACTIVE] != 0) {
+    const int m = (int)sc[EG_M];
+    b = sc[EG_GN] / sc[EG_GAMMA];
+    al[m] = sc[EG_ALPHA];
+    be[m] = b;
+    sc[EG_M] = m + 1;
+    sc[EG_GAMMA] = sc[EG_GN];
+    if (!(sc[EG_GN] > 0)) sc[EG_ACTIVE] = 0;
+  }
+  sc[EG_BETA] = b;
 }
 }  // namespace
 
@@ -1489,7 +1523,7 @@ double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_
   if (n == 0) return 1.0;
   RedPlan R;
   R.build(n, {0, n}, s);
-  DevBuf<double> r(n), z(n), p(n), q(n), sc(4);
+  DevBuf<double> r(n), z(n), p(n), q(n), sc(EG_N), al(std::max(iters, 1)), be(std::max(iters, 1));
   hipLaunchKernelGGL(k_eig_init, dim3(g1(n)), dim3(256), 0, s, (int)n, seed,
                      ligmg_rng::STREAM_EIG + (uint64_t)level_idx, perm, r.p);
   LG_LAUNCH_CHECK();
@@ -1497,38 +1531,38 @@ double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_
   op_sub_block_mean(r.p, R, sc.p, s);
   hipLaunchKernelGGL(k_mul, dim3(g1(n)), dim3(256), 0, s, (int)n, dinv, r.p, z.p);
   d2d(p.p, z.p, n, s);
-  auto dot = [&](const double *a, const double *b) {
+  auto dot = [&](const double *a, const double *b, int slot) {
     const double *A_[1] = {a}, *B_[1] = {b};
-    R.dots(1, A_, B_, sc.p, s);
-    double h;
-    to_host(&h, sc.p, 1, s);
-    LG_HIP(hipStreamSynchronize(s));
-    return h;
+    R.dots(1, A_, B_, sc.p + slot, s);
   };
-  double gamma = dot(r.p, z.p);
-  std::vector<double> al, be;
-  for (int it = 0; it < iters && gamma > 0; it++) {
+  dot(r.p, z.p, EG_GAMMA);
+  hipLaunchKernelGGL(k_eig_start, dim3(1), dim3(1), 0, s, sc.p);
+  LG_LAUNCH_CHECK();
+  // a stopped estimator runs the remaining steps with alpha = beta = 0 and
+  // records nothing more
+  for (int it = 0; it < iters; it++) {
     op_spmv<double>(A, p.p, q.p, s);
-    double delta = dot(p.p, q.p);
-    if (!(delta > 0)) break;
-    double alpha = gamma / delta;
-    to_device(sc.p + 1, &alpha, 1, s);
-    hipLaunchKernelGGL(k_axpy_dev, dim3(g1(n)), dim3(256), 0, s, (int)n, sc.p + 1, -1.0, q.p, r.p);
+    dot(p.p, q.p, EG_DELTA);
+    hipLaunchKernelGGL(k_eig_alpha, dim3(1), dim3(1), 0, s, sc.p);
+    hipLaunchKernelGGL(k_axpy_dev, dim3(g1(n)), dim3(256), 0, s, (int)n, sc.p + EG_ALPHA, -1.0, q.p, r.p);
     hipLaunchKernelGGL(k_mul, dim3(g1(n)), dim3(256), 0, s, (int)n, dinv, r.p, z.p);
-    double gn = dot(r.p, z.p);
-    double beta = gn / gamma;
-    al.push_back(alpha);
-    be.push_back(beta);
-    hipLaunchKernelGGL(k_xpby, dim3(g1(n)), dim3(256), 0, s, (int)n, z.p, beta, p.p);
-    gamma = gn;
+    dot(r.p, z.p, EG_GN);
+    hipLaunchKernelGGL(k_eig_beta, dim3(1), dim3(1), 0, s, sc.p, al.p, be.p);
+    hipLaunchKernelGGL(k_xpby_dev, dim3(g1(n)), dim3(256), 0, s, (int)n, z.p, sc.p + EG_BETA, p.p);
+    LG_LAUNCH_CHECK();
   }
+  double h_sc[EG_N];
+  std::vector<double> ha(al.n), hb(be.n);
+  to_host(h_sc, sc.p, EG_N, s);
+  to_host(ha.data(), al.p, al.n, s);
+  to_host(hb.data(), be.p, be.n, s);
   LG_HIP(hipStreamSynchronize(s));
-  const int m = (int)al.size();
+  const int m = (int)h_sc[EG_M];  // entries beyond m were never written
   if (m == 0) return 1.0;
   std::vector<double> d(m), e(m > 1 ? m - 1 : 0);
   for (int j = 0; j < m; j++) {
-    d[j] = 1 / al[j] + (j > 0 ? be[j - 1] / al[j - 1] : 0);
-    if (j + 1 < m) e[j] = std::sqrt(be[j]) / al[j];
+    d[j] = 1 / ha[j] + (j > 0 ? hb[j - 1] / ha[j - 1] : 0);
+    if (j + 1 < m) e[j] = std::sqrt(hb[j]) / ha[j];
   }
   return tridiag_max_eig(d, e);
 }
