@@ -179,6 +179,19 @@ int ligmg_create(int64_t n, const int32_t *rowptr, const int32_t *colind,
 int ligmg_solve(ligmg_handle h, const double *b, double *x, int mem_kind,
                 ligmg_info *info, double *history, int32_t history_cap);
 
+/* Solver::solve(b, from) (ligmg.hpp:36-38 -> ligmg::solve, multigrid.hpp:
+ * 315-370 with from = level): solves the system of hierarchy level `level`.
+ * b and x have that level's n entries (ligmg_level_info) in mem_kind memory,
+ * numbered as ligmg_level_matrix reports the level (the caller's order for
+ * level 0, where this is ligmg_solve).  When the level is an elimination
+ * level and top_elim is set, it is applied once around the outer Krylov
+ * solve on level + 1; info->full_rel_residual is taken on the level's own
+ * matrix and info->base_level names the outer solve's level.  The cycle
+ * complexity divides by the nnz of level 0, as the reference's does. */
+int ligmg_solve_from(ligmg_handle h, int32_t level, const double *b, double *x,
+                     int mem_kind, ligmg_info *info, double *history,
+                     int32_t history_cap);
+
 /* k right-hand sides at once (column-major, leading dimension n), one
  * hierarchy traversal per cycle with SpMM kernels; every (block, column)
  * pair runs its own flexible-CG coefficients and convergence test.

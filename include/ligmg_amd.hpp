@@ -3,6 +3,7 @@
 //
 //   ligmg::Solver<T,F,DER>(A, opts) / Solver(A)   include/ligmg/ligmg.hpp:17-28
 //   Solver::solve(b)                              include/ligmg/ligmg.hpp:31-33
+//   Solver::solve(b, from)                        include/ligmg/ligmg.hpp:36-38
 //   Solver::print()                               include/ligmg/ligmg.hpp:40
 //   Solver::dump_csv(os, info)                    include/ligmg/ligmg.hpp:42-45
 //   Solver::dump_hierarchy(dir)                   include/ligmg/ligmg.hpp:47-52
@@ -96,6 +97,27 @@ class Solver {
     check(ligmg_solve(h_, b.data(), ci.solution.data(), LIGMG_MEM_HOST, &ci.info, hist.data(),
                       (int32_t)hist.size()),
           "ligmg_solve");
+    hist.resize(ci.info.history_len);
+    ci.residuals = std::move(hist);
+    ci._iters = ci.info.iters;
+    ci._time = ci.info.time;
+    ci._work = ci.info.work;
+    return ci;
+  }
+
+  // solve(b, from): the system of hierarchy level `from`; b has that
+  // level's size and is numbered as ligmg_level_matrix reports the level.
+  ConvergenceInfo solve(const std::vector<double> &b, int from) const {
+    int32_t kind;
+    int64_t n, nnz, nc;
+    check(ligmg_level_info(h_, from, &kind, &n, &nnz, &nc), "ligmg_level_info");
+    if ((int64_t)b.size() != n) throw Error(LIGMG_ERR_INVALID, "solve: b has the wrong length for the level");
+    ConvergenceInfo ci;
+    ci.solution.resize(n);
+    std::vector<double> hist(opts.iters + 2);
+    check(ligmg_solve_from(h_, from, b.data(), ci.solution.data(), LIGMG_MEM_HOST, &ci.info, hist.data(),
+                           (int32_t)hist.size()),
+          "ligmg_solve_from");
     hist.resize(ci.info.history_len);
     ci.residuals = std::move(hist);
     ci._iters = ci.info.iters;

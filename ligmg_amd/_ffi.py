@@ -80,6 +80,8 @@ SOLVER_PROTOS = {
                                P(ligmg_options), i32, P(i64), P(HANDLE)]),
     "ligmg_solve": (C.c_int, [HANDLE, C.c_void_p, C.c_void_p, C.c_int,
                               P(ligmg_info), P(f64), i32]),
+    "ligmg_solve_from": (C.c_int, [HANDLE, i32, C.c_void_p, C.c_void_p, C.c_int,
+                                   P(ligmg_info), P(f64), i32]),
     "ligmg_solve_batch": (C.c_int, [HANDLE, i32, C.c_void_p, C.c_void_p, C.c_int,
                                     P(ligmg_info), P(i32)]),
     "ligmg_edge_grad": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -11,7 +11,11 @@ because L is symmetric:
 The last line follows from dL = sum_e dw_e (u_i - u_j)(u_i - u_j)^T and
 d(L^+) b = -L^+ dL L^+ b for a connected graph (the nullspace, the
 constants, does not move with w).  The graph must be connected; b and the
-incoming gradient are projected to mean zero.
+incoming gradient are projected to mean zero, and so are the solutions: the
+solver returns L^+ b plus a multiple of the constant vector (with a top
+elimination level, x = P x_c + Q b is not mean-zero), and L^+ b is the
+mean-zero one, so the forward value and both gradients are exactly those of
+x = L^+ b for any loss, shift-invariant or not.
 """
 import ctypes as C
 
@@ -71,6 +75,7 @@ class LaplacianSolve(torch.autograd.Function):
         bb = b.detach().to(torch.float64)
         bb = bb - bb.mean()
         x = solver.solve(bb).solution
+        x = x - x.mean()
         ctx.solver = solver
         ctx.pattern = pattern
         ctx.save_for_backward(x)
@@ -83,6 +88,7 @@ class LaplacianSolve(torch.autograd.Function):
         g = gx.detach().to(torch.float64)
         g = g - g.mean()
         lam = ctx.solver.solve(g).solution
+        lam = lam - lam.mean()
         P = ctx.pattern
         gw = torch.empty(P.m, dtype=torch.float64, device=x.device)
         torch.cuda.current_stream().synchronize()

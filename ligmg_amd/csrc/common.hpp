@@ -151,6 +151,44 @@ __device__ __forceinline__ float wave_tree_sum(float v) {
   return v;
 }
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+
+// Orders the LDS accesses of one wavefront around this point: LDS executes a
+// wavefront's instructions in order, so after the fence every lane sees
+// what the other lanes stored before it.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Rows (segments) longer than this are folded by a whole wavefront
+// (wave_seq_fold); shorter ones by one lane each.
+constexpr int SEQ_LANE = 256;
+
+// Left fold from 0.0 of term(k), k = a, a+1, ..., b-1, by one wavefront:
+// the exact sequential sum, computed by every lane.  The terms are read 64
+// at a time (coalesced; the next 64 are in flight during the fold), staged
+// in buf (this wavefront's 64-entry LDS slot) and added in index order.
+template <class Term>
+__device__ __forceinline__ double wave_seq_fold(int64_t a, int64_t b, double *buf, const Term &term) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  double nxt = (a + lane < b) ? term(a + lane) : 0.0;
+  double acc = 0.0;
+  for (int64_t base = a; base < b; base += WAVE) {
+    buf[lane] = nxt;
+    wave_lds_sync();
+    nxt = (base + WAVE + lane < b) ? term(base + WAVE + lane) : 0.0;
+    if (b - base >= WAVE) {
+#pragma unroll
+      for (int t = 0; t < WAVE; t++) acc += buf[t];
+    } else {
+      const int cnt = (int)(b - base);
+      for (int t = 0; t < cnt; t++) acc += buf[t];
+    }
+    wave_lds_sync();
+  }
+  return acc;
+}
 #endif
 
 }  // namespace lg

@@ -464,7 +464,7 @@ int ligmg_adjacency_to_laplacian(int64_t n, const int32_t *rowptr, const int32_t
     LG_LAUNCH_CHECK();
     const int64_t tot = lg::exclusive_scan_i32(len.p, orp.p, n + 1, s);
     lg::DevBuf<double> deg(n);
-    lg::row_sums_canonical(n, rp.p, v.p, deg.p, s);
+    lg::row_sums_seq(n, rp.p, v.p, deg.p, s);
     lg::DevBuf<int32_t> oci(std::max<int64_t>(tot, 1));
     lg::DevBuf<double> ov(std::max<int64_t>(tot, 1));
     hipLaunchKernelGGL(lg::k_lap_fill, dim3(lg::gs(n)), dim3(256), 0, s, n, rp.p, ci.p, v.p,

@@ -338,6 +338,70 @@ int64_t segmented_canon_sum(const uint64_t *keys, const uint64_t *vals, int64_t 
   return nseg;
 }
 
+namespace {
+// One lane per segment of at most SEQ_LANE items, summed left to right; a
+// longer segment is appended to the long list (in any order: each is summed
+// on its own by k_seg_seq_long).
+__global__ void __launch_bounds__(256) k_seg_seq(const uint64_t *keys, const uint64_t *vals,
+                                                 const int32_t *start, int64_t nseg, int64_t n,
+                                                 uint64_t *out_keys, double *out_sums,
+                                                 int32_t *nlong, int32_t *longs) {
+  const int64_t seg = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (seg >= nseg) return;
+  const int64_t a = start[seg], b = (seg + 1 < nseg) ? start[seg + 1] : n;
+  out_keys[seg] = keys[a];
+  if (b - a > SEQ_LANE) {
+    longs[atomicAdd(nlong, 1)] = (int32_t)seg;
+    return;
+  }
+  double acc = 0.0;
+#pragma unroll 8
+  for (int64_t i = a; i < b; i++) acc += __longlong_as_double((long long)vals[i]);
+  out_sums[seg] = acc;
+}
+
+// One wavefront per long segment; the grid covers the most long segments
+// there can be, and waves past the count exit.
+__global__ void __launch_bounds__(256) k_seg_seq_long(const uint64_t *vals, const int32_t *start,
+                                                      int64_t nseg, int64_t n, const int32_t *nlong,
+                                                      const int32_t *longs, double *out_sums) {
+  __shared__ double buf[256];
+  const int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (j >= *nlong) return;
+  const int64_t seg = longs[j];
+  const int64_t a = start[seg], b = (seg + 1 < nseg) ? start[seg + 1] : n;
+  const double acc = wave_seq_fold(a, b, buf + (threadIdx.x / WAVE) * WAVE,
+                                   [&](int64_t i) { return __longlong_as_double((long long)vals[i]); });
+  if ((threadIdx.x & (WAVE - 1)) == 0) out_sums[seg] = acc;
+}
+}  // namespace
+
+int64_t segmented_seq_sum(const uint64_t *keys, const uint64_t *vals, int64_t n,
+                          uint64_t *out_keys, double *out_sums, hipStream_t s) {
+  if (n <= 0) return 0;
+  LG_CHECK(n < ((int64_t)1 << 31), 1, "segmented_seq_sum: too many items");
+  DevBuf<int32_t> flags(n), pos(n);
+  hipLaunchKernelGGL(k_seg_heads, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, n, flags.p);
+  LG_LAUNCH_CHECK();
+  int64_t nseg = exclusive_scan_i32(flags.p, pos.p, n, s);
+  DevBuf<int32_t> start(nseg);
+  hipLaunchKernelGGL(k_compact_write, dim3(grid_for(n, 256)), dim3(256), 0, s, flags.p, pos.p, n,
+                     start.p);
+  LG_LAUNCH_CHECK();
+  const int64_t maxlong = n / (SEQ_LANE + 1);
+  DevBuf<int32_t> nlong(1), longs(std::max<int64_t>(maxlong, 1));
+  fill_i32(nlong.p, 0, 1, s);
+  hipLaunchKernelGGL(k_seg_seq, dim3((unsigned)((nseg + 255) / 256)), dim3(256), 0, s, keys, vals,
+                     start.p, nseg, n, out_keys, out_sums, nlong.p, longs.p);
+  LG_LAUNCH_CHECK();
+  if (maxlong > 0) {
+    hipLaunchKernelGGL(k_seg_seq_long, dim3((unsigned)((maxlong + 3) / 4)), dim3(256), 0, s, vals,
+                       start.p, nseg, n, nlong.p, longs.p, out_sums);
+    LG_LAUNCH_CHECK();
+  }
+  return nseg;
+}
+
 int64_t compact_flags(const int32_t *flags, int64_t n, int32_t *out, hipStream_t s) {
   if (n <= 0) return 0;
   DevBuf<int32_t> pos(n);

@@ -22,6 +22,13 @@ void radix_sort_pairs(uint64_t *keys, uint64_t *vals, int64_t n, int bits, hipSt
 int64_t segmented_canon_sum(const uint64_t *keys, const uint64_t *vals, int64_t n,
                             uint64_t *out_keys, double *out_sums, hipStream_t s);
 
+// Sequential sum (a left fold from 0.0 in array order) of the values of
+// each run of equal keys in a sorted key array: the order of a loop that
+// adds the items one by one.  Same outputs and return value as
+// segmented_canon_sum.
+int64_t segmented_seq_sum(const uint64_t *keys, const uint64_t *vals, int64_t n,
+                          uint64_t *out_keys, double *out_sums, hipStream_t s);
+
 // out[k] = index of the k-th nonzero flag (ascending); returns the count.
 int64_t compact_flags(const int32_t *flags, int64_t n, int32_t *out, hipStream_t s);
 

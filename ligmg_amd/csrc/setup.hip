@@ -2,12 +2,19 @@
 // aggregation, Galerkin products by contraction (sort + canonical segmented
 // sum), smoother set-up and the coarse pseudo-inverse.
 //
-// Every floating-point sum whose order could change a result uses the
-// canonical order: a row (or run of equal keys) of at most 64 terms is
-// summed left to right by one lane; a longer one by a whole wavefront as 64
-// strided partial sums combined by the tree p[l] += p[l+off], off = 32..1.
-// Masked row positions contribute +0.0.  Files compiled with
-// -ffp-contract=off, so products and sums round exactly as written.
+// Summation orders.  The sums that the reference computes on one rank in a
+// fixed order of its own are left folds from 0.0 in ascending index
+// (run_rows_seq, segmented_seq_sum): col_sum and the Row/Column scaling
+// sums (CombBLAS Reduce), the vote tally (tally_votes' loop over voters)
+// and the conductance sums (ext_int_ratio's loop).  Every other
+// floating-point sum whose order could change a result (SpMV-shaped sums
+// whose order the reference leaves to CombBLAS/PETSc) uses the canonical
+// order: a row (or run of equal keys) of at most 64 terms is summed left to
+// right by one lane; a longer one by a whole wavefront as 64 strided partial
+// sums combined by the tree p[l] += p[l+off], off = 32..1.  Masked row
+// positions contribute +0.0 (which changes neither kind of sum).  Files
+// compiled with -ffp-contract=off, so products and sums round exactly as
+// written.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -119,18 +126,21 @@ __global__ void k_rows_huge(int64_t nrows, const int32_t *row, const int32_t *cp
 
 }  // namespace
 
-// Rows longer than CHUNK of one matrix and their chunks.
+// Rows longer than CHUNK of one matrix and their chunks, and the rows
+// longer than SEQ_LANE (summed by a wavefront each in sequential order).
 struct HugeRows {
   int64_t nrows = 0, nchunks = 0;
   DevBuf<int32_t> row, cptr, cstart, cend, crow;
   DevBuf<uint8_t> cacc;
+  int64_t nlong = 0;
+  DevBuf<int32_t> lrow;
 };
 
 namespace {
 
-__global__ void k_huge_flags(int64_t n, const int32_t *rp, int32_t *flag) {
+__global__ void k_huge_flags(int64_t n, const int32_t *rp, int32_t lim, int32_t *flag) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    flag[i] = (rp[i + 1] - rp[i] > CHUNK) ? 1 : 0;
+    flag[i] = (rp[i + 1] - rp[i] > lim) ? 1 : 0;
 }
 __global__ void k_huge_nchunks(int64_t nh, const int32_t *row, const int32_t *rp, int32_t *cnt) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
@@ -158,9 +168,18 @@ __global__ void k_huge_fill(int64_t nh, const int32_t *row, const int32_t *rp, c
 void find_huge(int64_t n, const int32_t *rp, HugeRows &Hr, hipStream_t s) {
   Hr.nrows = 0;
   Hr.nchunks = 0;
+  Hr.nlong = 0;
   if (n <= 0) return;
   DevBuf<int32_t> flag(n), idx(n);
-  hipLaunchKernelGGL(k_huge_flags, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, rp, flag.p);
+  hipLaunchKernelGGL(k_huge_flags, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, rp, (int32_t)SEQ_LANE,
+                     flag.p);
+  LG_LAUNCH_CHECK();
+  Hr.nlong = compact_flags(flag.p, n, idx.p, s);
+  if (Hr.nlong == 0) return;  // no row is longer than CHUNK either
+  Hr.lrow.alloc(Hr.nlong);
+  d2d(Hr.lrow.p, idx.p, Hr.nlong, s);
+  hipLaunchKernelGGL(k_huge_flags, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, rp, (int32_t)CHUNK,
+                     flag.p);
   LG_LAUNCH_CHECK();
   const int64_t nh = compact_flags(flag.p, n, idx.p, s);
   if (nh == 0) return;
@@ -205,6 +224,52 @@ void run_rows(int64_t n, const int32_t *rp, const Op &op, const HugeRows &Hr, hi
     LG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_rows_huge<Op>, dim3((unsigned)((Hr.nrows + 63) / 64)), dim3(64), 0, s, Hr.nrows,
                        Hr.row.p, Hr.cptr.p, op, (const Acc *)cacc);
+    LG_LAUNCH_CHECK();
+  }
+}
+
+// Sequential row sums for DblSum operators: each row's terms folded from
+// 0.0 in stored (ascending column) order.  Rows of at most SEQ_LANE entries
+// take one lane each; longer ones one wavefront each (wave_seq_fold).  The
+// term of position k is op.add applied to 0.0, which is the term itself
+// (a -0.0 term becomes +0.0, which no fold from +0.0 can tell apart).
+template <class Op>
+__global__ void __launch_bounds__(256) k_rows_seq(int n, const int32_t *__restrict__ rp, Op op) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n || row_skip(op, row)) return;
+  const int a = rp[row], b = rp[row + 1];
+  if (b - a > SEQ_LANE) return;
+  double acc = 0.0;
+#pragma unroll 8
+  for (int k = a; k < b; k++) op.add(acc, row, k);
+  op.finish(row, acc);
+}
+
+template <class Op>
+__global__ void __launch_bounds__(256) k_rows_seq_long(int64_t nrows, const int32_t *__restrict__ lrow,
+                                                       const int32_t *__restrict__ rp, Op op) {
+  __shared__ double buf[256];
+  const int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (j >= nrows) return;
+  const int row = lrow[j];
+  if (row_skip(op, row)) return;  // the whole wavefront: one row per wavefront
+  const double acc = wave_seq_fold(rp[row], rp[row + 1], buf + (threadIdx.x / WAVE) * WAVE, [&](int64_t k) {
+    double t = 0.0;
+    op.add(t, row, (int)k);
+    return t;
+  });
+  if ((threadIdx.x & (WAVE - 1)) == 0) op.finish(row, acc);
+}
+
+template <class Op>
+void run_rows_seq(int64_t n, const int32_t *rp, const Op &op, const HugeRows &Hr, hipStream_t s) {
+  static_assert(std::is_same<typename Op::Acc, double>::value, "sequential rows are double sums");
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_rows_seq<Op>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int)n, rp, op);
+  LG_LAUNCH_CHECK();
+  if (Hr.nlong) {
+    hipLaunchKernelGGL(k_rows_seq_long<Op>, dim3((unsigned)((Hr.nlong + 3) / 4)), dim3(256), 0, s, Hr.nlong,
+                       Hr.lrow.p, rp, op);
     LG_LAUNCH_CHECK();
   }
 }
@@ -743,9 +808,11 @@ struct OpScale : NoAcc {
   __device__ void finish(int, int) const {}
 };
 
-// ---- a10: col_sum = Reduce(Column) in ascending row order
-// row sums over every stored entry (the weighted degree of an adjacency
-// matrix, main.cpp:82-84), in the canonical order
+// ---- a10: col_sum = Reduce(Column, plus, 0.0): each column folded in
+// ascending row order (aggregation.hpp:863-864), read through the
+// transposed-entry positions.  Row sums (Reduce(Row), the weighted degree
+// of an adjacency matrix, main.cpp:86) fold each row in ascending column
+// order.  Both run through run_rows_seq.
 struct OpRowSum : DblSum {
   const double *v;
   double *out;
@@ -907,7 +974,8 @@ __global__ void k_reject_large(int n, const int32_t *sizes, int64_t max_size, in
 
 // ---- ext_int_ratio (aggregation.hpp:563-620): weight of an aggregated
 // vertex's off-diagonal edges leaving (ext) / staying in (!ext) its
-// aggregate, canonical row sums with the other entries +0
+// aggregate, sequential row sums (the reference's loop runs over each row in
+// ascending column order) with the other entries +0
 struct OpEdgeSplit : DblSum {
   const int32_t *ci, *state, *ind;
   const double *v;
@@ -989,10 +1057,10 @@ unsigned g1(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256);
 
 }  // namespace
 
-void row_sums_canonical(int64_t n, const int32_t *rp, const double *v, double *out, hipStream_t s) {
+void row_sums_seq(int64_t n, const int32_t *rp, const double *v, double *out, hipStream_t s) {
   HugeRows Hr;
   find_huge(n, rp, Hr, s);
-  run_rows(n, rp, OpRowSum{{}, v, out}, Hr, s);
+  run_rows_seq(n, rp, OpRowSum{{}, v, out}, Hr, s);
 }
 
 // ----------------------------------------------------------------------------
@@ -1286,13 +1354,13 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
   const int sc = o.aff_scaling;
   if (sc == LIGMG_AFF_ROW || sc == LIGMG_AFF_ROWCOLUMN || sc == LIGMG_AFF_SYM) {
     DevBuf<double> rs(n);
-    run_rows(n, A.rp.p, OpRowSum{{}, raw.p, rs.p}, Hr, s);
+    run_rows_seq(n, A.rp.p, OpRowSum{{}, raw.p, rs.p}, Hr, s);
     hipLaunchKernelGGL(k_div_rows, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, rs.p, raw.p);
     LG_LAUNCH_CHECK();
   }
   if (sc == LIGMG_AFF_COLUMN || sc == LIGMG_AFF_ROWCOLUMN) {
     DevBuf<double> csum(n);
-    run_rows(n, A.rp.p, OpColSum{{}, tpos.p, raw.p, csum.p}, Hr, s);
+    run_rows_seq(n, A.rp.p, OpColSum{{}, tpos.p, raw.p, csum.p}, Hr, s);
     hipLaunchKernelGGL(k_div_cols, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, A.ci.p, csum.p, raw.p);
     LG_LAUNCH_CHECK();
   }
@@ -1307,7 +1375,7 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
   run_rows(n, A.rp.p, OpScale{{}, A.ci.p, raw.p, rowmax.p, colmax.p, o.thresh, o.aff_scaling, S.p}, Hr,
            s);
   cs.alloc(n);
-  run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S.p, cs.p}, Hr, s);
+  run_rows_seq(n, A.rp.p, OpColSum{{}, tpos.p, S.p, cs.p}, Hr, s);
 }
 
 // ----------------------------------------------------------------------------
@@ -1341,8 +1409,10 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
       hipLaunchKernelGGL(k_voter_pairs, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, m, idx.p, vind.p,
                          vw.p, keys.p, vals.p);
       LG_LAUNCH_CHECK();
+      // stable sort: each target's voters stay in ascending voter index,
+      // the order in which tally_votes adds them (aggregation.hpp:539-547)
       radix_sort_pairs(keys.p, vals.p, m, kb, s);
-      int64_t nu = segmented_canon_sum(keys.p, vals.p, m, uk.p, us.p, s);
+      int64_t nu = segmented_seq_sum(keys.p, vals.p, m, uk.p, us.p, s);
       hipLaunchKernelGGL(k_scatter_sums, dim3(grid_for(nu, 256, 8192)), dim3(256), 0, s, nu, uk.p, us.p,
                          nv.p);
       LG_LAUNCH_CHECK();
@@ -1367,8 +1437,8 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
     if (iter >= o.agg_iters / 2 && o.rejection_ratio >= 0) {
       LG_CHECK(A.v.p, LIGMG_ERR_UNSUPPORTED, "conductance rejection needs the level matrix values");
       DevBuf<double> ext(n), in(n);
-      run_rows(n, A.rp.p, OpEdgeSplit{{}, A.ci.p, state.p, ind.p, A.v.p, 1, ext.p}, Hr, s);
-      run_rows(n, A.rp.p, OpEdgeSplit{{}, A.ci.p, state.p, ind.p, A.v.p, 0, in.p}, Hr, s);
+      run_rows_seq(n, A.rp.p, OpEdgeSplit{{}, A.ci.p, state.p, ind.p, A.v.p, 1, ext.p}, Hr, s);
+      run_rows_seq(n, A.rp.p, OpEdgeSplit{{}, A.ci.p, state.p, ind.p, A.v.p, 0, in.p}, Hr, s);
       hipLaunchKernelGGL(k_conductance, dim3(g1(n)), dim3(256), 0, s, (int)n, ext.p, in.p, o.rejection_ratio,
                          o.rejection_to_seed ? 1 : 0, state.p, ind.p);
       LG_LAUNCH_CHECK();
@@ -1393,7 +1463,7 @@ void col_sums_of(const DCsr &A, const double *S, DevBuf<double> &cs, hipStream_t
   DevBuf<int32_t> tpos(A.nnz);
   LG_CHECK(transpose_positions(A, Hr, tpos.p, s), LIGMG_ERR_INVALID, "matrix pattern is not structurally symmetric");
   cs.alloc(n);
-  run_rows(n, A.rp.p, OpColSum{{}, tpos.p, S, cs.p}, Hr, s);
+  run_rows_seq(n, A.rp.p, OpColSum{{}, tpos.p, S, cs.p}, Hr, s);
 }
 
 // ----------------------------------------------------------------------------

@@ -9,7 +9,7 @@ void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, 
                      int64_t ncols, DCsr &out, hipStream_t s);
 
 // out[i] = sum of row i's stored values in the canonical order.
-void row_sums_canonical(int64_t n, const int32_t *rp, const double *v, double *out, hipStream_t s);
+void row_sums_seq(int64_t n, const int32_t *rp, const double *v, double *out, hipStream_t s);
 // Throws LIGMG_ERR_INVALID unless rp[0] = 0, rp is non-decreasing, every
 // column is in [0, ncols) and, with sorted, every row's columns strictly
 // increase.  Reads columns only after the row pointers check out.

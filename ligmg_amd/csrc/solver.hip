@@ -1804,7 +1804,7 @@ struct Solver {
   int solve_batch(int k, const double *B, double *X, int mem, ligmg_info *info, int32_t *iters_out);
 
   // ---------------------------------------------------------------- solve
-  int solve(const double *b, double *x, int mem, ligmg_info *info, double *hist, int cap);
+  int solve(const double *b, double *x, int mem, ligmg_info *info, double *hist, int cap, int from = 0);
 };
 
 template <>
@@ -1962,16 +1962,27 @@ void Solver::scale_vec(T *p, int64_t n, double a) {
   LG_LAUNCH_CHECK();
 }
 
-int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double *hist, int cap) {
-  const DLevel &L0 = H[0];
+// solve (multigrid.hpp:315-370) of the system of level `from`: b and x have
+// H[from].n entries, in the caller's order for level 0 and in the level's
+// setup order (the order of its introspection) below it.  An elimination
+// level `from` is applied once (R b, outer Krylov on from + 1, x = P x + Q b);
+// the full residual is taken on H[from].A.
+int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double *hist, int cap, int from) {
+  LG_CHECK(from >= 0 && from < (int)H.size(), LIGMG_ERR_INVALID,
+           "ligmg_solve_from: level " + std::to_string(from) + " out of range (" + std::to_string(H.size()) +
+               " levels)");
+  const DLevel &L0 = H[from];
   const int64_t n = L0.A.n;
-  if (perm.p) {
+  // caller order -> solve layout: the input permutation and level 0's
+  // layout for level 0, the level's own layout below it
+  const int32_t *ip = from == 0 ? perm.p : L0.sperm.p;
+  if (ip) {
     const double *src = b;
     if (mem == LIGMG_MEM_HOST) {
       to_device(tmp0.p, b, n, s);
       src = tmp0.p;
     }
-    gather_vec(n, perm.p, src, b0.p, s);
+    gather_vec(n, ip, src, b0.p, s);
   } else if (mem == LIGMG_MEM_HOST) {
     to_device(b0.p, b, n, s);
   } else {
@@ -1980,10 +1991,10 @@ int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double 
   LG_HIP(hipStreamSynchronize(s));
   auto t0 = std::chrono::steady_clock::now();
   const bool do_once = o.top_elim && L0.kind == LIGMG_LEVEL_ELIM;
-  const size_t base = do_once ? 1 : 0;
+  const size_t base = do_once ? (size_t)from + 1 : (size_t)from;
   const DLevel &B = H[base];
   const int64_t nb_ = B.A.n;
-  if (outer.x.n < (size_t)nb_) outer.alloc(nb_, red[base].nb);
+  if (outer.x.n < (size_t)nb_ || outer.sc.n < (size_t)red[base].nb * SC_N) outer.alloc(nb_, red[base].nb);
   rhs_base.ensure(std::max<int64_t>(nb_, 1));
   double *rhs = rhs_base.p;
   if (do_once) op_spmv<double>(L0.R, b0.p, rhs, s);
@@ -2002,7 +2013,7 @@ int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double 
   // full-system residual (multigrid.hpp:357-366)
   op_residual<double>(L0.A, sol.p, b0.p, tmp0.p, s);
   const double *A_[2] = {tmp0.p, b0.p}, *B_[2] = {tmp0.p, b0.p};
-  RedPlan &R0 = red[0];
+  RedPlan &R0 = red[from];
   R0.dots<double>(2, A_, B_, dsc.p, s);
   std::vector<double> d = download(dsc.p, 2 * R0.nb, s);
   double rr = 0, bb = 0;
@@ -2010,12 +2021,12 @@ int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double 
     rr += d[c * 2];
     bb += d[c * 2 + 1];
   }
-  if (perm.p) {
+  if (ip) {
     if (mem == LIGMG_MEM_HOST) {
-      scatter_vec(n, perm.p, sol.p, tmp0.p, s);
+      scatter_vec(n, ip, sol.p, tmp0.p, s);
       to_host(x, tmp0.p, n, s);
     } else {
-      scatter_vec(n, perm.p, sol.p, x, s);
+      scatter_vec(n, ip, sol.p, x, s);
     }
   } else if (mem == LIGMG_MEM_HOST) {
     to_host(x, sol.p, n, s);
@@ -2024,7 +2035,9 @@ int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double 
   }
   LG_HIP(hipStreamSynchronize(s));
   if (info) {
-    // cycle_complexity (multigrid.hpp:372-387), ConvergenceInfo (util.hpp:143-170)
+    // cycle_complexity (multigrid.hpp:372-387): the work of the levels from
+    // the base level down, over the nnz of level 0 whatever `from` is;
+    // ConvergenceInfo (util.hpp:143-170)
     double sum = 0, mult = 1;
     for (size_t l = base; l < H.size(); l++) {
       const DLevel &L = H[l];
@@ -2230,6 +2243,18 @@ int ligmg_solve(ligmg_handle h, const double *b, double *x, int mem_kind, ligmg_
   if (!h || !b || !x) return LIGMG_ERR_INVALID;
   try {
     return h->S.solve(b, x, mem_kind, info, history, cap);
+  } catch (const lg::Error &e) {
+    return fail(e);
+  } catch (const std::exception &e) {
+    return fail_unknown(e.what());
+  }
+}
+
+int ligmg_solve_from(ligmg_handle h, int32_t level, const double *b, double *x, int mem_kind, ligmg_info *info,
+                     double *history, int32_t cap) {
+  if (!h || !b || !x) return LIGMG_ERR_INVALID;
+  try {
+    return h->S.solve(b, x, mem_kind, info, history, cap, level);
   } catch (const lg::Error &e) {
     return fail(e);
   } catch (const std::exception &e) {

@@ -204,30 +204,36 @@ class Solver:
         self.close()
 
     # --------------------------------------------------------------- solve
-    def solve(self, b, history_cap=1024):
+    def solve(self, b, history_cap=1024, level=0):
         """Solve L x = b to opts.tol relative residual.  ``b`` is a numpy
-        array (host) or a cuda torch tensor (device); the solution has the
-        same kind."""
+        array (host) or a torch tensor (a cuda one stays on the device); the
+        solution has the same kind and shape.  ``level`` > 0 solves the
+        system of that hierarchy level (Solver::solve(b, from),
+        ligmg.hpp:36-38): b then has the level's size and is numbered as
+        level_matrix(level) reports it."""
         info = _ffi.ligmg_info()
         hist = np.zeros(history_cap, dtype=np.float64)
+        n = self.n if level == 0 else self.level_info(level)[1]
         if _is_torch(b):
             import torch
+            if tuple(b.shape) not in ((n,), (n, 1)):
+                raise ValueError(f"b must have shape ({n},) or ({n}, 1), not {tuple(b.shape)}")
             bb = b.to(torch.float64).contiguous()
             x = torch.empty_like(bb)
             mem = _ffi.MEM_DEVICE if bb.is_cuda else _ffi.MEM_HOST
             if bb.is_cuda:
                 torch.cuda.current_stream().synchronize()
-            rc = self._lib.ligmg_solve(self._h, bb.data_ptr(), x.data_ptr(), mem, C.byref(info),
-                                       hist.ctypes.data_as(C.POINTER(C.c_double)), history_cap)
+            rc = self._lib.ligmg_solve_from(self._h, level, bb.data_ptr(), x.data_ptr(), mem, C.byref(info),
+                                            hist.ctypes.data_as(C.POINTER(C.c_double)), history_cap)
         else:
             bb = np.ascontiguousarray(b, dtype=np.float64)
-            if bb.shape != (self.n,):
-                raise ValueError(f"b must have shape ({self.n},)")
+            if bb.shape != (n,):
+                raise ValueError(f"b must have shape ({n},)")
             x = np.empty_like(bb)
-            rc = self._lib.ligmg_solve(self._h, bb.ctypes.data, x.ctypes.data, _ffi.MEM_HOST,
-                                       C.byref(info), hist.ctypes.data_as(C.POINTER(C.c_double)),
-                                       history_cap)
-        _check(rc, "ligmg_solve")
+            rc = self._lib.ligmg_solve_from(self._h, level, bb.ctypes.data, x.ctypes.data, _ffi.MEM_HOST,
+                                            C.byref(info), hist.ctypes.data_as(C.POINTER(C.c_double)),
+                                            history_cap)
+        _check(rc, "ligmg_solve_from")
         return ConvergenceInfo(hist[:info.history_len].copy(), info, x)
 
     def solve_batch(self, B):
@@ -237,6 +243,8 @@ class Solver:
         info = _ffi.ligmg_info()
         if _is_torch(B):
             import torch
+            if B.dim() != 2 or B.shape[0] != self.n or not 1 <= B.shape[1] <= 64:
+                raise ValueError(f"B must have shape ({self.n}, k) with 1 <= k <= 64, not {tuple(B.shape)}")
             k = B.shape[1]
             Bc = B.to(torch.float64).t().contiguous()  # column-major storage
             Xc = torch.empty_like(Bc)
@@ -249,8 +257,8 @@ class Solver:
             X = Xc.t()
         else:
             Bc = np.asfortranarray(B, dtype=np.float64)
-            if Bc.ndim != 2 or Bc.shape[0] != self.n:
-                raise ValueError(f"B must have shape ({self.n}, k)")
+            if Bc.ndim != 2 or Bc.shape[0] != self.n or not 1 <= Bc.shape[1] <= 64:
+                raise ValueError(f"B must have shape ({self.n}, k) with 1 <= k <= 64")
             k = Bc.shape[1]
             X = np.empty_like(Bc, order="F")
             its = np.zeros(k, dtype=np.int32)

@@ -6,7 +6,12 @@
 // Deliberate, documented deviations from the reference (all are places where
 // the reference's result depends on an unpinned third-party order or on
 // rand() seeded with the clock):
-//  * every order-dependent floating-point sum uses canon_sum() below;
+//  * the sums that CombBLAS Reduce or LigMG's own loops compute on one rank
+//    (col_sum, the vote tally, the Row/Column scaling sums, weighted
+//    degrees) are sequential left folds from 0.0 in ascending index, as the
+//    reference computes them (seq_sum below); every other order-dependent
+//    floating-point sum (SpMV-shaped sums whose order CombBLAS and PETSc
+//    leave unpinned) uses canon_sum();
 //  * test vectors come from a counter-based RNG (ligmg_rng::tv_bits) with an
 //    exact integer mean instead of rand()/RAND_MAX (combblas_extra.hpp:16-22);
 //  * P entries of an elimination level use row e of L instead of column e
@@ -64,6 +69,36 @@ double canon_sum(const double *a, int64_t m) {
   double s = 0.0;
   for (int64_t c = 0; c < m; c += 4096) s += canon_sum_chunk(a + c, std::min<int64_t>(4096, m - c));
   return s;
+}
+
+// The reference's one-rank order for the two sums that decide aggregation
+// and for the other CombBLAS reductions: a left fold from 0.0 over the terms
+// in ascending index.  tally_votes adds each voter's weight into
+// new_votes[ind] for i = 0, 1, ... (aggregation.hpp:539-547); the mpi
+// allreduce that follows is the identity on one rank (549-550).
+// SpParMat::Reduce(Column | Row, plus, 0.0) folds each column (row) from the
+// identity in ascending row (column) order (aggregation.hpp:353-359,
+// 863-864; main.cpp:86).
+//
+// decision_order is a test hook that plants the wrong order on purpose
+// (1: the canonical split of canon_sum; 2: the tally in descending voter
+// order), so tests can show that a hand-built case tells the orders apart.
+int decision_order = 0;
+
+double seq_sum(const double *a, int64_t m) {
+  if (decision_order == 1) return canon_sum(a, m);
+  double s = 0.0;
+  for (int64_t i = 0; i < m; i++) s += a[i];
+  return s;
+}
+
+double tally_sum(const double *a, int64_t m) {
+  if (decision_order == 2) {
+    double s = 0.0;
+    for (int64_t i = m - 1; i >= 0; i--) s += a[i];
+    return s;
+  }
+  return seq_sum(a, m);
 }
 
 struct Csr {
@@ -379,8 +414,10 @@ int32_t transpose_pos(const Csr &A, int64_t r, int32_t c) {
   return (int32_t)(p - A.ci.data());
 }
 
-// column sums of values on A's pattern in ascending row order: column c's
-// entries are V(r, c) for r in row c's columns
+// column sums of values on A's pattern, Reduce(Column, plus, 0.0): column
+// c's entries V(r, c) for r in row c's columns, folded in ascending r.  A
+// pruned position holds +0.0, which leaves a sequential sum unchanged, so
+// summing over the stored pattern equals summing over the pruned matrix.
 void column_sums(const Csr &A, const std::vector<double> &V, std::vector<double> &out) {
   out.assign(A.n, 0.0);
   std::vector<double> buf;
@@ -388,7 +425,7 @@ void column_sums(const Csr &A, const std::vector<double> &V, std::vector<double>
     const int32_t len = A.rp[c + 1] - A.rp[c];
     buf.assign(len, 0.0);
     for (int32_t k = A.rp[c]; k < A.rp[c + 1]; k++) buf[k - A.rp[c]] = V[transpose_pos(A, c, A.ci[k])];
-    out[c] = canon_sum(buf.data(), len);
+    out[c] = seq_sum(buf.data(), len);
   }
 }
 
@@ -413,7 +450,7 @@ void strength(const Csr &A, double *X, int T, int sweeps, int metric, int scalin
   // leaves its (zero) entries at 0.
   if (scaling == LIGMG_AFF_ROW || scaling == LIGMG_AFF_ROWCOLUMN || scaling == LIGMG_AFF_SYM) {
     for (int64_t r = 0; r < A.n; r++) {
-      const double rs = canon_sum(raw.data() + A.rp[r], A.rp[r + 1] - A.rp[r]);
+      const double rs = seq_sum(raw.data() + A.rp[r], A.rp[r + 1] - A.rp[r]);
       for (int32_t k = A.rp[r]; k < A.rp[r + 1]; k++) raw[k] = rs > 0 ? raw[k] / rs : 0.0;
     }
   }
@@ -496,7 +533,9 @@ void reject_aggregates(std::vector<int32_t> &state, std::vector<int32_t> &ind, i
 
 // ext_int_ratio + conductance_rejection (aggregation.hpp:563-620, 752-776):
 // per aggregated vertex, the weight of its off-diagonal edges leaving its
-// aggregate over the weight staying inside (canonical sums); Decided and
+// aggregate over the weight staying inside, each folded from 0 in ascending
+// column order as the reference's dcsc_global_local_iter loop does
+// (aggregation.hpp:583-606; "sum -= val" equals "sum += -val"); Decided and
 // SeedNeighbor vertices above rejection_ratio become {i, Undecided} (or
 // {i, Seed} with rejection_to_seed).
 void conductance_rejection(const Csr &A, std::vector<int32_t> &state, std::vector<int32_t> &ind,
@@ -514,7 +553,7 @@ void conductance_rejection(const Csr &A, std::vector<int32_t> &state, std::vecto
         if (aggregated(state[j]) && ind[j] == ind[i]) in[k - A.rp[i]] = -A.v[k];
         else ext[k - A.rp[i]] = -A.v[k];
       }
-    ratio[i] = canon_sum(ext.data(), len) / canon_sum(in.data(), len);
+    ratio[i] = seq_sum(ext.data(), len) / seq_sum(in.data(), len);
   }
   for (int64_t i = 0; i < n; i++)
     if (ratio[i] > o.rejection_ratio && (state[i] == ST_DECIDED || state[i] == ST_SEEDNEIGHBOR)) {
@@ -555,8 +594,9 @@ int64_t aggregate(const Csr &A, const std::vector<double> &S, const std::vector<
         vf[i] = va[i].ind;
       }
     }
+    // lists[t] holds t's voters' weights in ascending voter index
     for (int64_t t = 0; t < n; t++) {
-      double nv = canon_sum(lists[t].data(), (int64_t)lists[t].size());
+      double nv = tally_sum(lists[t].data(), (int64_t)lists[t].size());
       votes[t] = nv + votes[t];
     }
     std::vector<int32_t> ns = state, ni = ind;
@@ -1238,14 +1278,17 @@ double cycle_complexity(const Solver &S, size_t start) {
 }
 
 // ---- a22/a23: ligmg::solve (multigrid.hpp:315-370) -------------------------
+// The system of level `from` (Solver::solve(b, from), ligmg.hpp:36-38): an
+// elimination level is applied once around the outer Krylov solve on
+// from + 1; the full residual is taken on H[from].
 int solve(const Solver &S, const double *bin, double *xout, ligmg_info *info, double *hist,
-          int cap) {
+          int cap, size_t from = 0) {
   auto t0 = std::chrono::steady_clock::now();
-  const Level &L0 = S.H[0];
+  const Level &L0 = S.H[from];
   const int64_t n = L0.A.n;
   std::vector<double> b(bin, bin + n);
   bool do_once = S.opts.top_elim && L0.kind == LIGMG_LEVEL_ELIM;
-  size_t base = do_once ? 1 : 0;
+  size_t base = do_once ? from + 1 : from;
   std::vector<double> rhs;
   if (do_once) {
     rhs.resize(L0.n_coarse);
@@ -1316,6 +1359,8 @@ struct oracle_solver : Solver {};
 extern "C" {
 
 double oracle_canon_sum(const double *a, int64_t m) { return canon_sum(a, m); }
+double oracle_seq_sum(const double *a, int64_t m) { return seq_sum(a, m); }
+void oracle_set_decision_order(int32_t mode) { decision_order = mode; }
 
 void oracle_default_options(ligmg_options *o) {
   memset(o, 0, sizeof(*o));
@@ -1407,6 +1452,17 @@ int oracle_solve(oracle_handle h, const double *b, double *x, ligmg_info *info, 
     int rc = solve(*h, bp.data(), xp.data(), info, hist, cap);
     for (int64_t i = 0; i < n; i++) x[h->perm[i]] = xp[i];
     return rc;
+  } catch (...) {
+    return LIGMG_ERR_NUMERIC;
+  }
+}
+
+int oracle_solve_from(oracle_handle h, int32_t level, const double *b, double *x, ligmg_info *info,
+                      double *hist, int32_t cap) {
+  if (level < 0 || level >= (int)h->H.size()) return LIGMG_ERR_INVALID;
+  if (level == 0) return oracle_solve(h, b, x, info, hist, cap);
+  try {
+    return solve(*h, b, x, info, hist, cap, (size_t)level);
   } catch (...) {
     return LIGMG_ERR_NUMERIC;
   }
@@ -1538,7 +1594,7 @@ int oracle_aggregate_with_strength(int64_t n, const int32_t *rp, const int32_t *
         if (p == e || *p != c) return LIGMG_ERR_INVALID;
         buf[k - A.rp[c]] = s[p - A.ci.data()];
       }
-      cs[c] = canon_sum(buf.data(), len);
+      cs[c] = seq_sum(buf.data(), len);
     }
     std::vector<int32_t> a, i2, st;
     *n_coarse = aggregate(A, s, cs, *opts, a, i2, st);
@@ -1646,7 +1702,8 @@ int oracle_extract(int64_t n, const int32_t *rp, const int32_t *ci, const double
   return 0;
 }
 
-// L = D - A: D the row sums of A (every stored entry, canonical order), the
+// L = D - A: D the row sums of A (Reduce(Row): every stored entry in
+// ascending column order from 0.0), the
 // union of A's pattern and the diagonal; EWiseApply gives d - a on the
 // diagonal and 0 - a elsewhere (main.cpp:82-104).
 int oracle_adjacency_to_laplacian(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
@@ -1654,7 +1711,7 @@ int oracle_adjacency_to_laplacian(int64_t n, const int32_t *rp, const int32_t *c
   orp[0] = 0;
   int64_t o = 0;
   for (int64_t i = 0; i < n; i++) {
-    const double d = canon_sum(v + rp[i], rp[i + 1] - rp[i]);
+    const double d = seq_sum(v + rp[i], rp[i + 1] - rp[i]);
     bool done = false;
     for (int32_t k = rp[i]; k < rp[i + 1]; k++) {
       if (!done && ci[k] > i) {

@@ -5,7 +5,10 @@
  * and bench.py's cpu_baseline leg may load it.  It restates the reference's
  * algorithm (include/ligmg/{elimination,aggregation,smoothing,multigrid,cg,
  * util}.hpp) in plain single-threaded C++ with the summation orders fixed
- * (see canon_sum in oracle.cpp), so that the device implementation can be
+ * (seq_sum: the reference's own one-rank order for col_sum, the vote tally
+ * and the other CombBLAS Reduce sums; canon_sum: a fixed order for the sums
+ * the reference leaves to CombBLAS/PETSc internals), so that the device
+ * implementation can be
  * compared with it bit for bit on elimination sets, aggregates and coarse
  * matrices.
  *
@@ -29,6 +32,12 @@ extern "C" {
 typedef struct oracle_solver *oracle_handle;
 
 double oracle_canon_sum(const double *a, int64_t m);
+/* left fold from 0.0 in index order: the reference's one-rank order for
+ * col_sum, the vote tally and the other CombBLAS Reduce sums */
+double oracle_seq_sum(const double *a, int64_t m);
+/* test hook: 0 = the reference's order (default); 1 = plant the canonical
+ * split in the sequential sums; 2 = plant a descending-voter tally */
+void oracle_set_decision_order(int32_t mode);
 void oracle_default_options(ligmg_options *opts);
 
 int oracle_create(int64_t n, const int32_t *rowptr, const int32_t *colind,
@@ -38,6 +47,10 @@ int oracle_destroy(oracle_handle h);
 int oracle_ordering(oracle_handle h, int32_t *perm);
 int oracle_solve(oracle_handle h, const double *b, double *x, ligmg_info *info,
                  double *history, int32_t history_cap);
+/* the system of level `level` (b, x in that level's numbering) */
+int oracle_solve_from(oracle_handle h, int32_t level, const double *b,
+                      double *x, ligmg_info *info, double *history,
+                      int32_t history_cap);
 int oracle_num_levels(oracle_handle h, int32_t *out);
 int oracle_level_info(oracle_handle h, int32_t level, int32_t *kind,
                       int64_t *n, int64_t *nnz, int64_t *n_coarse);

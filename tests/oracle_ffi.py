@@ -16,6 +16,8 @@ i32, i64, u64, f64, P = C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.POINTER
 H = C.c_void_p
 PROTOS = {
     "oracle_canon_sum": (f64, [P(f64), i64]),
+    "oracle_seq_sum": (f64, [P(f64), i64]),
+    "oracle_set_decision_order": (None, [i32]),
     "oracle_default_options": (None, [P(_ffi.ligmg_options)]),
     "oracle_create": (C.c_int, [i64, P(i32), P(i32), P(f64), P(_ffi.ligmg_options), i32, P(i64),
                                 P(H)]),
@@ -28,6 +30,7 @@ PROTOS = {
                                                 C.c_void_p, C.c_void_p]),
     "oracle_ordering": (C.c_int, [H, C.c_void_p]),
     "oracle_solve": (C.c_int, [H, P(f64), P(f64), P(_ffi.ligmg_info), P(f64), i32]),
+    "oracle_solve_from": (C.c_int, [H, i32, P(f64), P(f64), P(_ffi.ligmg_info), P(f64), i32]),
     "oracle_num_levels": (C.c_int, [H, P(i32)]),
     "oracle_level_info": (C.c_int, [H, i32, P(i32), P(i64), P(i64), P(i64)]),
     "oracle_level_matrix": (C.c_int, [H, i32, P(i32), P(i32), P(f64)]),
@@ -167,13 +170,13 @@ class OracleSolver:
         assert lib().oracle_apply_cycle(self.h, l, ptr(r, f64), ptr(z, f64)) == 0
         return z
 
-    def solve(self, b, cap=1024):
+    def solve(self, b, cap=1024, level=0):
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.empty_like(b)
         info = _ffi.ligmg_info()
         hist = np.empty(cap, np.float64)
-        rc = lib().oracle_solve(self.h, ptr(b, f64), ptr(x, f64), C.byref(info), ptr(hist, f64),
-                                cap)
+        rc = lib().oracle_solve_from(self.h, level, ptr(b, f64), ptr(x, f64), C.byref(info),
+                                     ptr(hist, f64), cap)
         if rc != 0:
             raise RuntimeError(f"oracle_solve failed ({rc})")
         return x, info, hist[:info.history_len].copy()

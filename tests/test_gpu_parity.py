@@ -174,6 +174,29 @@ def test_voting_with_injected_strength(seed):
     assert np.array_equal(da[0], agg) and np.array_equal(da[1], ind) and np.array_equal(da[2], st)
 
 
+@pytest.mark.parametrize("m", [100, 300, 5000])
+@pytest.mark.parametrize("iters", [1, 2, 10])
+def test_knife_edge_votes_equal_col_sum(m, iters):
+    """votes[c] == col_sum[c] exactly (tests/decision_cases.py): the device
+    seeds c as the reference's order does, and the whole voting result is
+    the oracle's.  m = 100 takes the one-lane sums, 300 the wavefront sums,
+    5000 a row longer than 4096 entries."""
+    import decision_cases as D
+    g, S, c, seed = D.flipping_case(m)
+    print(f"m={m} iters={iters} seed={seed} centre={c}", flush=True)
+    da = ligmg_amd.aggregate_with_strength(g, S, ligmg_amd.Options(agg_iters=iters))
+    rp, ci, _ = O.csr_args(g)
+    agg, ind, st = (np.empty(g.n, np.int32) for _ in range(3))
+    nc = O.i64()
+    o = O.default_options(agg_iters=iters)
+    assert O.lib().oracle_aggregate_with_strength(
+        g.n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(S, O.f64), O.C.byref(o), O.ptr(agg, O.i32),
+        O.ptr(ind, O.i32), O.ptr(st, O.i32), O.C.byref(nc)) == 0
+    assert da[3] == nc.value
+    assert np.array_equal(da[0], agg) and np.array_equal(da[1], ind) and np.array_equal(da[2], st)
+    assert da[1][c] == c and da[2][c] == (0 if iters == 1 else 3)  # Seed, then DecidedRoot
+
+
 def direct_error(g, b, x):
     A = g.to_scipy().tocsc()
     xr = np.zeros(g.n)

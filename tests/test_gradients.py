@@ -114,3 +114,50 @@ def test_autograd_matches_dense_reference_and_fd():
             vals.append(((x2[tpi] - x2[tpj]) ** 2).sum().item())
         fd = (vals[0] - vals[1]) / (2 * h)
         assert fd == pytest.approx(rgw[e], rel=1e-3, abs=1e-6 * rl)
+
+
+@pytest.mark.gpu
+def test_autograd_pendant_graph_shift_dependent_loss():
+    """A graph with a top elimination level (pendant paths on R-MAT): the
+    solver's raw solution there is P x_c + Q b, not mean-zero.  The forward
+    must still be x = L^+ b and the gradients those of x for a loss that a
+    constant shift of x changes, sum(c * x) + sum(x^2)."""
+    import torch
+    import ligmg_amd
+    from ligmg_amd import _ffi
+    from ligmg_amd.autograd import EdgePattern, laplacian_solve
+    g = graphs.rmat_config(10, "lognormal", pendants=True)
+    ei, ej, w = g.upper_edges()
+    s = ligmg_amd.Solver(g)
+    assert s.level_info(0)[0] == _ffi.LEVEL_ELIM, "the case must have a top elimination level"
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(g.n)
+    c = rng.standard_normal(g.n)
+    dev = torch.device("cuda:0")
+    pat = EdgePattern(g.n, ei, ej, dev)
+    wt = torch.tensor(w, device=dev, dtype=torch.float64, requires_grad=True)
+    bt = torch.tensor(b, device=dev, requires_grad=True)
+    ct = torch.tensor(c, device=dev)
+    x = laplacian_solve(wt, bt, pat)
+    loss = (ct * x).sum() + (x * x).sum()
+    loss.backward()
+    # dense FP64 reference on the pattern's edge order
+    pei, pej = pat.ei.cpu().numpy(), pat.ej.cpu().numpy()
+    wmap = {(int(a), int(bq)): float(v) for a, bq, v in zip(ei, ej, w)}
+    wp = np.array([wmap[(int(a), int(bq))] for a, bq in zip(pei, pej)])
+    Lp = np.linalg.pinv(dense_L(g.n, pei, pej, wp))
+    rx = Lp @ (b - b.mean())
+    gx = c + 2 * rx
+    lam = Lp @ (gx - gx.mean())
+    rgw = -(lam[pei] - lam[pej]) * (rx[pei] - rx[pej])
+    xg = x.detach().cpu().numpy()
+    gw = wt.grad.detach().cpu().numpy()
+    gb = bt.grad.detach().cpu().numpy()
+    err_x = np.linalg.norm(xg - rx) / np.linalg.norm(rx)
+    err_w = np.linalg.norm(gw - rgw) / np.linalg.norm(rgw)
+    err_b = np.linalg.norm(gb - lam) / np.linalg.norm(lam)
+    print(f"n={g.n} x err {err_x:.2e} grad w err {err_w:.2e} grad b err {err_b:.2e}")
+    # a solve to 1e-8 relative residual; an unprojected solution would be
+    # off by its mean shift, an O(1) error in x, grad b and the c . x term
+    assert err_x <= 1e-6 and err_w <= 1e-5 and err_b <= 1e-6
+    assert abs(xg.mean()) <= 1e-12 * np.abs(xg).max()

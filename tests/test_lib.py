@@ -86,3 +86,26 @@ def test_rhs_mean_zero_per_block():
     B = graphs.rhs(400, 3, 0, comp_ptr=cp)
     for c in range(3):
         assert np.all(np.abs(B[cp[c]:cp[c + 1]].sum(0)) < 1e-12)
+
+
+def test_solve_rejects_wrong_shapes_before_the_library():
+    """A right-hand side of the wrong length must raise ValueError in Python,
+    before ligmg_solve (which reads n values of b and writes n of x) is
+    called, for numpy arrays and torch tensors alike."""
+    import torch
+    from ligmg_amd.solver import Solver
+
+    class NoLib:
+        def __getattr__(self, name):
+            raise AssertionError(f"library called: {name}")
+
+    s = Solver.__new__(Solver)
+    s._lib, s._h, s.n = NoLib(), None, 10
+    for b in (np.zeros(5), torch.zeros(5, dtype=torch.float64), torch.zeros(11),
+              torch.zeros(10, 2), torch.zeros(2, 5)):
+        with pytest.raises(ValueError):
+            s.solve(b)
+    for B in (np.zeros((5, 2)), torch.zeros(5, 2), torch.zeros(10), torch.zeros(10, 65),
+              torch.zeros(10, 0)):
+        with pytest.raises(ValueError):
+            s.solve_batch(B)
