@@ -141,6 +141,12 @@ typedef struct ligmg_options {
                                    weight exceeds it leave their aggregate  */
   int32_t rejection_to_seed;    /* false: a rejected vertex becomes undecided;
                                    true: a seed                             */
+  int32_t spmm_tune;            /* 1: batched solves time the SpMM tiles
+                                   (right-hand sides per lane, gathers in
+                                   flight) of every level operator at the
+                                   first solve of each batch width and keep
+                                   the fastest; 0: the fixed tile.  Results
+                                   are the same bits either way. (new)      */
 } ligmg_options;
 
 /* ConvergenceInfo (util.hpp:143-170) plus the fields the reference only
@@ -299,6 +305,21 @@ int ligmg_bench_spmv(ligmg_handle h, int32_t level, int32_t precision,
 int ligmg_bench_smoother(ligmg_handle h, int32_t level, int32_t precision,
                          int32_t iters, double *avg_ms, int64_t *bytes);
 
+/* (new) ligmg_bench_spmv for a block of k <= 64 right-hand sides (k rounded
+ * up to a power of two, kp; row-major n x kp blocks, the batched solves'
+ * SpMM with its tuned tile).  Algorithmic bytes of one launch:
+ * nnz*(s_v + 4) + (n+1)*4 + 2*n*kp*s_x. */
+int ligmg_bench_spmm(ligmg_handle h, int32_t level, int32_t precision,
+                     int32_t k, int32_t iters, double *avg_ms, int64_t *bytes);
+
+/* (new) Times `iters` launches of the coarsest level's dense pseudo-inverse
+ * apply to a block of k <= 64 right-hand sides (k rounded up to a power of
+ * two, kp; FP64, on the matrix cores when kp >= 16, a matrix-vector product
+ * when kp = 1) on the solver's stream.  Returns the average milliseconds
+ * per launch and its floating-point operations, 2 * n_c^2 * kp. */
+int ligmg_bench_coarse(ligmg_handle h, int32_t k, int32_t iters,
+                       double *avg_ms, double *flops);
+
 /* z = M r: one cycle of the preconditioner from level `level` down (the
  * V- or K-cycle of multigrid.hpp:86-130 with a zero initial guess) in the
  * given precision.  r and z are host arrays of length n in FP64.  Mirrors
@@ -311,6 +332,25 @@ int ligmg_apply_cycle(ligmg_handle h, int32_t level, int32_t precision,
  * LIGMG_PREC_MIXED).  x and y are host arrays of length n in FP64. */
 int ligmg_level_spmv(ligmg_handle h, int32_t level, int32_t precision,
                      const double *x, double *y);
+
+/* ---- the "ligmg" PETSc preconditioner without PETSc ---------------------
+ * PCCreate_LIGMG / PCSetUp_LIGMG / PCApply_LIGMG / PCDestroy_LIGMG
+ * (petsc_shim.hpp:18-74) as plain calls; the PETSc shim
+ * (csrc/petsc/ligmg_pc.cpp) is a thin layer over them.  Set-up: begin(n),
+ * then every row of the preconditioning matrix in order 0..n-1 (columns in
+ * any order, no duplicates; host arrays), then setup(), which builds the
+ * solver with default options as the reference's PCSetUp does (an earlier
+ * solver is destroyed).  apply() is one full solve to the default tolerance
+ * (PCApply, :40-62), b and x host arrays of length n; info may be NULL. */
+typedef struct ligmg_pc *ligmg_pc_handle;
+int ligmg_pc_create(ligmg_pc_handle *out);
+int ligmg_pc_begin(ligmg_pc_handle pc, int64_t n);
+int ligmg_pc_add_row(ligmg_pc_handle pc, int64_t row, int64_t ncols,
+                     const int64_t *cols, const double *vals);
+int ligmg_pc_setup(ligmg_pc_handle pc);
+int ligmg_pc_apply(ligmg_pc_handle pc, const double *b, double *x,
+                   ligmg_info *info);
+int ligmg_pc_destroy(ligmg_pc_handle pc);
 
 /* ---- graph preprocessing on the device (main.cpp:79-110, cc.hpp) ------ */
 /* All arrays live in mem_kind memory (LIGMG_MEM_HOST or LIGMG_MEM_DEVICE);

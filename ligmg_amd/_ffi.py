@@ -35,6 +35,7 @@ class ligmg_options(C.Structure):
         ("tail_nnz", i32),
         ("rejection_ratio", f64),
         ("rejection_to_seed", i32),
+        ("spmm_tune", i32),
     ]
 
 
@@ -109,6 +110,14 @@ SOLVER_PROTOS = {
     "ligmg_bench_spmv": (C.c_int, [HANDLE, i32, i32, i32, P(f64), P(i64)]),
     "ligmg_bench_smoother": (C.c_int, [HANDLE, i32, i32, i32, P(f64), P(i64)]),
     "ligmg_components": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, P(i64)]),
+    "ligmg_bench_coarse": (C.c_int, [HANDLE, i32, i32, P(f64), P(f64)]),
+    "ligmg_bench_spmm": (C.c_int, [HANDLE, i32, i32, i32, i32, P(f64), P(i64)]),
+    "ligmg_pc_create": (C.c_int, [P(C.c_void_p)]),
+    "ligmg_pc_begin": (C.c_int, [C.c_void_p, i64]),
+    "ligmg_pc_add_row": (C.c_int, [C.c_void_p, i64, i64, P(i64), P(f64)]),
+    "ligmg_pc_setup": (C.c_int, [C.c_void_p]),
+    "ligmg_pc_apply": (C.c_int, [C.c_void_p, P(f64), P(f64), P(ligmg_info)]),
+    "ligmg_pc_destroy": (C.c_int, [C.c_void_p]),
     "ligmg_largest_component": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, P(i64),
                                           P(i64)]),
     "ligmg_extract": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, i64, C.c_int,

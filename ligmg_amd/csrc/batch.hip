@@ -80,35 +80,75 @@ struct BDiag {  // FP32 Laplacian form: + d[i] x[i, c]
 };
 
 // One workgroup per unit (the SpMV units).  Complete-row units: group g of
-// kp lanes handles rows g, g + G, ...; long-row chunks: the groups split the
-// chunk's entries and reduce through LDS into kp partial sums.
-template <class T, class Val, class Epi>
+// kp / VW lanes handles rows g, g + G, ...; lane t of a group handles the
+// VW adjacent right-hand sides t*VW .. t*VW + VW - 1 with one wide load per
+// gathered row of X, and keeps U entries' gathers in flight.  Long-row
+// chunks: groups of kp lanes (one right-hand side each, whatever VW) split
+// the chunk's entries and reduce through LDS into kp partial sums.  Every
+// (row, right-hand side) sum of a complete row is one lane adding the row's
+// entries in stored order, so the tile (VW, U) does not change any bit.
+template <class T, int VW>
+struct BVec {
+  typedef T type __attribute__((ext_vector_type(VW)));
+};
+template <class T>
+struct BVec<T, 1> {
+  typedef T type;
+};
+template <class T, int VW>
+__device__ __forceinline__ double bvec_at(const typename BVec<T, VW>::type &x, int v) {
+  if constexpr (VW == 1) return (double)x;
+  else return (double)x[v];
+}
+
+template <class T, int VW, int U, class Val, class Epi>
 __global__ void __launch_bounds__(256)
     k_spmm_units(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, Val val,
                  const T *__restrict__ X, int kp, int lk, const int32_t *__restrict__ urow0,
                  const int32_t *__restrict__ unrows, const int32_t *__restrict__ ukoff,
                  double *__restrict__ parts, Epi epi) {
-  const int G = 256 >> lk;
-  const int g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
+  typedef typename BVec<T, VW>::type V;
   const int u = blockIdx.x;
   const int row0 = urow0[u], nr = unrows[u], k0 = ukoff[u];
   if (nr > 0) {
+    const int lg = lk - (VW == 1 ? 0 : VW == 2 ? 1 : VW == 4 ? 2 : 3);  // log2(lanes per row)
+    const int G = 256 >> lg;
+    const int g = threadIdx.x >> lg, c0 = (threadIdx.x & ((1 << lg) - 1)) * VW;
     for (int r = g; r < nr; r += G) {
       const int row = row0 + r;
       const int a = rp[row], b = rp[row + 1];
-      double s = 0;
+      double s[VW];
+#pragma unroll
+      for (int v = 0; v < VW; v++) s[v] = 0;
       int j = a;
-      for (; j + 1 < b; j += 2) {  // two independent gathers in flight
-        const int c0 = ci[j], c1 = ci[j + 1];
-        const double v0 = (double)val(j), v1 = (double)val(j + 1);
-        const double x0 = (double)X[(int64_t)c0 * kp + c], x1 = (double)X[(int64_t)c1 * kp + c];
-        s += v0 * x0;
-        s += v1 * x1;
+      for (; j + U - 1 < b; j += U) {  // U independent gathers in flight
+        int cc[U];
+        double vv[U];
+        V xx[U];
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+          cc[t] = ci[j + t];
+          vv[t] = (double)val(j + t);
+        }
+#pragma unroll
+        for (int t = 0; t < U; t++) xx[t] = *(const V *)(X + (int64_t)cc[t] * kp + c0);
+#pragma unroll
+        for (int t = 0; t < U; t++)
+#pragma unroll
+          for (int v = 0; v < VW; v++) s[v] += vv[t] * bvec_at<T, VW>(xx[t], v);
       }
-      if (j < b) s += (double)val(j) * (double)X[(int64_t)ci[j] * kp + c];
-      epi(row, c, s);
+      for (; j < b; j++) {
+        const double vj = (double)val(j);
+        const V xj = *(const V *)(X + (int64_t)ci[j] * kp + c0);
+#pragma unroll
+        for (int v = 0; v < VW; v++) s[v] += vj * bvec_at<T, VW>(xj, v);
+      }
+#pragma unroll
+      for (int v = 0; v < VW; v++) epi(row, c0 + v, s[v]);
     }
   } else {
+    const int G = 256 >> lk;
+    const int g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
     __shared__ double red[256];
     const int row = row0;
     const int cnt = min(k0 + UNIT_NNZ, rp[row + 1]) - k0;
@@ -156,15 +196,46 @@ const float *bvals<float>(const DCsr &A) {
   return A.v32.p;
 }
 
+// SpMM tiles (right-hand sides per lane VW, gathers in flight U) by id;
+// id 0 is the fixed tile.  VW never exceeds the block width kp.
+constexpr int SPMM_NCFG = 6;
+constexpr int SPMM_VW[SPMM_NCFG] = {1, 1, 2, 2, 4, 4};
+constexpr int SPMM_U[SPMM_NCFG] = {2, 4, 2, 4, 2, 4};
+
+template <class T>
+constexpr int prec_slot() {
+  return std::is_same<T, float>::value ? 1 : 0;
+}
+
+template <class T, int VW, int U, class Val, class Epi>
+void launch_spmm_tile(const DCsr &A, Val val, const T *X, int kp, Epi epi, double *parts, hipStream_t s) {
+  hipLaunchKernelGGL((k_spmm_units<T, VW, U, Val, Epi>), dim3((unsigned)A.U.nunits), dim3(256), 0, s, A.rp.p,
+                     A.ci.p, val, X, kp, log2i(kp), A.U.row0.p, A.U.nrows.p, A.U.koff.p, parts, epi);
+  LG_LAUNCH_CHECK();
+}
+
+template <class T, class Val, class Epi>
+void launch_spmm_cfg(int cfg, const DCsr &A, Val val, const T *X, int kp, Epi epi, DevBuf<double> &parts,
+                     hipStream_t s) {
+  switch (cfg) {
+    case 1: launch_spmm_tile<T, 1, 4>(A, val, X, kp, epi, parts.p, s); break;
+    case 2: launch_spmm_tile<T, 2, 2>(A, val, X, kp, epi, parts.p, s); break;
+    case 3: launch_spmm_tile<T, 2, 4>(A, val, X, kp, epi, parts.p, s); break;
+    case 4: launch_spmm_tile<T, 4, 2>(A, val, X, kp, epi, parts.p, s); break;
+    case 5: launch_spmm_tile<T, 4, 4>(A, val, X, kp, epi, parts.p, s); break;
+    default: launch_spmm_tile<T, 1, 2>(A, val, X, kp, epi, parts.p, s); break;
+  }
+}
+
 template <class T, class Val, class Epi>
 void launch_spmm(const DCsr &A, Val val, const T *X, int kp, Epi epi, DevBuf<double> &parts,
                  hipStream_t s) {
   if (A.U.nunits == 0) return;
   LG_CHECK(parts.n >= (size_t)std::max<int64_t>(A.U.nparts, 1) * kp, LIGMG_ERR_INVALID,
            "spmm partial-sum buffer too small");
-  hipLaunchKernelGGL((k_spmm_units<T, Val, Epi>), dim3((unsigned)A.U.nunits), dim3(256), 0, s, A.rp.p,
-                     A.ci.p, val, X, kp, log2i(kp), A.U.row0.p, A.U.nrows.p, A.U.koff.p, parts.p, epi);
-  LG_LAUNCH_CHECK();
+  int cfg = A.U.spmm_cfg[prec_slot<T>()][log2i(kp)];
+  if (cfg < 0 || SPMM_VW[cfg] > kp) cfg = 0;
+  launch_spmm_cfg<T>(cfg, A, val, X, kp, epi, parts, s);
   if (A.U.nlong) {
     hipLaunchKernelGGL((k_spmm_long<Epi>), dim3(grid_for(A.U.nlong * kp, 256)), dim3(256), 0, s,
                        A.U.lrow.p, A.U.lpart.p, A.U.nlong, kp, (const double *)parts.p, epi);
@@ -318,6 +389,46 @@ __global__ void __launch_bounds__(256) k_bsub_mean(T *a, int kp, int lk, const i
 }  // namespace
 
 // ---------------------------------------------------------------- host side
+// Times every tile that fits kp on A (the operator's own values and
+// Laplacian-form diagonal, X of zeros, a plain store) and records the
+// fastest for this precision and width; a tile already picked is kept.
+template <class T>
+int tune_spmm(DCsr &A, int kp, DevBuf<double> &parts, hipStream_t s) {
+  int8_t &pick = A.U.spmm_cfg[prec_slot<T>()][log2i(kp)];
+  if (pick >= 0 || A.U.nunits == 0) return pick < 0 ? 0 : pick;
+  DevBuf<T> X((size_t)std::max<int64_t>(A.ncols, 1) * kp), Y((size_t)std::max<int64_t>(A.n, 1) * kp);
+  fill_u8((uint8_t *)X.p, 0, (int64_t)X.bytes(), s);
+  hipEvent_t e0, e1;
+  LG_HIP(hipEventCreate(&e0));
+  LG_HIP(hipEventCreate(&e1));
+  float best = 0;
+  int besti = 0;
+  for (int c = 0; c < SPMM_NCFG; c++) {
+    if (SPMM_VW[c] > kp) continue;
+    A.U.spmm_cfg[prec_slot<T>()][log2i(kp)] = (int8_t)c;
+    op_bspmv<T>(A, X.p, Y.p, kp, parts, s);  // warm
+    LG_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < 3; r++) op_bspmv<T>(A, X.p, Y.p, kp, parts, s);
+    LG_HIP(hipEventRecord(e1, s));
+    LG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (c == 0 || ms < best) {
+      best = ms;
+      besti = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  pick = (int8_t)besti;
+  return besti;
+}
+
+void spmm_tile(int cfg, int *vw, int *u) {
+  *vw = SPMM_VW[cfg];
+  *u = SPMM_U[cfg];
+}
+
 template <class T>
 void op_bspmv(const DCsr &A, const T *X, T *Y, int kp, DevBuf<double> &parts, hipStream_t s) {
   dispatch_spmm<T>(A, X, kp, BStore<T>{Y, kp}, parts, s);
@@ -395,6 +506,7 @@ void op_bsub_block_mean(T *a, int kp, const RedPlan &P, const double *sums, hipS
 
 #define LG_BINST(T)                                                                                  \
   template void op_bspmv<T>(const DCsr &, const T *, T *, int, DevBuf<double> &, hipStream_t);       \
+  template int tune_spmm<T>(DCsr &, int, DevBuf<double> &, hipStream_t);                             \
   template void op_bresidual<T>(const DCsr &, const T *, const T *, T *, int, DevBuf<double> &,     \
                                 hipStream_t);                                                        \
   template void op_bcheb_step<T>(const DCsr &, const T *, const T *, const T *, const T *, T *,     \

@@ -57,6 +57,9 @@ struct Units {
   DevBuf<int32_t> lpart;  // nlong + 1 slot offsets
   DevBuf<double> parts;   // partial sums (also used as float storage)
   int32_t row_items = 4;  // entries per lane of the row reduction (row_items_for)
+  // batched SpMM tile picked per precision (0 FP64, 1 FP32) and log2 of the
+  // block width; -1 until tuned (the fixed tile is used)
+  int8_t spmm_cfg[2][8] = {{-1, -1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1, -1}};
 };
 
 struct HugeRows;  // rows too long for one wavefront of the setup row kernels

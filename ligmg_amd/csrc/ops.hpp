@@ -81,6 +81,12 @@ template <class T> void op_sub_block_mean(T *a, const RedPlan &P, const double *
 // row-major n x kp, kp a power of two <= 64.
 template <class T>
 void op_bspmv(const DCsr &A, const T *X, T *Y, int kp, DevBuf<double> &parts, hipStream_t s);
+// picks A's batched SpMM tile for this precision and block width by timing
+// the candidates once (kept in A.U); returns the tile id
+template <class T>
+int tune_spmm(DCsr &A, int kp, DevBuf<double> &parts, hipStream_t s);
+// right-hand sides per lane and gathers in flight of tile id cfg
+void spmm_tile(int cfg, int *vw, int *u);
 template <class T>
 void op_bresidual(const DCsr &A, const T *X, const T *B, T *R, int kp, DevBuf<double> &parts,
                   hipStream_t s);

@@ -12,6 +12,8 @@
 // matrix (its rows are extracted to CSR and uploaded once at PCSetUp); the
 // reference requires a MatShell wrapping a CombBLAS SpParMat.  Apply runs a
 // full solve to the options' tolerance, as the reference does (:40-62).
+// Everything but the PETSc calls lives in the library's ligmg_pc_* helper
+// (csrc/pc.hip), which tests drive without PETSc.
 #include <petsc/private/pcimpl.h>
 #include <petscksp.h>
 
@@ -21,65 +23,54 @@
 
 namespace {
 
-struct LigmgPC {
-  ligmg_handle h = nullptr;
-  PetscInt n = 0;
-};
+PetscErrorCode check(int rc, const char *what) {
+  PetscCheck(rc == LIGMG_OK, PETSC_COMM_SELF, PETSC_ERR_LIB, "%s: %s", what, ligmg_last_error());
+  return PETSC_SUCCESS;
+}
 
+// rows of pc->pmat in order into the helper, then the solver with default
+// options (ligmg_pc_setup)
 PetscErrorCode PCSetUp_LIGMG(PC pc) {
   PetscFunctionBegin;
-  LigmgPC *ctx = (LigmgPC *)pc->data;
-  if (ctx->h) {
-    ligmg_destroy(ctx->h);
-    ctx->h = nullptr;
-  }
+  ligmg_pc_handle h = (ligmg_pc_handle)pc->data;
   PetscInt m, n;
   PetscCall(MatGetSize(pc->pmat, &m, &n));
   PetscCheck(m == n, PETSC_COMM_SELF, PETSC_ERR_ARG_WRONG, "ligmg needs a square matrix");
-  std::vector<int32_t> rp(m + 1, 0), ci;
-  std::vector<double> v;
+  PetscCall(check(ligmg_pc_begin(h, m), "ligmg_pc_begin"));
+  std::vector<int64_t> c64;
+  std::vector<double> v64;
   for (PetscInt i = 0; i < m; i++) {
     PetscInt nc;
     const PetscInt *cols;
     const PetscScalar *vals;
     PetscCall(MatGetRow(pc->pmat, i, &nc, &cols, &vals));
-    for (PetscInt k = 0; k < nc; k++) {
-      ci.push_back((int32_t)cols[k]);
-      v.push_back((double)PetscRealPart(vals[k]));
-    }
-    rp[i + 1] = (int32_t)ci.size();
+    c64.assign(cols, cols + nc);
+    v64.resize(nc);
+    for (PetscInt k = 0; k < nc; k++) v64[k] = (double)PetscRealPart(vals[k]);
+    const int rc = ligmg_pc_add_row(h, i, nc, c64.data(), v64.data());
     PetscCall(MatRestoreRow(pc->pmat, i, &nc, &cols, &vals));
+    PetscCall(check(rc, "ligmg_pc_add_row"));
   }
-  ligmg_options o;
-  ligmg_options_default(&o);  // the reference PC always uses default_options()
-  int rc = ligmg_create(m, rp.data(), ci.data(), v.data(), LIGMG_MEM_HOST, &o, 0, nullptr, &ctx->h);
-  PetscCheck(rc == LIGMG_OK, PETSC_COMM_SELF, PETSC_ERR_LIB, "ligmg_create: %s", ligmg_last_error());
-  ctx->n = m;
+  PetscCall(check(ligmg_pc_setup(h), "ligmg_pc_setup"));
   PetscFunctionReturn(PETSC_SUCCESS);
 }
 
 PetscErrorCode PCApply_LIGMG(PC pc, Vec x, Vec y) {
   PetscFunctionBegin;
-  LigmgPC *ctx = (LigmgPC *)pc->data;
   const PetscScalar *b;
   PetscScalar *out;
   PetscCall(VecGetArrayRead(x, &b));
   PetscCall(VecGetArray(y, &out));
-  ligmg_info info;
-  int rc = ligmg_solve(ctx->h, (const double *)b, (double *)out, LIGMG_MEM_HOST, &info, nullptr, 0);
+  const int rc = ligmg_pc_apply((ligmg_pc_handle)pc->data, (const double *)b, (double *)out, nullptr);
   PetscCall(VecRestoreArray(y, &out));
   PetscCall(VecRestoreArrayRead(x, &b));
-  PetscCheck(rc == LIGMG_OK, PETSC_COMM_SELF, PETSC_ERR_LIB, "ligmg_solve: %s", ligmg_last_error());
+  PetscCall(check(rc, "ligmg_pc_apply"));
   PetscFunctionReturn(PETSC_SUCCESS);
 }
 
 PetscErrorCode PCDestroy_LIGMG(PC pc) {
   PetscFunctionBegin;
-  LigmgPC *ctx = (LigmgPC *)pc->data;
-  if (ctx) {
-    if (ctx->h) ligmg_destroy(ctx->h);
-    delete ctx;
-  }
+  ligmg_pc_destroy((ligmg_pc_handle)pc->data);
   pc->data = nullptr;
   PetscFunctionReturn(PETSC_SUCCESS);
 }
@@ -88,7 +79,9 @@ PetscErrorCode PCDestroy_LIGMG(PC pc) {
 
 extern "C" PetscErrorCode PCCreate_LIGMG(PC pc) {
   PetscFunctionBegin;
-  pc->data = new LigmgPC();
+  ligmg_pc_handle h = nullptr;
+  PetscCall(check(ligmg_pc_create(&h), "ligmg_pc_create"));
+  pc->data = (void *)h;
   pc->ops->apply = PCApply_LIGMG;
   pc->ops->applytranspose = PCApply_LIGMG;  // L is symmetric
   pc->ops->setup = PCSetUp_LIGMG;

@@ -1632,6 +1632,39 @@ struct Solver {
     bparts.alloc((size_t)maxparts * kp);
     bpart_red.alloc((size_t)maxseg * 2 * kp);
     bkp = kp;
+    if (o.spmm_tune) {
+      // every operator a batched solve applies: the FP64 ones (outer loop,
+      // FP64 cycles) and, in mixed precision, their FP32 forms
+      for (size_t l = 0; l < H.size(); l++) {
+        DLevel &L = H[l];
+        int t64[4] = {-1, -1, -1, -1}, t32[4] = {-1, -1, -1, -1}, i = 0;
+        for (DCsr *M : {&L.A, &L.M, &L.P, &L.R}) {
+          if (M->U.nunits) {
+            t64[i] = tune_spmm<double>(*M, kp, bparts, s);
+            if (mixed) t32[i] = tune_spmm<float>(*M, kp, bparts, s);
+          }
+          i++;
+        }
+        if (o.verbose) {
+          std::string line;
+          const char *nm[4] = {"A", "M", "P", "R"};
+          for (int j = 0; j < 4; j++) {
+            int vw, u;
+            if (t64[j] >= 0) {
+              spmm_tile(t64[j], &vw, &u);
+              line += std::string(" ") + nm[j] + " fp64 " + std::to_string(vw) + "x" + std::to_string(u);
+            }
+            if (t32[j] >= 0) {
+              spmm_tile(t32[j], &vw, &u);
+              line += std::string(" ") + nm[j] + " fp32 " + std::to_string(vw) + "x" + std::to_string(u);
+            }
+          }
+          if (!line.empty())
+            fprintf(stderr, "[ligmg] spmm tiles k=%d level %zu (rhs per lane x gathers in flight):%s\n", kp, l,
+                    line.c_str());
+        }
+      }
+    }
   }
 
   // block version of cycle(): same buffer discipline
@@ -1971,8 +2004,12 @@ int Solver::solve(const double *b, double *x, int mem, ligmg_info *info, double 
   LG_CHECK(from >= 0 && from < (int)H.size(), LIGMG_ERR_INVALID,
            "ligmg_solve_from: level " + std::to_string(from) + " out of range (" + std::to_string(H.size()) +
                " levels)");
-  const DLevel &L0 = H[from];
+  DLevel &L0 = H[from];
   const int64_t n = L0.A.n;
+  // the matrix of an elimination level below the top is not applied by the
+  // cycles, so it has no SpMV units until a solve starts from it (full
+  // residual, or the outer Krylov solve when top_elim is off)
+  if (L0.A.U.nunits == 0 && L0.A.nnz > 0) build_units(L0.A, s);
   // caller order -> solve layout: the input permutation and level 0's
   // layout for level 0, the level's own layout below it
   const int32_t *ip = from == 0 ? perm.p : L0.sperm.p;
@@ -2135,6 +2172,7 @@ void ligmg_options_default(ligmg_options *o) {
   o->tail_nnz = 4096;
   o->rejection_ratio = -1;
   o->rejection_to_seed = 0;
+  o->spmm_tune = 1;
 }
 
 const char *ligmg_last_error(void) { return lg::g_last_error.c_str(); }
@@ -2524,6 +2562,96 @@ int ligmg_bench_smoother(ligmg_handle h, int32_t l, int32_t prec, int32_t iters,
     else *avg_ms = time_cheb<double>(S, L, L.dinv.p, iters, S.s);
     const int64_t sv = f32 ? 4 : 8, sx = f32 ? 4 : 8;
     *bytes = L.A.nnz * (sv + 4) + (n + 1) * 4 + 5 * n * sx + (f32 && L.A.d32.p ? n * 8 : 0);
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_bench_spmm(ligmg_handle h, int32_t l, int32_t prec, int32_t k, int32_t iters, double *avg_ms,
+                     int64_t *bytes) {
+  if (!h || l < 0 || l >= (int)h->S.H.size() || k < 1 || k > 64 || iters < 1 || !avg_ms || !bytes)
+    return LIGMG_ERR_INVALID;
+  try {
+    lg::Solver &S = h->S;
+    lg::DCsr &A = S.H[l].A;
+    hipStream_t s = S.s;
+    if (A.U.nunits == 0) lg::build_units(A, s);
+    const bool f32 = prec == LIGMG_PREC_MIXED;
+    if (f32 && !A.v32.p) lg::make_v32(A, s, true);
+    int kp = 1;
+    while (kp < k) kp <<= 1;
+    const int64_t n = A.n;
+    lg::DevBuf<double> parts((size_t)std::max<int64_t>(A.U.nparts, 1) * kp);
+    lg::DevBuf<double> x(f32 ? 0 : n * kp), y(f32 ? 0 : n * kp);
+    lg::DevBuf<float> x32(f32 ? n * kp : 0), y32(f32 ? n * kp : 0);
+    if (f32) {
+      lg::DevBuf<double> t(n * kp);
+      lg::fill_f64(t.p, 1.0, n * kp, s);
+      lg::op_d2f(t.p, x32.p, n * kp, s);
+      LG_HIP(hipStreamSynchronize(s));
+    } else {
+      lg::fill_f64(x.p, 1.0, n * kp, s);
+    }
+    // the batched solve's tile for this operator and width
+    if (S.o.spmm_tune) {
+      if (f32) lg::tune_spmm<float>(A, kp, parts, s);
+      else lg::tune_spmm<double>(A, kp, parts, s);
+    }
+    auto run = [&]() {
+      if (f32) lg::op_bspmv<float>(A, x32.p, y32.p, kp, parts, s);
+      else lg::op_bspmv<double>(A, x.p, y.p, kp, parts, s);
+    };
+    run();  // warm-up
+    hipEvent_t e0, e1;
+    LG_HIP(hipEventCreate(&e0));
+    LG_HIP(hipEventCreate(&e1));
+    LG_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < iters; i++) run();
+    LG_HIP(hipEventRecord(e1, s));
+    LG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / iters;
+    const int64_t sv = f32 ? 4 : 8, sx = f32 ? 4 : 8;
+    *bytes = A.nnz * (sv + 4) + (n + 1) * 4 + 2 * n * kp * sx;
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_bench_coarse(ligmg_handle h, int32_t k, int32_t iters, double *avg_ms, double *flops) {
+  if (!h || k < 1 || k > 64 || iters < 1 || !avg_ms || !flops) return LIGMG_ERR_INVALID;
+  try {
+    lg::Solver &S = h->S;
+    lg::DLevel &L = S.H.back();
+    LG_CHECK(L.kind == LIGMG_LEVEL_COARSE && L.pinv.p, LIGMG_ERR_INVALID, "ligmg_bench_coarse: no dense coarse level");
+    int kp = 1;
+    while (kp < k) kp <<= 1;
+    const int64_t n = L.n;
+    lg::DevBuf<double> X((size_t)std::max<int64_t>(n, 1) * kp), Y((size_t)std::max<int64_t>(n, 1) * kp);
+    lg::fill_f64(X.p, 1.0, (int64_t)X.n, S.s);
+    auto once = [&] {
+      if (kp == 1) lg::op_dense_apply<double>(L.pinv.p, n, X.p, Y.p, S.s);
+      else lg::op_bdense_apply<double>(L.pinv.p, n, X.p, Y.p, kp, S.s);
+    };
+    once();
+    hipEvent_t e0, e1;
+    LG_HIP(hipEventCreate(&e0));
+    LG_HIP(hipEventCreate(&e1));
+    LG_HIP(hipEventRecord(e0, S.s));
+    for (int i = 0; i < iters; i++) once();
+    LG_HIP(hipEventRecord(e1, S.s));
+    LG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / iters;
+    *flops = 2.0 * (double)n * (double)n * (double)kp;
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return fail(e);

@@ -38,6 +38,11 @@
 
 #include "../ligmg_amd/csrc/rng.hpp"
 
+#ifdef _OPENMP
+#include <omp.h>
+#include <parallel/algorithm>
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -175,8 +180,12 @@ double diag_of(const Csr &A, int64_t i) {
   return 0.0;
 }
 
-// y = A x
+// y = A x.  Loops marked "omp" run their iterations on several threads when
+// the oracle is built with OpenMP; each iteration writes its own outputs and
+// keeps its own summation order, so the results do not depend on the thread
+// count (the CPU baseline's multi-threaded leg).
 void spmv(const Csr &A, const double *x, double *y) {
+#pragma omp parallel for schedule(dynamic, 512)
   for (int64_t i = 0; i < A.n; i++) {
     double s = 0.0;
     for (int32_t k = A.rp[i]; k < A.rp[i + 1]; k++) s += A.v[k] * x[A.ci[k]];
@@ -255,9 +264,13 @@ struct Entry {
 // Contract a list of (row, col, value) triples into CSR: stable order by
 // (row, col), values of equal keys combined with canon_sum in list order.
 Csr contract(int64_t nrows, std::vector<Entry> &E) {
-  std::stable_sort(E.begin(), E.end(), [](const Entry &a, const Entry &b) {
-    return a.r != b.r ? a.r < b.r : a.c < b.c;
-  });
+  auto by_key = [](const Entry &a, const Entry &b) { return a.r != b.r ? a.r < b.r : a.c < b.c; };
+  // a stable sort has one result, whatever algorithm or thread count
+#ifdef _OPENMP
+  __gnu_parallel::stable_sort(E.begin(), E.end(), by_key);
+#else
+  std::stable_sort(E.begin(), E.end(), by_key);
+#endif
   Csr C;
   C.n = nrows;
   C.rp.assign(nrows + 1, 0);
@@ -364,11 +377,12 @@ void tv_init(int64_t n, uint64_t seed, int level, int T, double *X) {
 // ---- a8: jacobi with IgnoreDiagSR, b = 0 (smoothing.hpp:11-32, 78-98) ------
 void tv_jacobi(const Csr &A, double *X, int T, int sweeps) {
   const double w = 2.0 / 3.0;
-  std::vector<double> Y((size_t)A.n * T), buf;
+  std::vector<double> Y((size_t)A.n * T);
   for (int s = 0; s < sweeps; s++) {
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < A.n; i++) {
       int32_t len = A.rp[i + 1] - A.rp[i];
-      buf.resize(len);
+      std::vector<double> buf(len);
       double d = 0;
       bool has_d = false;
       for (int32_t k = A.rp[i]; k < A.rp[i + 1]; k++)
@@ -420,10 +434,10 @@ int32_t transpose_pos(const Csr &A, int64_t r, int32_t c) {
 // summing over the stored pattern equals summing over the pruned matrix.
 void column_sums(const Csr &A, const std::vector<double> &V, std::vector<double> &out) {
   out.assign(A.n, 0.0);
-  std::vector<double> buf;
+#pragma omp parallel for schedule(dynamic, 512)
   for (int64_t c = 0; c < A.n; c++) {
     const int32_t len = A.rp[c + 1] - A.rp[c];
-    buf.assign(len, 0.0);
+    std::vector<double> buf(len, 0.0);
     for (int32_t k = A.rp[c]; k < A.rp[c + 1]; k++) buf[k - A.rp[c]] = V[transpose_pos(A, c, A.ci[k])];
     out[c] = seq_sum(buf.data(), len);
   }
@@ -439,9 +453,11 @@ void strength(const Csr &A, double *X, int T, int sweeps, int metric, int scalin
     nrm[i] = s;
   }
   std::vector<double> raw(A.nnz()), rowmax(A.n, 0.0), colmax(A.n, 0.0);
+#pragma omp parallel for schedule(dynamic, 512)
+  for (int64_t r = 0; r < A.n; r++)
+    for (int32_t k = A.rp[r]; k < A.rp[r + 1]; k++) raw[k] = affinity_raw(metric, X, T, nrm.data(), r, A.ci[k], A.v[k]);
   for (int64_t r = 0; r < A.n; r++)
     for (int32_t k = A.rp[r]; k < A.rp[r + 1]; k++) {
-      raw[k] = affinity_raw(metric, X, T, nrm.data(), r, A.ci[k], A.v[k]);
       rowmax[r] = std::max(rowmax[r], raw[k]);
       colmax[A.ci[k]] = std::max(colmax[A.ci[k]], raw[k]);
     }
@@ -577,6 +593,7 @@ int64_t aggregate(const Csr &A, const std::vector<double> &S, const std::vector<
   std::vector<std::vector<double>> lists(n);
   for (int iter = 0; iter < o.agg_iters; iter++) {
     double th = o.agg_init_factor * std::pow(o.agg_iter_factor, iter);
+#pragma omp parallel for schedule(dynamic, 512)
     for (int64_t i = 0; i < n; i++) {
       Vote y;
       for (int32_t k = A.rp[i]; k < A.rp[i + 1]; k++) {
@@ -1361,6 +1378,15 @@ extern "C" {
 double oracle_canon_sum(const double *a, int64_t m) { return canon_sum(a, m); }
 double oracle_seq_sum(const double *a, int64_t m) { return seq_sum(a, m); }
 void oracle_set_decision_order(int32_t mode) { decision_order = mode; }
+int32_t oracle_set_threads(int32_t nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  return omp_get_max_threads();
+#else
+  (void)nthreads;
+  return 1;
+#endif
+}
 
 void oracle_default_options(ligmg_options *o) {
   memset(o, 0, sizeof(*o));
@@ -1405,6 +1431,7 @@ void oracle_default_options(ligmg_options *o) {
   o->tail_nnz = 4096;
   o->rejection_ratio = -1;
   o->rejection_to_seed = 0;
+  o->spmm_tune = 1;
 }
 
 int oracle_create(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,

@@ -38,6 +38,9 @@ double oracle_seq_sum(const double *a, int64_t m);
 /* test hook: 0 = the reference's order (default); 1 = plant the canonical
  * split in the sequential sums; 2 = plant a descending-voter tally */
 void oracle_set_decision_order(int32_t mode);
+/* threads of the OpenMP loops (nthreads <= 0: unchanged); returns the
+ * count now in effect (1 without OpenMP).  Results do not depend on it. */
+int32_t oracle_set_threads(int32_t nthreads);
 void oracle_default_options(ligmg_options *opts);
 
 int oracle_create(int64_t n, const int32_t *rowptr, const int32_t *colind,

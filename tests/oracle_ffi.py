@@ -18,6 +18,7 @@ PROTOS = {
     "oracle_canon_sum": (f64, [P(f64), i64]),
     "oracle_seq_sum": (f64, [P(f64), i64]),
     "oracle_set_decision_order": (None, [i32]),
+    "oracle_set_threads": (i32, [i32]),
     "oracle_default_options": (None, [P(_ffi.ligmg_options)]),
     "oracle_create": (C.c_int, [i64, P(i32), P(i32), P(f64), P(_ffi.ligmg_options), i32, P(i64),
                                 P(H)]),

@@ -116,3 +116,43 @@ def test_union_with_collapsing_blocks(prec):
         nb = np.linalg.norm(B[lo:hi], axis=0)
         nr = np.linalg.norm(R[lo:hi], axis=0)
         assert np.all(nr <= 1e-7 * nb + 1e-300), (blk, nr / nb)
+
+
+@pytest.mark.parametrize("k", [1, 2, 7, 16, 64])
+@pytest.mark.parametrize("prec", [_ffi.PREC_FP64, _ffi.PREC_MIXED])
+def test_spmm_tiles_tuned_equal_fixed(k, prec):
+    """The batched SpMM tile tuner (right-hand sides per lane, gathers in
+    flight, picked per level operator by timing) changes no bit: every
+    (row, right-hand side) sum of a complete row is one lane adding the
+    row's entries in order, whatever the tile; hub-row chunks keep one tile."""
+    g = graphs.union([graphs.rmat_config(12, "lognormal", pendants=True),
+                      graphs.grid3d(12, 10, 9, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=2)])
+    B = graphs.rhs(g.n, k, 5, comp_ptr=g.comp_ptr).reshape(g.n, k, order="F")
+    out = []
+    for tune in (0, 1):
+        s = ligmg_amd.Solver(g, ligmg_amd.Options(precision=prec, outer_ksp_type=_ffi.KSP_FCG, spmm_tune=tune),
+                             comp_ptr=g.comp_ptr)
+        X, info, its = s.solve_batch(B)
+        assert info.status == _ffi.CONVERGED
+        out.append((X, its))
+    print(f"k={k} prec={prec} iterations {out[0][1].min()}..{out[0][1].max()}")
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_grid3d_48_iterations_match_oracle():
+    """The 3-D grid of the batched workload at 48^3 (weights uniform[0.1, 10]):
+    the device's FP64 solve needs the oracle's iteration count (35), and the
+    batched FP32-preconditioned columns the same within two.  The count grows
+    with the grid in the oracle too (24^3: 24, 48^3: 35, 96^3: 55), which is
+    where the 160^3 grid's 74-81 come from."""
+    g = graphs.grid3d(48, 48, 48, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=0)
+    b = graphs.rhs(g.n, 1, 0)
+    opts = dict(outer_ksp_type=_ffi.KSP_FCG)
+    _, rinfo, _ = O.OracleSolver(g, O.default_options(**opts)).solve(b)
+    one = ligmg_amd.Solver(g, ligmg_amd.Options(**opts)).solve(b)
+    print(f"oracle {rinfo.iters} device fp64 {one.iters()}")
+    assert abs(one.iters() - rinfo.iters) <= 1
+    B = graphs.rhs(g.n, 8, 1).reshape(g.n, 8, order="F")
+    X, info, its = ligmg_amd.Solver(g, ligmg_amd.Options(precision=_ffi.PREC_MIXED, **opts)).solve_batch(B)
+    print(f"batched mixed iterations {its.min()}..{its.max()}")
+    assert info.status == _ffi.CONVERGED and np.all(np.abs(its - rinfo.iters) <= 2)

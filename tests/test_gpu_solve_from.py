@@ -68,6 +68,24 @@ def test_solve_from_matches_oracle(pend, which, variant):
     assert info.cycle_complexity() == pytest.approx(rinfo.cycle_complexity, rel=1e-12)
 
 
+def test_solve_from_elimination_level_without_top_elim(pend):
+    """top_elim off: the outer Krylov solve runs on the elimination level's
+    own matrix, which the cycles never apply."""
+    d = ligmg_amd.Solver(pend, ligmg_amd.Options(top_elim=0))
+    r = O.OracleSolver(pend, O.default_options(top_elim=0))
+    kinds = [d.level_info(l)[0] for l in range(d.num_levels())]
+    l, _ = pick_levels(kinds)
+    A, b = level_system(d, l, seed=3)
+    info = d.solve(b, level=l)
+    xr, rinfo, rhist = r.solve(b, level=l)
+    assert info.base_level == l == rinfo.base_level
+    assert info.status == _ffi.CONVERGED and info.rel_residual() <= 1e-8
+    k = min(6, len(rhist), len(info.residuals))
+    assert np.allclose(np.asarray(info.residuals)[:k], rhist[:k], rtol=1e-7)
+    full = np.linalg.norm(b - A @ info.solution) / np.linalg.norm(b)
+    assert info.full_rel_residual == pytest.approx(full, rel=1e-6, abs=1e-14)
+
+
 def test_solve_from_level0_is_solve(pend):
     d = ligmg_amd.Solver(pend)
     b = graphs.rhs(pend.n, 1, 3)

@@ -257,6 +257,53 @@ def test_strength_matches_numpy_formula(scaling):
         assert S.max() <= 1.0 + 1e-15
 
 
+@pytest.mark.parametrize("scaling", [_ffi.AFF_MAX, _ffi.AFF_ROW])
+@pytest.mark.parametrize("metric", [_ffi.STRENGTH_ALGEBRAIC_DISTANCE, _ffi.STRENGTH_NONE])
+def test_strength_metrics_match_numpy_formula(metric, scaling):
+    """aggregation.hpp:333-346: algebraic distance s = 1 / max_t |x_t[j] -
+    x_t[i]| on the smoothed test vectors; None s = -L_ij for negative
+    entries (no test vectors); the diagonal 0 either way, then the scaling
+    and the 1e-8 prune."""
+    g = random_graph(40, 0.12, 11)
+    n, T = g.n, 10
+    X = np.empty(n * T)
+    O.lib().oracle_tv_init(n, 1, 0, T, O.ptr(X, O.f64))
+    X0 = X.reshape(n, T).copy()
+    rp, ci, v = O.csr_args(g)
+    S = np.empty(g.nnz)
+    cs = np.empty(n)
+    assert O.lib().oracle_strength(n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(v, O.f64),
+                                   O.ptr(X, O.f64), T, 4, metric, scaling, 1e-8,
+                                   O.ptr(S, O.f64), O.ptr(cs, O.f64)) == 0
+    A = dense(g)
+    rows = np.repeat(np.arange(n), np.diff(g.rowptr))
+    cols = g.colind
+    if metric == _ffi.STRENGTH_NONE:
+        assert np.array_equal(X.reshape(n, T), X0)  # no smoothing without test vectors
+        raw = np.where(g.vals < 0, -g.vals, 0.0)
+    else:
+        d = np.diag(A)
+        N = np.where(A < 0, A, 0.0)
+        Y = X0.copy()
+        for _ in range(4):
+            Y = (2 / 3) * (-(N @ Y)) / d[:, None] + (1 / 3) * Y
+        assert np.allclose(X.reshape(n, T), Y, rtol=1e-12, atol=1e-15)
+        dist = np.abs(Y[cols] - Y[rows]).max(1)
+        with np.errstate(divide="ignore"):
+            raw = np.where(rows != cols, 1.0 / dist, 0.0)
+    Raw = sp.csr_matrix((raw, cols, g.rowptr), shape=(n, n)).toarray()
+    if scaling == _ffi.AFF_ROW:
+        sc = (Raw / Raw.sum(1)[:, None])[rows, cols]
+    else:
+        rmax, cmax = Raw.max(1), Raw.max(0)
+        sc = raw / np.maximum(rmax[rows], cmax[cols])
+    sc[sc < 1e-8] = 0.0
+    assert np.allclose(S, sc, rtol=1e-12, atol=0)
+    assert np.all(S[rows == cols] == 0) and np.all(S[(rows != cols) & (g.vals < 0)] > 0)
+    Sm = sp.csr_matrix((S, cols, g.rowptr), shape=(n, n)).toarray()
+    assert np.allclose(cs, Sm.sum(0), rtol=1e-12)
+
+
 # ------------------------------------------------------------ voting (a11-a14)
 
 def aggregate(g, S, **opts):

@@ -117,3 +117,23 @@ def test_knife_edge_two_rounds_differ(plant):
     plant(1)
     agg1, _, _, nc1 = oracle_aggregate(g, S, agg_iters=2)
     assert nc1 > nc and not np.array_equal(agg, agg1)
+
+
+def test_oracle_results_do_not_depend_on_threads():
+    """The oracle's OpenMP loops (the CPU baseline's multi-threaded leg) give
+    the single-threaded bits: same hierarchy, same solution."""
+    g = graphs.rmat_config(12, "lognormal", pendants=True)
+    b = graphs.rhs(g.n, 1, 0)
+    out = []
+    for t in (1, 8):
+        assert O.lib().oracle_set_threads(t) == t
+        r = O.OracleSolver(g)
+        x, info, hist = r.solve(b)
+        mats = [r.level_matrix(l) for l in range(r.num_levels())]
+        out.append((x, hist, mats))
+    O.lib().oracle_set_threads(1)
+    (x1, h1, m1), (x8, h8, m8) = out
+    assert np.array_equal(x1, x8) and np.array_equal(h1, h8)
+    for a, c in zip(m1, m8):
+        for u, v in zip(a, c):
+            assert np.array_equal(u, v)
