@@ -363,6 +363,20 @@ int ligmg_pc_destroy(ligmg_pc_handle pc);
 int ligmg_components(int64_t n, const int32_t *rowptr, const int32_t *colind,
                      int mem_kind, int32_t *label, int64_t *ncomp);
 
+/* ligmg_components without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  rowptr, colind and label are device
+ * pointers; n may be at most LIGMG_NOALLOC_CC_MAX_N.  The work space (three
+ * words per vertex plus flags, 48 MiB) is a static array that each GPU the
+ * library runs on holds until the process ends; every call on a GPU shares
+ * it, so calls that overlap on one GPU (two host threads) race.  Work runs
+ * on the calling thread's default stream (hipStreamPerThread) and is
+ * finished when the call returns.  Same results as ligmg_components. */
+#define LIGMG_NOALLOC_CC_MAX_N 4194304
+int ligmg_components_noalloc(int64_t n, const int32_t *rowptr,
+                             const int32_t *colind, int32_t *label,
+                             int64_t *ncomp);
+
 /* largest_cc (cc.hpp:55-105), first half: new_id[i] = rank of i among the
  * vertices of the largest component (ascending original order), -1
  * elsewhere; ties between equally large components go to the smallest

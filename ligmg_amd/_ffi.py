@@ -66,6 +66,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 ORDER_NATURAL, ORDER_DEGREE = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
 LEVEL_ELIM, LEVEL_AGG, LEVEL_COARSE = 0, 1, 2
+NOALLOC_CC_MAX_N = 4194304  # LIGMG_NOALLOC_CC_MAX_N
 LEVEL_NAMES = {LEVEL_ELIM: "elim", LEVEL_AGG: "agg", LEVEL_COARSE: "coarse"}
 CONVERGED, DIVERGED_ITS, DIVERGED_BREAKDOWN, DIVERGED_NAN = range(4)
 W_UNIT, W_UNIFORM, W_LOGNORMAL, W_GAUSS_DIST = range(4)
@@ -110,6 +111,7 @@ SOLVER_PROTOS = {
     "ligmg_bench_spmv": (C.c_int, [HANDLE, i32, i32, i32, P(f64), P(i64)]),
     "ligmg_bench_smoother": (C.c_int, [HANDLE, i32, i32, i32, P(f64), P(i64)]),
     "ligmg_components": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, P(i64)]),
+    "ligmg_components_noalloc": (C.c_int, [i64, C.c_void_p, C.c_void_p, C.c_void_p, P(i64)]),
     "ligmg_bench_coarse": (C.c_int, [HANDLE, i32, i32, P(f64), P(f64)]),
     "ligmg_bench_spmm": (C.c_int, [HANDLE, i32, i32, i32, i32, P(f64), P(i64)]),
     "ligmg_pc_create": (C.c_int, [P(C.c_void_p)]),

@@ -12,6 +12,8 @@
 // the result is the same.  Every kernel is a bounded data-parallel pass: the
 // host repeats them until nothing changes.
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "../../include/ligmg_amd.h"
 #include "prims.hpp"
@@ -230,25 +232,61 @@ int64_t nnz_of(const int32_t *rp, int64_t n, int mem, hipStream_t s) {
   return e[1];
 }
 
+// roots (count > 0) of the forest: the number of components
+__global__ void k_cc_count_roots(int64_t n, const int32_t *count, int *ncomp) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (count[i] > 0) atomicAdd(ncomp, 1);
+}
+
 // hooks + shortcuts to a forest of stars whose roots are the smallest index
-// of every component
-void cc_forest(int64_t n, const int32_t *rp, const int32_t *ci, DevBuf<int32_t> &parent, hipStream_t s) {
-  parent.alloc(n);
-  hipLaunchKernelGGL(k_cc_init, dim3(gs(n)), dim3(256), 0, s, n, parent.p);
+// of every component; changed is one word of device scratch
+void cc_forest_into(int64_t n, const int32_t *rp, const int32_t *ci, int32_t *parent, int *changed,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_cc_init, dim3(gs(n)), dim3(256), 0, s, n, parent);
   LG_LAUNCH_CHECK();
-  DevBuf<int> changed(1);
   for (int round = 0;; round++) {
     LG_CHECK(round < 256, LIGMG_ERR_NUMERIC, "connected components did not converge");
-    fill_i32(changed.p, 0, 1, s);
-    hipLaunchKernelGGL(k_cc_hook, dim3(gs(n * WAVE)), dim3(256), 0, s, n, rp, ci, parent.p, changed.p);
+    fill_i32(changed, 0, 1, s);
+    hipLaunchKernelGGL(k_cc_hook, dim3(gs(n * WAVE)), dim3(256), 0, s, n, rp, ci, parent, changed);
     LG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_cc_jump, dim3(gs(n)), dim3(256), 0, s, n, parent.p);
+    hipLaunchKernelGGL(k_cc_jump, dim3(gs(n)), dim3(256), 0, s, n, parent);
     LG_LAUNCH_CHECK();
     int h = 0;
-    to_host(&h, changed.p, 1, s);
+    to_host(&h, changed, 1, s);
     LG_HIP(hipStreamSynchronize(s));
     if (!h) break;
   }
+}
+
+void cc_forest(int64_t n, const int32_t *rp, const int32_t *ci, DevBuf<int32_t> &parent, hipStream_t s) {
+  parent.alloc(n);
+  DevBuf<int> changed(1);
+  cc_forest_into(n, rp, ci, parent.p, changed.p, s);
+}
+
+// connected_components on a validated device CSR, into caller buffers:
+// parent, maxv and count hold n words each, flags 2 (the forest's change
+// flag, the component count); label (device, n) receives the labels.
+// Returns the number of components.  Every buffer is written before it is
+// read, so scratch left over from an earlier call does not matter.
+int64_t cc_labels_into(int64_t n, const int32_t *rp, const int32_t *ci, int32_t *parent, int32_t *maxv,
+                       int32_t *count, int *flags, int32_t *label, hipStream_t s) {
+  cc_forest_into(n, rp, ci, parent, flags, s);
+  fill_i32(maxv, -1, n, s);
+  fill_i32(count, 0, n, s);
+  fill_i32(flags + 1, 0, 1, s);
+  hipLaunchKernelGGL(k_cc_stats, dim3(gs(n)), dim3(256), 0, s, n, (const int32_t *)parent, maxv, count);
+  LG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_cc_label, dim3(gs(n)), dim3(256), 0, s, n, (const int32_t *)parent, (const int32_t *)maxv,
+                     label);
+  LG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_cc_count_roots, dim3(gs(n)), dim3(256), 0, s, n, (const int32_t *)count, flags + 1);
+  LG_LAUNCH_CHECK();
+  int k = 0;
+  to_host(&k, flags + 1, 1, s);
+  LG_HIP(hipStreamSynchronize(s));
+  return k;
 }
 
 }  // namespace
@@ -264,36 +302,69 @@ __global__ void k_sorted_check(int64_t n, const int32_t *rp, const int32_t *ci, 
   }
 }
 
-void validate_csr(int64_t n, int64_t ncols, const int32_t *rp, const int32_t *ci, hipStream_t s, bool sorted) {
+// bad: one word of device scratch
+void validate_csr_into(int64_t n, int64_t ncols, const int32_t *rp, const int32_t *ci, hipStream_t s, bool sorted,
+                       int *bad) {
   int32_t m = 0;
   to_host(&m, rp + n, 1, s);
   LG_HIP(hipStreamSynchronize(s));
   LG_CHECK(m >= 0, LIGMG_ERR_INVALID, "rowptr[n] must be non-negative");
-  DevBuf<int> bad(1);
-  fill_i32(bad.p, 0, 1, s);
-  hipLaunchKernelGGL(k_rowptr_check, dim3(gs(n + 1)), dim3(256), 0, s, n, (int64_t)m, rp, bad.p);
+  fill_i32(bad, 0, 1, s);
+  hipLaunchKernelGGL(k_rowptr_check, dim3(gs(n + 1)), dim3(256), 0, s, n, (int64_t)m, rp, bad);
   LG_LAUNCH_CHECK();
   int h = 0;
-  to_host(&h, bad.p, 1, s);
+  to_host(&h, bad, 1, s);
   LG_HIP(hipStreamSynchronize(s));
   LG_CHECK(!(h & 1), LIGMG_ERR_INVALID, "rowptr must start at 0 and be non-decreasing");
-  hipLaunchKernelGGL(k_colind_check, dim3(gs(m)), dim3(256), 0, s, (int64_t)m, ncols, ci, bad.p);
+  hipLaunchKernelGGL(k_colind_check, dim3(gs(m)), dim3(256), 0, s, (int64_t)m, ncols, ci, bad);
   LG_LAUNCH_CHECK();
-  to_host(&h, bad.p, 1, s);
+  to_host(&h, bad, 1, s);
   LG_HIP(hipStreamSynchronize(s));
   LG_CHECK(!(h & 2), LIGMG_ERR_INVALID, "column index out of range");
   if (sorted) {
-    hipLaunchKernelGGL(k_sorted_check, dim3(gs(n * WAVE)), dim3(256), 0, s, n, rp, ci, bad.p);
+    hipLaunchKernelGGL(k_sorted_check, dim3(gs(n * WAVE)), dim3(256), 0, s, n, rp, ci, bad);
     LG_LAUNCH_CHECK();
-    to_host(&h, bad.p, 1, s);
+    to_host(&h, bad, 1, s);
     LG_HIP(hipStreamSynchronize(s));
     LG_CHECK(!(h & 4), LIGMG_ERR_INVALID, "columns must be sorted and unique within every row");
   }
 }
 
+void validate_csr(int64_t n, int64_t ncols, const int32_t *rp, const int32_t *ci, hipStream_t s, bool sorted) {
+  DevBuf<int> bad(1);
+  validate_csr_into(n, ncols, rp, ci, s, sorted, bad.p);
+}
+
 }  // namespace lg
 
+// Scratch of ligmg_components_noalloc: parent, per-root maximum and per-root
+// count (LIGMG_NOALLOC_CC_MAX_N words each), then flag words.  Each GPU the
+// library runs on holds its own copy until the process ends.  Host code
+// never uses this name as an address: scratch_on_device() looks the
+// device's copy up once per device.
+constexpr int64_t NOALLOC_CC_N = LIGMG_NOALLOC_CC_MAX_N;
+constexpr int64_t NOALLOC_CC_WORDS = 3 * NOALLOC_CC_N + 64;
+static __device__ __align__(256) int32_t scratch[NOALLOC_CC_WORDS];
+
 namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+int32_t *scratch_addr[MAX_DEVICES] = {};
+
+// this device's copy of scratch, looked up on the first call on the device
+int32_t *scratch_on_device() {
+  int dev = 0;
+  LG_HIP(hipGetDevice(&dev));
+  LG_CHECK(dev >= 0 && dev < MAX_DEVICES, LIGMG_ERR_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_addr[dev]) {
+    void *p = nullptr;
+    LG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    scratch_addr[dev] = (int32_t *)p;
+  }
+  return scratch_addr[dev];
+}
+
 thread_local hipStream_t prep_stream = nullptr;
 hipStream_t prep_s() {
   if (!prep_stream) LG_HIP(hipStreamCreateWithFlags(&prep_stream, hipStreamNonBlocking));
@@ -315,24 +386,37 @@ int ligmg_components(int64_t n, const int32_t *rowptr, const int32_t *colind, in
     const int64_t m = lg::nnz_of(rowptr, n, mem_kind, s);
     lg::In<int32_t> rp(rowptr, n + 1, mem_kind, s), ci(colind, m, mem_kind, s);
     lg::validate_csr(n, n, rp.p, ci.p, s, false);
-    lg::DevBuf<int32_t> parent, maxv(n), count(n), lab(n);
-    lg::cc_forest(n, rp.p, ci.p, parent, s);
-    lg::fill_i32(maxv.p, -1, n, s);
-    lg::fill_i32(count.p, 0, n, s);
-    hipLaunchKernelGGL(lg::k_cc_stats, dim3(lg::gs(n)), dim3(256), 0, s, n, (const int32_t *)parent.p, maxv.p,
-                       count.p);
-    LG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(lg::k_cc_label, dim3(lg::gs(n)), dim3(256), 0, s, n, (const int32_t *)parent.p,
-                       (const int32_t *)maxv.p, lab.p);
-    LG_LAUNCH_CHECK();
+    lg::DevBuf<int32_t> parent(n), maxv(n), count(n), lab(n);
+    lg::DevBuf<int> flags(2);
+    const int64_t k = lg::cc_labels_into(n, rp.p, ci.p, parent.p, maxv.p, count.p, flags.p, lab.p, s);
     lg::out_copy(label, (const int32_t *)lab.p, n, mem_kind, s);
-    if (ncomp) {
-      auto c = lg::download(count.p, n, s);
-      int64_t k = 0;
-      for (int64_t i = 0; i < n; i++) k += c[i] > 0;
-      *ncomp = k;
-    }
+    if (ncomp) *ncomp = k;
     LG_HIP(hipStreamSynchronize(s));
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return prep_fail(e);
+  }
+}
+
+int ligmg_components_noalloc(int64_t n, const int32_t *rowptr, const int32_t *colind, int32_t *label,
+                             int64_t *ncomp) {
+  if (n <= 0 || !rowptr || !colind || !label) return LIGMG_ERR_INVALID;
+  if (n > NOALLOC_CC_N) {
+    lg::g_last_error = "ligmg_components_noalloc: n = " + std::to_string(n) + " is above the maximum of " +
+                       std::to_string(NOALLOC_CC_N) + " vertices (use ligmg_components)";
+    return LIGMG_ERR_INVALID;
+  }
+  try {
+    // the calling thread's default stream of the current device: no stream
+    // is created either
+    hipStream_t s = hipStreamPerThread;
+    int32_t *sc = scratch_on_device();
+    int32_t *parent = sc, *maxv = sc + NOALLOC_CC_N, *count = sc + 2 * NOALLOC_CC_N;
+    int *flags = (int *)(sc + 3 * NOALLOC_CC_N);
+    lg::nnz_of(rowptr, n, LIGMG_MEM_DEVICE, s);
+    lg::validate_csr_into(n, n, rowptr, colind, s, false, flags + 2);
+    const int64_t k = lg::cc_labels_into(n, rowptr, colind, parent, maxv, count, flags, label, s);
+    if (ncomp) *ncomp = k;
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return prep_fail(e);
