@@ -61,7 +61,7 @@ def cpu_full_workload():
     return out or None
 
 
-def cpu_baseline(scale):
+def cpu_baseline(scale, threads=None):
     """The CPU oracle (the project's C++ restatement, not the reference
     binary, which cannot be built without MPI/PETSc/CombBLAS) timed here on a
     bounded sample of the same construction: setup and solve, on one thread
@@ -75,7 +75,7 @@ def cpu_baseline(scale):
     b = graphs.rhs(g.n, 1, 0)
     nth = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     legs = {}
-    for th in (1, nth):
+    for th in (threads or (1, nth)):
         th = oracle_ffi.lib().oracle_set_threads(th)
         t = time.perf_counter()
         s = oracle_ffi.OracleSolver(g)
@@ -94,10 +94,9 @@ def cpu_baseline(scale):
         "cores": max(legs),
         "kind": "port",
         "sample": (f"CPU oracle FP64 V-cycle/CG to 1e-8 on R-MAT {scale} of the same construction (n={g.n}, "
-                   f"nnz={g.nnz}; the full workload has 16.6x the entries), measured in this run: solve "
-                   f"{top['solve_s']:.2f} s and setup {top['setup_s']:.2f} s on {max(legs)} threads, solve "
-                   f"{legs[1]['solve_s']:.2f} s and setup {legs[1]['setup_s']:.2f} s on 1 thread, "
-                   f"{top['iters']} iterations"),
+                   f"nnz={g.nnz}), measured in this run: "
+                   + ", ".join(f"solve {v['solve_s']:.2f} s and setup {v['setup_s']:.2f} s on {k} thread(s)"
+                               for k, v in sorted(legs.items())) + f", {top['iters']} iterations"),
         "legs": {str(k): v for k, v in legs.items()},
         "full_workload": cpu_full_workload(),
     }
@@ -289,6 +288,10 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-scale", type=int, default=19)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="only the CPU oracle leg (no GPU), at --cpu-scale with --cpu-threads")
+    ap.add_argument("--cpu-threads", default="",
+                    help="comma-separated thread counts of the CPU leg (default: 1 and the thread share)")
     ap.add_argument("--ordering", choices=["degree", "natural"], default="natural",
                     help="vertex order of the hierarchy (default: the input's)")
     ap.add_argument("--spmv-only", action="store_true",
@@ -298,6 +301,10 @@ def main():
                          "sides and a 160^3 grid x 64; grad: gradients through a 1M-vertex geometric graph; "
                          "kcycle: the FP64 K-cycle on R-MAT 22")
     args = ap.parse_args()
+    if args.cpu_only:
+        th = [int(t) for t in args.cpu_threads.split(",") if t] or None
+        print(json.dumps({"cpu_baseline": cpu_baseline(args.cpu_scale, th)}), flush=True)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build runs on one GPU only (no sharding or collectives); "
