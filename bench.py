@@ -123,7 +123,8 @@ def bench_batched(args):
     ns = np.random.default_rng(0).integers(20000, 400001, 48)
     u = graphs.union([graphs.barabasi_albert(int(n), 5, seed=i) for i, n in enumerate(ns)])
     g3 = graphs.grid3d(160, 160, 160, wkind=_ffi.W_UNIFORM, p0=0.1, p1=10.0, seed=0)
-    opts = ligmg_amd.Options(precision=_ffi.PREC_MIXED, outer_ksp_type=_ffi.KSP_FCG)
+    opts = ligmg_amd.Options(precision=_ffi.PREC_MIXED, outer_ksp_type=_ffi.KSP_FCG,
+                             spmm_tune=0 if args.no_tune else 1)
     dev = torch.device("cuda:0")
     cases = []
     for name, g, k, cp in (("ba48", u, 16, u.comp_ptr), ("grid160", g3, 64, None)):
@@ -179,7 +180,8 @@ def bench_batched(args):
               f"{d['coarse_dense']['tflops']:.3f} TFLOP/s", file=sys.stderr, flush=True)
     _line("batched solve time per right-hand side to 1e-8 (FP32 preconditioner, FP64 FCG)",
           ms_step / nrhs, "ms", False, ms_step, args, "f64 outer / f32 preconditioner",
-          {"workload": "ba48x16rhs_plus_grid160^3x64rhs", "rhs": nrhs}, batches=detail)
+          {"workload": "ba48x16rhs_plus_grid160^3x64rhs", "rhs": nrhs, "spmm_tune": 0 if args.no_tune else 1},
+          batches=detail)
 
 
 def bench_kcycle(args):
@@ -288,6 +290,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-scale", type=int, default=19)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="batched workload: the fixed SpMM tile")
     ap.add_argument("--cpu-only", action="store_true",
                     help="only the CPU oracle leg (no GPU), at --cpu-scale with --cpu-threads")
     ap.add_argument("--cpu-threads", default="",

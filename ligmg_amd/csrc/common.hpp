@@ -166,21 +166,35 @@ __device__ __forceinline__ void wave_lds_sync() {
 constexpr int SEQ_LANE = 256;
 
 // Left fold from 0.0 of term(k), k = a, a+1, ..., b-1, by one wavefront:
-// the exact sequential sum, computed by every lane.  The terms are read 64
-// at a time (coalesced; the next 64 are in flight during the fold), staged
-// in buf (this wavefront's 64-entry LDS slot) and added in index order.
+// the exact sequential sum, computed by every lane.  The terms are read in
+// blocks of SEQ_BLOCK = 512 (8 per lane, coalesced), and the next block's
+// reads are in flight while the current one, staged in buf (this
+// wavefront's SEQ_BLOCK-entry LDS slot), is added in index order.  A block
+// that deep hides the latency of gathered terms: one 64-term chunk at a
+// time left a 257 000-term hub row waiting ~4000 times for a gather.
+constexpr int SEQ_CHUNKS = 8;
+constexpr int SEQ_BLOCK = SEQ_CHUNKS * WAVE;
 template <class Term>
 __device__ __forceinline__ double wave_seq_fold(int64_t a, int64_t b, double *buf, const Term &term) {
   const int lane = threadIdx.x & (WAVE - 1);
-  double nxt = (a + lane < b) ? term(a + lane) : 0.0;
-  double acc = 0.0;
-  for (int64_t base = a; base < b; base += WAVE) {
-    buf[lane] = nxt;
-    wave_lds_sync();
-    nxt = (base + WAVE + lane < b) ? term(base + WAVE + lane) : 0.0;
-    if (b - base >= WAVE) {
+  double nx[SEQ_CHUNKS];
+  auto fetch = [&](int64_t base) {
 #pragma unroll
-      for (int t = 0; t < WAVE; t++) acc += buf[t];
+    for (int c = 0; c < SEQ_CHUNKS; c++) {
+      const int64_t k = base + c * WAVE + lane;
+      nx[c] = (k < b) ? term(k) : 0.0;
+    }
+  };
+  fetch(a);
+  double acc = 0.0;
+  for (int64_t base = a; base < b; base += SEQ_BLOCK) {
+#pragma unroll
+    for (int c = 0; c < SEQ_CHUNKS; c++) buf[c * WAVE + lane] = nx[c];
+    wave_lds_sync();
+    if (base + SEQ_BLOCK < b) fetch(base + SEQ_BLOCK);
+    if (b - base >= SEQ_BLOCK) {
+#pragma unroll 64
+      for (int t = 0; t < SEQ_BLOCK; t++) acc += buf[t];
     } else {
       const int cnt = (int)(b - base);
       for (int t = 0; t < cnt; t++) acc += buf[t];

@@ -11,6 +11,7 @@
 // while any handle captures one (on any thread) the pool takes fresh blocks
 // instead of synchronising the device, which would end that capture.
 #include <atomic>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -120,10 +121,23 @@ void dev_free(void *p) {
   if (it == P.live.end()) return;
   const Pool::Key key = it->second;
   P.live.erase(it);
-  // hipFree synchronises; during a capture the block is parked even above
-  // the cap, which the next free outside a capture enforces again
+  // hipFree synchronises, so above the cap a block goes back to HIP, except
+  // during a capture, when it is parked even above the cap.  The first free
+  // outside a capture that finds the cache above the cap also returns ready
+  // blocks (no queued work reads them) until the cache is under the cap;
+  // blocks still pending follow once an allocation has synchronised them
+  // into the ready lists and a later free finds the cache above the cap.
   if (P.cached + key.second > CACHE_MAX && g_capturing.load() == 0) {
     (void)hipFree(p);
+    for (auto q = P.ready.begin(); q != P.ready.end() && P.cached > CACHE_MAX;) {
+      auto &blocks = q->second;
+      while (!blocks.empty() && P.cached > CACHE_MAX) {
+        (void)hipFree(blocks.back());
+        blocks.pop_back();
+        P.cached -= q->first.second;
+      }
+      q = blocks.empty() ? P.ready.erase(q) : std::next(q);
+    }
     return;
   }
   P.pending[key].push_back(p);

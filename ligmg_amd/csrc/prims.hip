@@ -365,12 +365,12 @@ __global__ void __launch_bounds__(256) k_seg_seq(const uint64_t *keys, const uin
 __global__ void __launch_bounds__(256) k_seg_seq_long(const uint64_t *vals, const int32_t *start,
                                                       int64_t nseg, int64_t n, const int32_t *nlong,
                                                       const int32_t *longs, double *out_sums) {
-  __shared__ double buf[256];
+  __shared__ double buf[4 * SEQ_BLOCK];
   const int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
   if (j >= *nlong) return;
   const int64_t seg = longs[j];
   const int64_t a = start[seg], b = (seg + 1 < nseg) ? start[seg + 1] : n;
-  const double acc = wave_seq_fold(a, b, buf + (threadIdx.x / WAVE) * WAVE,
+  const double acc = wave_seq_fold(a, b, buf + (threadIdx.x / WAVE) * SEQ_BLOCK,
                                    [&](int64_t i) { return __longlong_as_double((long long)vals[i]); });
   if ((threadIdx.x & (WAVE - 1)) == 0) out_sums[seg] = acc;
 }

@@ -69,7 +69,10 @@ __global__ void __launch_bounds__(256) k_rows(int n, const int32_t *__restrict__
   const bool huge = valid && (b - a > CHUNK);
   const bool lng = valid && !huge && (b - a > WAVE);
   Acc acc = op.zero();
+  // unrolled loops: the gathers of several entries are in flight at once;
+  // the adds keep their order
   if (valid && !lng && !huge)
+#pragma unroll 4
     for (int k = a; k < b; k++) op.add(acc, row, k);
   uint64_t mask = __ballot(lng);
   while (mask) {
@@ -77,6 +80,7 @@ __global__ void __launch_bounds__(256) k_rows(int n, const int32_t *__restrict__
     mask &= mask - 1;
     const int la = __shfl(a, L, WAVE), lb = __shfl(b, L, WAVE), lr = __shfl(row, L, WAVE);
     Acc p = op.zero();
+#pragma unroll 4
     for (int k = la + lane; k < lb; k += WAVE) op.add(p, lr, k);
     for (int off = 32; off >= 1; off >>= 1) {
       Acc o = op.shfl_down(p, off);
@@ -104,6 +108,7 @@ __global__ void __launch_bounds__(256) k_rows_chunks(int64_t nchunks, const int3
     if (lane == 0)
       for (int k = la; k < lb; k++) op.add(p, lr, k);
   } else {
+#pragma unroll 4
     for (int k = la + lane; k < lb; k += WAVE) op.add(p, lr, k);
     for (int off = 32; off >= 1; off >>= 1) {
       Acc o = op.shfl_down(p, off);
@@ -248,12 +253,12 @@ __global__ void __launch_bounds__(256) k_rows_seq(int n, const int32_t *__restri
 template <class Op>
 __global__ void __launch_bounds__(256) k_rows_seq_long(int64_t nrows, const int32_t *__restrict__ lrow,
                                                        const int32_t *__restrict__ rp, Op op) {
-  __shared__ double buf[256];
+  __shared__ double buf[4 * SEQ_BLOCK];
   const int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
   if (j >= nrows) return;
   const int row = lrow[j];
   if (row_skip(op, row)) return;  // the whole wavefront: one row per wavefront
-  const double acc = wave_seq_fold(rp[row], rp[row + 1], buf + (threadIdx.x / WAVE) * WAVE, [&](int64_t k) {
+  const double acc = wave_seq_fold(rp[row], rp[row + 1], buf + (threadIdx.x / WAVE) * SEQ_BLOCK, [&](int64_t k) {
     double t = 0.0;
     op.add(t, row, (int)k);
     return t;
