@@ -320,6 +320,13 @@ int ligmg_bench_spmm(ligmg_handle h, int32_t level, int32_t precision,
 int ligmg_bench_coarse(ligmg_handle h, int32_t k, int32_t iters,
                        double *avg_ms, double *flops);
 
+/* (new) Diagnostic: builds the SpMV work units of the row pointers rp
+ * (host array of n + 1 non-decreasing offsets from 0) with the device
+ * builder and with the host loop, and sets *same to 1 when every table
+ * (unit rows, row counts, offsets, long rows, partial slots) agrees, else
+ * 0.  Also reports the unit count. */
+int ligmg_check_units(int64_t n, const int32_t *rp, int32_t *same, int64_t *nunits);
+
 /* z = M r: one cycle of the preconditioner from level `level` down (the
  * V- or K-cycle of multigrid.hpp:86-130 with a zero initial guess) in the
  * given precision.  r and z are host arrays of length n in FP64.  Mirrors

@@ -7,8 +7,13 @@
 
 namespace lg {
 
-// Build the SpMV work units of A from its row pointers (host greedy pass).
+// Build the SpMV work units of A from its row pointers: the greedy packing
+// of complete rows and UNIT_NNZ-chunks, on the device for operators of at
+// least UNITS_DEVICE_MIN_ROWS rows.  build_units_host makes the same tables
+// with a loop on the host.
+constexpr int64_t UNITS_DEVICE_MIN_ROWS = 65536;
 void build_units(DCsr &A, hipStream_t s);
+void build_units_host(DCsr &A, hipStream_t s);
 // Column-sliced copy of A for precision prec (0: FP64, 1: FP32) with
 // slices of `width` columns (width >= ncols: no slicing).
 void build_slices(DCsr &A, int prec, int64_t width, hipStream_t s);

@@ -46,42 +46,74 @@ __global__ void k_order_perm(int64_t n, const uint64_t *vals, const int32_t *rp,
   }
 }
 
-// one wavefront per new row: (new row << cb | new column, old position)
-__global__ void k_order_entries(int64_t n, const int32_t *rp, const int32_t *ci, const int32_t *perm,
-                                const int32_t *iperm, const int32_t *nrp, int cb, uint64_t *keys,
-                                uint64_t *vals) {
+// Entry-parallel passes over a destination CSR (row pointers nrp) filled
+// from a source one: body(r, nrp[r], q) handles destination position q of
+// row r.  Rows of at most HUGE_ROW entries take one wavefront each; longer
+// ones one wavefront per HUGE_ROW-entry piece (a second launch), so a hub
+// row does not hold up the whole pass on one wavefront.
+template <class Body>
+__global__ void k_dst_rows(int64_t n, const int32_t *nrp, Body body) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
   const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
   for (int64_t r = w0; r < n; r += nw) {
-    const int32_t o = perm[r];
-    const int32_t a = rp[o], len = rp[o + 1] - a, d = nrp[r];
-    for (int32_t t = lane; t < len; t += WAVE) {
-      keys[d + t] = ((uint64_t)r << cb) | (uint64_t)iperm[ci[a + t]];
-      vals[d + t] = (uint64_t)(a + t);
-    }
+    const int32_t qa = nrp[r], qb = nrp[r + 1];
+    if (qb - qa > HUGE_ROW) continue;
+    for (int32_t q = qa + lane; q < qb; q += WAVE) body(r, qa, q);
   }
 }
+template <class Body>
+__global__ void k_dst_pieces(int64_t npieces, const int32_t *pstart, const int32_t *pend, const int32_t *prow,
+                             const int32_t *nrp, Body body) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t c = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (c >= npieces) return;
+  const int32_t r = prow[c], d = nrp[r];
+  for (int32_t q = pstart[c] + lane; q < pend[c]; q += WAVE) body(r, d, q);
+}
+
+// (new row << cb | new column, old position)
+struct OrderBody {
+  const int32_t *rp, *ci, *perm, *iperm;
+  int cb;
+  uint64_t *keys, *vals;
+  __device__ void operator()(int64_t r, int32_t d, int32_t q) const {
+    const int32_t src = rp[perm[r]] + (q - d);
+    keys[q] = ((uint64_t)r << cb) | (uint64_t)iperm[ci[src]];
+    vals[q] = (uint64_t)src;
+  }
+};
 
 // Entries of B^T in row order (row x = A row perm[x]): key = new column in
 // the low bits, x in the high 32; value = position of the transposed entry.
 // A stable sort on the low bits then lists every column y (= row y of B,
 // the pattern being symmetric) with its rows x ascending: B's rows, sorted.
-__global__ void k_order_entries_t(int64_t n, const int32_t *rp, const int32_t *ci, const int32_t *perm,
-                                  const int32_t *iperm, const int32_t *tpos, const int32_t *nrp, uint64_t *keys,
-                                  uint64_t *vals) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
-  const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
-  for (int64_t r = w0; r < n; r += nw) {
-    const int32_t o = perm[r];
-    const int32_t a = rp[o], len = rp[o + 1] - a, d = nrp[r];
-    for (int32_t t = lane; t < len; t += WAVE) {
-      keys[d + t] = ((uint64_t)r << 32) | (uint64_t)(uint32_t)iperm[ci[a + t]];
-      vals[d + t] = (uint64_t)(uint32_t)tpos[a + t];
-    }
+struct OrderTBody {
+  const int32_t *rp, *ci, *perm, *iperm, *tpos;
+  uint64_t *keys, *vals;
+  __device__ void operator()(int64_t r, int32_t d, int32_t q) const {
+    const int32_t src = rp[perm[r]] + (q - d);
+    keys[q] = ((uint64_t)r << 32) | (uint64_t)(uint32_t)iperm[ci[src]];
+    vals[q] = (uint64_t)(uint32_t)tpos[src];
   }
-}
+};
+
+// destination row r = source row rperm[r] (null: r), columns relabeled by
+// ciperm (null: unchanged)
+template <class V>
+struct PermBody {
+  const int32_t *rp, *ci;
+  const V *v;
+  const int32_t *rperm, *ciperm;
+  int32_t *nci;
+  V *nv;
+  __device__ void operator()(int64_t r, int32_t d, int32_t q) const {
+    const int32_t src = rp[rperm ? rperm[r] : (int32_t)r] + (q - d);
+    const int32_t c = ci[src];
+    nci[q] = ciperm ? ciperm[c] : c;
+    if (v) nv[q] = v[src];
+  }
+};
 
 __global__ void k_order_unpack_t(int64_t m, const uint64_t *keys, const uint64_t *vals, const double *v,
                                  int32_t *nci, double *nv) {
@@ -102,22 +134,6 @@ __global__ void k_order_unpack(int64_t m, const uint64_t *keys, const uint64_t *
 
 // B row i = A row rperm[i]; entries keep their order, columns relabeled by
 // ciperm (null: unchanged).  One wavefront per row.
-template <class V>
-__global__ void k_perm_rows(int64_t n, const int32_t *rp, const int32_t *ci, const V *v, const int32_t *rperm,
-                            const int32_t *ciperm, const int32_t *nrp, int32_t *nci, V *nv) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
-  for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; r < n; r += nw) {
-    const int32_t o = rperm ? rperm[r] : (int32_t)r;
-    const int32_t a = rp[o], len = rp[o + 1] - a, d = nrp[r];
-    for (int32_t t = lane; t < len; t += WAVE) {
-      const int32_t c = ci[a + t];
-      nci[d + t] = ciperm ? ciperm[c] : c;
-      if (v) nv[d + t] = v[a + t];
-    }
-  }
-}
-
 __global__ void k_perm_len(int64_t n, const int32_t *rp, const int32_t *rperm, int32_t *len) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -171,6 +187,20 @@ void degree_perm(const DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32
   LG_HIP(hipStreamSynchronize(s));
 }
 
+template <class Body>
+void for_dst_entries(int64_t n, const int32_t *nrp, const Body &body, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_dst_rows<Body>, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, nrp, body);
+  LG_LAUNCH_CHECK();
+  HugeRows Hr;
+  find_huge(n, nrp, Hr, s);
+  if (Hr.nchunks) {
+    hipLaunchKernelGGL(k_dst_pieces<Body>, dim3((unsigned)((Hr.nchunks + 3) / 4)), dim3(256), 0, s, Hr.nchunks,
+                       Hr.cstart.p, Hr.cend.p, Hr.crow.p, nrp, body);
+    LG_LAUNCH_CHECK();
+  }
+}
+
 void permute_rows(const DCsr &A, const int32_t *rperm, const int32_t *ciperm, int64_t ncols, DCsr &B,
                   hipStream_t s) {
   const int64_t n = A.n, m = A.nnz;
@@ -186,11 +216,8 @@ void permute_rows(const DCsr &A, const int32_t *rperm, const int32_t *ciperm, in
   LG_CHECK(total == m, LIGMG_ERR_INVALID, "permute_rows: row pointers inconsistent");
   B.ci.alloc(std::max<int64_t>(m, 1));
   if (A.v.p) B.v.alloc(std::max<int64_t>(m, 1));
-  if (m > 0) {
-    hipLaunchKernelGGL(k_perm_rows<double>, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p,
-                       A.ci.p, (const double *)A.v.p, rperm, ciperm, (const int32_t *)B.rp.p, B.ci.p, B.v.p);
-    LG_LAUNCH_CHECK();
-  }
+  if (m > 0)
+    for_dst_entries(n, B.rp.p, PermBody<double>{A.rp.p, A.ci.p, A.v.p, rperm, ciperm, B.ci.p, B.v.p}, s);
   LG_HIP(hipStreamSynchronize(s));
 }
 
@@ -215,12 +242,10 @@ void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, co
   B.nnz = m;
   B.ci.alloc(std::max<int64_t>(m, 1));
   B.v.alloc(std::max<int64_t>(m, 1));
-  if (m > 0 && tpos) {  // one sort by column of B^T's entries (see k_order_entries_t)
+  if (m > 0 && tpos) {  // one sort by column of B^T's entries (see OrderTBody)
     const int cb = std::max(1, bits_for((uint64_t)n));
     DevBuf<uint64_t> keys(m), vals(m);
-    hipLaunchKernelGGL(k_order_entries_t, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p,
-                       A.ci.p, perm, iperm, tpos, B.rp.p, keys.p, vals.p);
-    LG_LAUNCH_CHECK();
+    for_dst_entries(n, B.rp.p, OrderTBody{A.rp.p, A.ci.p, perm, iperm, tpos, keys.p, vals.p}, s);
     radix_sort_pairs(keys.p, vals.p, m, cb, s);
     hipLaunchKernelGGL(k_order_unpack_t, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, m, keys.p, vals.p,
                        A.v.p, B.ci.p, B.v.p);
@@ -228,9 +253,7 @@ void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, co
   } else if (m > 0) {  // a sort by (row, column)
     const int cb = bits_for((uint64_t)n);
     DevBuf<uint64_t> keys(m), vals(m);
-    hipLaunchKernelGGL(k_order_entries, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p,
-                       A.ci.p, perm, iperm, B.rp.p, cb, keys.p, vals.p);
-    LG_LAUNCH_CHECK();
+    for_dst_entries(n, B.rp.p, OrderBody{A.rp.p, A.ci.p, perm, iperm, cb, keys.p, vals.p}, s);
     radix_sort_pairs(keys.p, vals.p, m, 2 * cb, s);
     hipLaunchKernelGGL(k_order_unpack, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, m, keys.p, vals.p,
                        ((uint64_t)1 << cb) - 1, A.v.p, B.ci.p, B.v.p);

@@ -41,7 +41,7 @@ constexpr int MAXT = 16;
 // done cooperatively by the wavefront.  Rows longer than CHUNK are left to
 // k_rows_chunks (one wavefront per 4096-entry chunk) and k_rows_huge.
 // --------------------------------------------------------------------------
-constexpr int CHUNK = 4096;
+constexpr int CHUNK = HUGE_ROW;
 
 // An Op with skip(row) leaves such rows out entirely (no add, no finish):
 // for operators whose results are only read for the other rows.
@@ -55,41 +55,43 @@ __device__ __forceinline__ bool row_skip(const Op &op, int row) {
   else return false;
 }
 
+// Rows of at most WAVE entries, one lane each, summed left to right.
 template <class Op>
 __global__ void __launch_bounds__(256) k_rows(int n, const int32_t *__restrict__ rp, Op op) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n || row_skip(op, row)) return;
+  const int a = rp[row], b = rp[row + 1];
+  if (b - a > WAVE) return;
+  typename Op::Acc acc = op.zero();
+  // unrolled: the gathers of several entries are in flight at once; the
+  // adds keep their order
+#pragma unroll 4
+  for (int k = a; k < b; k++) op.add(acc, row, k);
+  op.finish(row, acc);
+}
+
+// Rows of WAVE + 1 .. CHUNK entries, one wavefront each: 64 strided partial
+// sums and the fixed tree.  One wavefront per row keeps every wavefront's
+// work within CHUNK / WAVE steps wherever the long rows sit (hubs are
+// neighbours in vertex order on R-MAT graphs and in the solve layout).
+template <class Op>
+__global__ void __launch_bounds__(256) k_rows_mid(int64_t nmid, const int32_t *__restrict__ mrow,
+                                                  const int32_t *__restrict__ rp, Op op) {
   using Acc = typename Op::Acc;
   const int lane = threadIdx.x & (WAVE - 1);
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = row < n && !row_skip(op, row);
-  int a = 0, b = 0;
-  if (valid) {
-    a = rp[row];
-    b = rp[row + 1];
-  }
-  const bool huge = valid && (b - a > CHUNK);
-  const bool lng = valid && !huge && (b - a > WAVE);
-  Acc acc = op.zero();
-  // unrolled loops: the gathers of several entries are in flight at once;
-  // the adds keep their order
-  if (valid && !lng && !huge)
+  const int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (j >= nmid) return;  // the whole wavefront
+  const int row = mrow[j];
+  if (row_skip(op, row)) return;
+  const int la = rp[row], lb = rp[row + 1];
+  Acc p = op.zero();
 #pragma unroll 4
-    for (int k = a; k < b; k++) op.add(acc, row, k);
-  uint64_t mask = __ballot(lng);
-  while (mask) {
-    const int L = __ffsll((long long)mask) - 1;
-    mask &= mask - 1;
-    const int la = __shfl(a, L, WAVE), lb = __shfl(b, L, WAVE), lr = __shfl(row, L, WAVE);
-    Acc p = op.zero();
-#pragma unroll 4
-    for (int k = la + lane; k < lb; k += WAVE) op.add(p, lr, k);
-    for (int off = 32; off >= 1; off >>= 1) {
-      Acc o = op.shfl_down(p, off);
-      p = op.merge(p, o);
-    }
-    p = op.shfl0(p);
-    if (lane == L) acc = p;
+  for (int k = la + lane; k < lb; k += WAVE) op.add(p, row, k);
+  for (int off = 32; off >= 1; off >>= 1) {
+    Acc o = op.shfl_down(p, off);
+    p = op.merge(p, o);
   }
-  if (valid && !huge) op.finish(row, acc);
+  if (lane == 0) op.finish(row, p);
 }
 
 // one wavefront per chunk of a huge row: the chunk's canonical sum
@@ -131,21 +133,14 @@ __global__ void k_rows_huge(int64_t nrows, const int32_t *row, const int32_t *cp
 
 }  // namespace
 
-// Rows longer than CHUNK of one matrix and their chunks, and the rows
-// longer than SEQ_LANE (summed by a wavefront each in sequential order).
-struct HugeRows {
-  int64_t nrows = 0, nchunks = 0;
-  DevBuf<int32_t> row, cptr, cstart, cend, crow;
-  DevBuf<uint8_t> cacc;
-  int64_t nlong = 0;
-  DevBuf<int32_t> lrow;
-};
-
 namespace {
 
-__global__ void k_huge_flags(int64_t n, const int32_t *rp, int32_t lim, int32_t *flag) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    flag[i] = (rp[i + 1] - rp[i] > lim) ? 1 : 0;
+// rows with lo < length <= hi
+__global__ void k_huge_flags(int64_t n, const int32_t *rp, int32_t lo, int32_t hi, int32_t *flag) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t len = rp[i + 1] - rp[i];
+    flag[i] = (len > lo && len <= hi) ? 1 : 0;
+  }
 }
 __global__ void k_huge_nchunks(int64_t nh, const int32_t *row, const int32_t *rp, int32_t *cnt) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
@@ -168,22 +163,34 @@ __global__ void k_huge_fill(int64_t nh, const int32_t *row, const int32_t *rp, c
   }
 }
 
+}  // namespace
+
 // Rows longer than CHUNK, in ascending order, and their chunks, built on the
 // device (two counts come back to size the tables).
 void find_huge(int64_t n, const int32_t *rp, HugeRows &Hr, hipStream_t s) {
   Hr.nrows = 0;
   Hr.nchunks = 0;
   Hr.nlong = 0;
+  Hr.nmid = 0;
   if (n <= 0) return;
   DevBuf<int32_t> flag(n), idx(n);
+  constexpr int32_t ANY = 0x7fffffff;
+  hipLaunchKernelGGL(k_huge_flags, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, rp, (int32_t)WAVE,
+                     (int32_t)CHUNK, flag.p);
+  LG_LAUNCH_CHECK();
+  Hr.nmid = compact_flags(flag.p, n, idx.p, s);
+  if (Hr.nmid) {
+    Hr.mrow.alloc(Hr.nmid);
+    d2d(Hr.mrow.p, idx.p, Hr.nmid, s);
+  }
   hipLaunchKernelGGL(k_huge_flags, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, rp, (int32_t)SEQ_LANE,
-                     flag.p);
+                     ANY, flag.p);
   LG_LAUNCH_CHECK();
   Hr.nlong = compact_flags(flag.p, n, idx.p, s);
   if (Hr.nlong == 0) return;  // no row is longer than CHUNK either
   Hr.lrow.alloc(Hr.nlong);
   d2d(Hr.lrow.p, idx.p, Hr.nlong, s);
-  hipLaunchKernelGGL(k_huge_flags, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, rp, (int32_t)CHUNK,
+  hipLaunchKernelGGL(k_huge_flags, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, rp, (int32_t)CHUNK, ANY,
                      flag.p);
   LG_LAUNCH_CHECK();
   const int64_t nh = compact_flags(flag.p, n, idx.p, s);
@@ -215,11 +222,18 @@ const HugeRows &huge_of(const DCsr &A, hipStream_t s) {
   return *A.huge;
 }
 
+namespace {
+
 template <class Op>
 void run_rows(int64_t n, const int32_t *rp, const Op &op, const HugeRows &Hr, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_rows<Op>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int)n, rp, op);
   LG_LAUNCH_CHECK();
+  if (Hr.nmid) {
+    hipLaunchKernelGGL(k_rows_mid<Op>, dim3((unsigned)((Hr.nmid + 3) / 4)), dim3(256), 0, s, Hr.nmid, Hr.mrow.p,
+                       rp, op);
+    LG_LAUNCH_CHECK();
+  }
   if (Hr.nrows) {
     using Acc = typename Op::Acc;
     static_assert(sizeof(Acc) <= 192, "Acc too large for the chunk buffer");
@@ -352,6 +366,15 @@ struct OpSchurCount : IntSum {
   }
 };
 
+__global__ void k_kept_flags(int64_t n, const int32_t *elim, int32_t *kept) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    kept[i] = 1 - elim[i];
+}
+__global__ void k_mask_ids(int64_t n, const int32_t *kept, int32_t *id) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (!kept[i]) id[i] = -1;
+}
+
 // every off-diagonal neighbour of an eliminated vertex must be kept
 __global__ void k_elim_check(int n, const int32_t *rp, const int32_t *ci, const int32_t *newid,
                              int32_t *bad) {
@@ -464,25 +487,24 @@ __global__ void k_unpack_vals32(const uint64_t *vals, int64_t m, float *v32) {
 
 // inverse diagonal for the smoother; isolated rows get 0 (see k_p_fill).
 // One wavefront per row (hub rows have hundreds of thousands of entries).
-__global__ void k_dinv_rows(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
-                            double *dinv, float *dinv32) {
+// The kept-column entries of positions [la, lb) of kept row `row`, in
+// stored order from list position o, by the whole wavefront.
+__device__ __forceinline__ void schur_copy_kept(int la, int lb, int o, int row, const int32_t *ci,
+                                                const double *v, const int32_t *newid, int cb, uint64_t *keys,
+                                                uint64_t *vals) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
-  for (int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; row < n; row += nw) {
-    double d = 0;
-    bool off = false;
-    for (int k = rp[row] + lane; k < rp[row + 1]; k += WAVE) {
-      if (ci[k] == row) d = v[k];
-      else if (v[k] != 0) off = true;
+  const uint64_t r = (uint64_t)newid[row] << cb;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int base = la; base < lb; base += WAVE) {
+    const int k = base + lane;
+    const int col = (k < lb) ? newid[ci[k]] : -1;
+    const uint64_t m = __ballot(col >= 0);
+    if (col >= 0) {
+      const int pos = o + __popcll(m & lt);
+      keys[pos] = r | (uint64_t)col;
+      vals[pos] = (uint64_t)__double_as_longlong(v[k]);
     }
-    // the diagonal sits in exactly one lane (sorted, unique columns)
-    for (int o = WAVE / 2; o >= 1; o >>= 1) d += __shfl_xor(d, o, WAVE);
-    const bool iso = !__any(off);
-    if (lane == 0) {
-      const double r = (d != 0 && !iso) ? 1.0 / d : 0.0;
-      dinv[row] = r;
-      if (dinv32) dinv32[row] = (float)r;
-    }
+    o += __popcll(m);
   }
 }
 
@@ -504,10 +526,12 @@ __global__ void __launch_bounds__(256) k_schur_fill(int n, const int32_t *rp, co
     b = rp[row + 1];
     kept = newid[row] >= 0;
   }
-  // kept rows longer than 16 entries are copied by the whole wavefront
-  // (coalesced); shorter ones and eliminated rows by their own lane
-  const bool lng = valid && kept && (b - a > 16);
-  if (valid && !lng) {
+  // kept rows of 17 .. WAVE entries are copied by the whole wavefront
+  // (coalesced), shorter ones and eliminated rows by their own lane; longer
+  // kept rows are left to k_schur_rows_fill and k_schur_chunk_fill
+  const bool wide = valid && kept && (b - a > WAVE);
+  const bool lng = valid && kept && !wide && (b - a > 16);
+  if (valid && !lng && !wide) {
     int o = off[row];
     if (kept) {
       uint64_t r = (uint64_t)newid[row] << cb;
@@ -538,25 +562,59 @@ __global__ void __launch_bounds__(256) k_schur_fill(int n, const int32_t *rp, co
     }
   }
   uint64_t mask = __ballot(lng);
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   while (mask) {
     const int L = __ffsll((long long)mask) - 1;
     mask &= mask - 1;
     const int la = __shfl(a, L, WAVE), lb = __shfl(b, L, WAVE), lr = __shfl(row, L, WAVE);
-    int o = off[lr];
-    const uint64_t r = (uint64_t)newid[lr] << cb;
-    for (int base = la; base < lb; base += WAVE) {
-      int k = base + lane;
-      int c = (k < lb) ? newid[ci[k]] : -1;
-      uint64_t m = __ballot(c >= 0);
-      if (c >= 0) {
-        int pos = o + __popcll(m & lt);
-        keys[pos] = r | (uint64_t)c;
-        vals[pos] = (uint64_t)__double_as_longlong(v[k]);
-      }
-      o += __popcll(m);
-    }
+    schur_copy_kept(la, lb, off[lr], lr, ci, v, newid, cb, keys, vals);
   }
+}
+
+// Kept entries of each CHUNK-entry chunk of a hub row, one wavefront per
+// chunk.  Hub rows belong to kept vertices (an eliminated vertex has a few
+// neighbours at most); a chunk of any other row counts 0.
+__global__ void __launch_bounds__(256) k_schur_chunk_count(int64_t nchunks, const int32_t *cstart,
+                                                           const int32_t *cend, const int32_t *crow,
+                                                           const int32_t *ci, const int32_t *newid,
+                                                           int32_t *ccnt) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t c = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (c >= nchunks) return;  // the whole wavefront
+  int cnt = 0;
+  if (newid[crow[c]] >= 0)
+    for (int k = cstart[c] + lane; k < cend[c]; k += WAVE) cnt += (newid[ci[k]] >= 0) ? 1 : 0;
+  for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, WAVE);
+  if (lane == 0) ccnt[c] = cnt;
+}
+
+// Kept rows of WAVE + 1 .. CHUNK entries (mrow), one wavefront each.
+__global__ void __launch_bounds__(256) k_schur_rows_fill(int64_t nmid, const int32_t *mrow, const int32_t *rp,
+                                                         const int32_t *ci, const double *v,
+                                                         const int32_t *newid, const int32_t *off, int cb,
+                                                         uint64_t *keys, uint64_t *vals) {
+  const int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (j >= nmid) return;  // the whole wavefront
+  const int32_t row = mrow[j];
+  if (newid[row] < 0) return;
+  schur_copy_kept(rp[row], rp[row + 1], off[row], row, ci, v, newid, cb, keys, vals);
+}
+
+// One chunk of a hub row per wavefront, at the row's list offset plus the
+// kept entries of the row's earlier chunks (cpos: exclusive scan of the
+// chunk counts; the row's first chunk sits (cstart - rp[row]) / CHUNK
+// chunks back).
+__global__ void __launch_bounds__(256) k_schur_chunk_fill(int64_t nchunks, const int32_t *cstart,
+                                                          const int32_t *cend, const int32_t *crow,
+                                                          const int32_t *rp, const int32_t *ci,
+                                                          const double *v, const int32_t *newid,
+                                                          const int32_t *off, const int32_t *cpos, int cb,
+                                                          uint64_t *keys, uint64_t *vals) {
+  const int64_t c = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (c >= nchunks) return;  // the whole wavefront
+  const int32_t row = crow[c], la = cstart[c];
+  if (newid[row] < 0) return;
+  const int64_t c0 = c - (la - rp[row]) / CHUNK;
+  schur_copy_kept(la, cend[c], off[row] + (cpos[c] - cpos[c0]), row, ci, v, newid, cb, keys, vals);
 }
 
 // CSR from sorted unique keys: rows counted, columns = low bits.
@@ -668,9 +726,22 @@ struct OpJacobiTV {
   __device__ void add(Acc &acc, int, int k) const {
     const double a = v[k];
     const double *xj = X + (int64_t)ci[k] * TS;
+    if (TS & 1) {
 #pragma unroll
-    for (int t = 0; t < N; t++)
-      if (t < T) acc.v[t] += (a < 0) ? a * xj[t] : 0.0;
+      for (int t = 0; t < N; t++)
+        if (t < T) acc.v[t] += (a < 0) ? a * xj[t] : 0.0;
+    } else {
+      // an even row stride keeps every row 16-byte aligned: the row is read
+      // two vectors per load (the odd half of a last pair may be padding
+      // inside the row, never used); the products and sums are unchanged
+#pragma unroll
+      for (int t = 0; t < N; t += 2)
+        if (t < T) {
+          const double2 p = *(const double2 *)(xj + t);
+          acc.v[t] += (a < 0) ? a * p.x : 0.0;
+          if (t + 1 < N && t + 1 < T) acc.v[t + 1] += (a < 0) ? a * p.y : 0.0;
+        }
+    }
   }
   __device__ void finish(int row, const Acc &acc) const {
     const double w = 2.0 / 3.0;
@@ -716,7 +787,17 @@ struct OpAffinity : DblMax {
     } else if (metric == LIGMG_STRENGTH_AFFINITY) {
       const double *xc = X + (int64_t)c * TS, *xr = X + (int64_t)row * TS;
       double top = 0;
-      for (int t = 0; t < T; t++) top += xc[t] * xr[t];
+      if (TS & 1) {
+        for (int t = 0; t < T; t++) top += xc[t] * xr[t];
+      } else {  // 16-byte rows reads, as in OpJacobiTV; same order of sums
+#pragma unroll
+        for (int t = 0; t < MAXT; t += 2)
+          if (t < T) {
+            const double2 a = *(const double2 *)(xc + t), b = *(const double2 *)(xr + t);
+            top += a.x * b.x;
+            if (t + 1 < T) top += a.y * b.y;
+          }
+      }
       r = top * top / (nrm[c] * nrm[c] * nrm[row] * nrm[row]);
     } else if (metric == LIGMG_STRENGTH_ALGEBRAIC_DISTANCE) {
       const double *xc = X + (int64_t)c * TS, *xr = X + (int64_t)row * TS;
@@ -1171,22 +1252,14 @@ bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &c
   DevBuf<int32_t> deg(n), elim(n);
   run_rows(n, A.rp.p, OpDeg{{}, A.v.p, deg.p}, Hr, s);
   run_rows(n, A.rp.p, OpElimMin{A.ci.p, deg.p, A.v.p, o.elim_degree, elim.p}, Hr, s);
-  // kept flags -> new ids
+  // kept flags -> new ids (-1 for an eliminated vertex)
   DevBuf<int32_t> kept(n), newid(n);
-  {
-    // kept = 1 - elim
-    std::vector<int32_t> e = download(elim.p, n, s);
-    for (auto &x : e) x = 1 - x;
-    to_device(kept.p, e.data(), n, s);
-  }
+  hipLaunchKernelGGL(k_kept_flags, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, elim.p, kept.p);
+  LG_LAUNCH_CHECK();
   int64_t n_kept = exclusive_scan_i32(kept.p, newid.p, n, s);
   if ((double)n_kept > (double)n * o.elim_factor) return false;  // elimination.hpp:226-228
-  {
-    std::vector<int32_t> k = download(kept.p, n, s), id = download(newid.p, n, s);
-    for (int64_t i = 0; i < n; i++)
-      if (!k[i]) id[i] = -1;
-    to_device(newid.p, id.data(), n, s);
-  }
+  hipLaunchKernelGGL(k_mask_ids, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, kept.p, newid.p);
+  LG_LAUNCH_CHECK();
   {
     DevBuf<int32_t> bad(1);
     fill_i32(bad.p, 0, 1, s);
@@ -1233,6 +1306,24 @@ bool setup_elimination(const DCsr &A, const ligmg_options &o, DLevel &L, DCsr &c
   hipLaunchKernelGGL(k_schur_fill, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, A.ci.p, A.v.p,
                      L.new_id.p, cnt.p, L.P.rp.p, L.P.v.p, cb, keys.p, vals.p);
   LG_LAUNCH_CHECK();
+  if (Hr.nmid) {
+    hipLaunchKernelGGL(k_schur_rows_fill, dim3((unsigned)((Hr.nmid + 3) / 4)), dim3(256), 0, s, Hr.nmid,
+                       Hr.mrow.p, A.rp.p, A.ci.p, A.v.p, L.new_id.p, cnt.p, cb, keys.p, vals.p);
+    LG_LAUNCH_CHECK();
+  }
+  if (Hr.nchunks) {
+    // hub rows: one wavefront per chunk
+    DevBuf<int32_t> ccnt(Hr.nchunks + 1), cpos(Hr.nchunks + 1);
+    const unsigned gw = (unsigned)((Hr.nchunks + 3) / 4);
+    hipLaunchKernelGGL(k_schur_chunk_count, dim3(gw), dim3(256), 0, s, Hr.nchunks, Hr.cstart.p, Hr.cend.p,
+                       Hr.crow.p, A.ci.p, L.new_id.p, ccnt.p);
+    LG_LAUNCH_CHECK();
+    fill_i32(ccnt.p + Hr.nchunks, 0, 1, s);
+    exclusive_scan_i32(ccnt.p, cpos.p, Hr.nchunks + 1, s);
+    hipLaunchKernelGGL(k_schur_chunk_fill, dim3(gw), dim3(256), 0, s, Hr.nchunks, Hr.cstart.p, Hr.cend.p,
+                       Hr.crow.p, A.rp.p, A.ci.p, A.v.p, L.new_id.p, cnt.p, cpos.p, cb, keys.p, vals.p);
+    LG_LAUNCH_CHECK();
+  }
   contract_to_csr(keys, vals, m, n_kept, n_kept, coarse, s);
   return true;
 }
@@ -1267,12 +1358,16 @@ void make_elim32(DLevel &L, hipStream_t s) {
 // Strength of connection (affinity_matrix + col_sum)
 // ----------------------------------------------------------------------------
 namespace {
-__global__ void k_div_rows(int n, const int32_t *rp, const double *rs, double *v) {
-  int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n) return;
-  const double d = rs[row];
-  for (int k = rp[row]; k < rp[row + 1]; k++) v[k] = d > 0 ? v[k] / d : 0.0;
-}
+// v[k] / (its row's sum), 0 where the sum is not positive
+struct OpDivRows : NoAcc {
+  const double *rs;
+  double *v;
+  __device__ void add(int &, int row, int k) const {
+    const double d = rs[row];
+    v[k] = d > 0 ? v[k] / d : 0.0;
+  }
+  __device__ void finish(int, int) const {}
+};
 __global__ void k_div_cols(int64_t nnz, const int32_t *ci, const double *cs, double *v) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
     const double d = cs[ci[k]];
@@ -1360,8 +1455,7 @@ void compute_strength(const DCsr &A, const ligmg_options &o, int level_idx, DevB
   if (sc == LIGMG_AFF_ROW || sc == LIGMG_AFF_ROWCOLUMN || sc == LIGMG_AFF_SYM) {
     DevBuf<double> rs(n);
     run_rows_seq(n, A.rp.p, OpRowSum{{}, raw.p, rs.p}, Hr, s);
-    hipLaunchKernelGGL(k_div_rows, dim3(g1(n)), dim3(256), 0, s, (int)n, A.rp.p, rs.p, raw.p);
-    LG_LAUNCH_CHECK();
+    run_rows(n, A.rp.p, OpDivRows{{}, rs.p, raw.p}, Hr, s);
   }
   if (sc == LIGMG_AFF_COLUMN || sc == LIGMG_AFF_ROWCOLUMN) {
     DevBuf<double> csum(n);
@@ -1496,13 +1590,69 @@ void setup_aggregation(const DCsr &A, const ligmg_options &o, int level_idx, DLe
 }
 
 // Inverse diagonal of A (0 where the diagonal is missing or zero).
+namespace {
+// 1/diag per row, 0 for an isolated vertex (no nonzero off-diagonal) or a
+// zero diagonal: the row machinery spreads hub rows over many wavefronts.
+// The diagonal sits at one position, so the merged sum is that value.
+struct DiagAcc {
+  double d;
+  int off;
+};
+struct OpDinv {
+  using Acc = DiagAcc;
+  const int32_t *ci;
+  const double *v;
+  double *dinv;
+  float *dinv32;
+  __device__ Acc zero() const { return {0.0, 0}; }
+  __device__ Acc merge(const Acc &a, const Acc &b) const { return {a.d + b.d, a.off | b.off}; }
+  __device__ Acc shfl_down(const Acc &a, int off) const {
+    return {__shfl_down(a.d, off, WAVE), __shfl_down(a.off, off, WAVE)};
+  }
+  __device__ Acc shfl0(const Acc &a) const { return {__shfl(a.d, 0, WAVE), __shfl(a.off, 0, WAVE)}; }
+  __device__ void add(Acc &acc, int row, int k) const {
+    if (ci[k] == row) acc.d = v[k];
+    else if (v[k] != 0) acc.off = 1;
+  }
+  __device__ void finish(int row, const Acc &acc) const {
+    const double r = (acc.d != 0 && acc.off) ? 1.0 / acc.d : 0.0;
+    dinv[row] = r;
+    if (dinv32) dinv32[row] = (float)r;
+  }
+};
+
+// FP32 Laplacian form: off-diagonals rounded to FP32 (the diagonal slot 0),
+// the diagonal -(sum of the rounded off-diagonals) in FP64, so the FP32
+// operator keeps the constant nullspace exactly.  Canonical row sums.
+struct OpLap32 : DblSum {
+  const int32_t *ci;
+  const double *v;
+  float *v32;
+  double *d32;
+  __device__ void add(double &acc, int row, int k) const {
+    if (ci[k] == row) {
+      v32[k] = 0.0f;
+    } else {
+      const float f = (float)v[k];
+      v32[k] = f;
+      acc += (double)f;
+    }
+  }
+  __device__ void finish(int row, double acc) const { d32[row] = -acc; }
+};
+}  // namespace
+
 void setup_dinv(const DCsr &A, DLevel &L, bool fp32, hipStream_t s) {
   const int64_t n = A.n;
   L.dinv.alloc(n);
   if (fp32) L.dinv32.alloc(n);
-  hipLaunchKernelGGL(k_dinv_rows, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, s, n, A.rp.p, A.ci.p,
-                     A.v.p, L.dinv.p, fp32 ? L.dinv32.p : nullptr);
-  LG_LAUNCH_CHECK();
+  run_rows(n, A.rp.p, OpDinv{A.ci.p, A.v.p, L.dinv.p, fp32 ? L.dinv32.p : nullptr}, huge_of(A, s), s);
+}
+
+void lap32_form(DCsr &A, hipStream_t s) {
+  A.v32.alloc(std::max<int64_t>(A.nnz, 1));
+  A.d32.alloc(std::max<int64_t>(A.n, 1));
+  run_rows(A.n, A.rp.p, OpLap32{{}, A.ci.p, A.v.p, A.v32.p, A.d32.p}, huge_of(A, s), s);
 }
 
 // ----------------------------------------------------------------------------

@@ -8,7 +8,29 @@ namespace lg {
 void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, int64_t nrows,
                      int64_t ncols, DCsr &out, hipStream_t s);
 
-// out[i] = sum of row i's stored values in the canonical order.
+// Row classes of one matrix for the row kernels: rows longer than HUGE_ROW
+// entries and their HUGE_ROW-entry chunks (chunk c covers [cstart[c],
+// cend[c]) of row crow[c]; a row's chunks are cptr[j] .. cptr[j+1]-1), the
+// rows of WAVE + 1 .. HUGE_ROW entries (a wavefront each, canonical sums)
+// and the rows longer than SEQ_LANE (a wavefront each, sequential sums).
+// find_huge builds it from row pointers (the counts come back to the host);
+// huge_of caches it per matrix.
+constexpr int HUGE_ROW = 4096;
+struct HugeRows {
+  int64_t nrows = 0, nchunks = 0;
+  DevBuf<int32_t> row, cptr, cstart, cend, crow;
+  DevBuf<uint8_t> cacc;
+  int64_t nlong = 0;  // rows of more than SEQ_LANE entries (sequential sums)
+  DevBuf<int32_t> lrow;
+  int64_t nmid = 0;  // rows of WAVE + 1 .. HUGE_ROW entries (canonical sums)
+  DevBuf<int32_t> mrow;
+};
+void find_huge(int64_t n, const int32_t *rp, HugeRows &Hr, hipStream_t s);
+const HugeRows &huge_of(const DCsr &A, hipStream_t s);
+// the FP32 Laplacian form of A's values (v32: rounded off-diagonals, 0 on
+// the diagonal; d32: -(sum of the rounded off-diagonals) in FP64)
+void lap32_form(DCsr &A, hipStream_t s);
+
 void row_sums_seq(int64_t n, const int32_t *rp, const double *v, double *out, hipStream_t s);
 // Throws LIGMG_ERR_INVALID unless rp[0] = 0, rp is non-decreasing, every
 // column is in [0, ncols) and, with sorted, every row's columns strictly

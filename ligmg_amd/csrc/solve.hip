@@ -5,6 +5,7 @@
 
 #include "ops.hpp"
 #include "prims.hpp"
+#include "setup.hpp"
 #include "solve_dev.hpp"
 
 namespace lg {
@@ -241,26 +242,6 @@ __global__ void k_prolong_add(const int32_t *agg, int64_t n, const T *xc, T *x) 
 
 // Laplacian-form FP32 copy: off-diagonals rounded, diagonal slot 0, and the
 // FP64 diagonal = -(sum of the rounded off-diagonals); one wavefront per row
-__global__ void k_lap32(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
-                        float *v32, double *d32) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t nw = (int64_t)gridDim.x * blockDim.x / WAVE;
-  for (int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; row < n; row += nw) {
-    double s = 0;
-    for (int k = rp[row] + lane; k < rp[row + 1]; k += WAVE) {
-      if (ci[k] == row) {
-        v32[k] = 0.0f;
-      } else {
-        float f = (float)v[k];
-        v32[k] = f;
-        s += (double)f;
-      }
-    }
-    s = wave_sum(s);
-    if (lane == 0) d32[row] = -s;
-  }
-}
-
 __global__ void k_d2f(const double *a, float *b, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -434,40 +415,96 @@ __global__ void __launch_bounds__(256) k_sub_mean(T *a, const int64_t *lo, const
 
 // ---------------------------------------------------------------- host side
 
-void build_units(DCsr &A, hipStream_t s) {
-  const int64_t n = A.n;
-  std::vector<int32_t> rp = download(A.rp.p, n + 1, s);
-  std::vector<int32_t> row0, nrows, koff, lrow, lpart;
-  int64_t r = 0, parts = 0;
-  while (r < n) {
-    int64_t len = rp[r + 1] - rp[r];
-    if (len > UNIT_NNZ) {
-      lrow.push_back((int32_t)r);
-      lpart.push_back((int32_t)parts);
-      for (int64_t k = rp[r]; k < rp[r + 1]; k += UNIT_NNZ) {
-        row0.push_back((int32_t)r);
-        nrows.push_back(-(int32_t)parts);
-        koff.push_back((int32_t)k);
-        parts++;
-      }
-      r++;
-      continue;
-    }
-    int64_t r1 = r, acc = 0;
-    while (r1 < n && r1 - r < UNIT_ROWS && (rp[r1 + 1] - rp[r1]) <= UNIT_NNZ &&
-           acc + (rp[r1 + 1] - rp[r1]) <= UNIT_NNZ) {
-      acc += rp[r1 + 1] - rp[r1];
-      r1++;
-    }
-    row0.push_back((int32_t)r);
-    nrows.push_back((int32_t)(r1 - r));
-    koff.push_back(rp[r]);
-    r = r1;
+namespace {
+
+// The greedy packing walks the rows from 0: a unit starting at a row of at
+// most UNIT_NNZ entries takes it and the following rows while there are at
+// most UNIT_ROWS of them and their entries fit in UNIT_NNZ; a longer row is
+// cut into UNIT_NNZ-chunks.  unit_next(r) is the row after the unit that
+// starts at r.  rp is non-decreasing, so "rows r..r1-1 fit" is monotone in
+// r1 and a binary search finds the largest r1 (any row longer than UNIT_NNZ
+// already breaks the fit, so it needs no test of its own).
+__host__ __device__ inline int64_t unit_next(const int32_t *rp, int64_t n, int64_t r) {
+  const int32_t base = rp[r];
+  if (rp[r + 1] - base > UNIT_NNZ) return r + 1;
+  int64_t lo = r + 1, hi = (n < r + UNIT_ROWS) ? n : r + UNIT_ROWS;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (rp[mid] - base <= UNIT_NNZ) lo = mid;
+    else hi = mid - 1;
   }
-  lpart.push_back((int32_t)parts);
-  Units &U = A.U;
+  return lo;
+}
+
+// jump table 0: next[r] for r < n, next[n] = n
+__global__ void k_unit_next(int64_t n, const int32_t *rp, int32_t *J) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r <= n; r += (int64_t)gridDim.x * blockDim.x)
+    J[r] = (int32_t)(r < n ? unit_next(rp, n, r) : n);
+}
+// table k+1 = table k applied twice
+__global__ void k_unit_jump(int64_t n, const int32_t *Jk, int32_t *Jk1) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r <= n; r += (int64_t)gridDim.x * blockDim.x)
+    Jk1[r] = Jk[Jk[r]];
+}
+// number of unit starts on the chain from row 0 (n > 0): the largest number
+// of steps that stays below n, plus the start itself
+__global__ void k_unit_chain_len(int64_t n, const int32_t *J, int K, int64_t stride, int64_t *out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t pos = 0, m = 0;
+  for (int k = K - 1; k >= 0; k--) {
+    const int32_t q = J[(int64_t)k * stride + pos];
+    if (q < n) {
+      pos = q;
+      m += (int64_t)1 << k;
+    }
+  }
+  out[0] = m + 1;
+}
+// start m of the chain, its unit count and, for a long row, its chunk count
+__global__ void k_unit_chain(int64_t M, const int32_t *J, int K, int64_t stride, const int32_t *rp,
+                             int32_t *start, int32_t *ucnt, int32_t *pcnt, int32_t *islong) {
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < M; m += (int64_t)gridDim.x * blockDim.x) {
+    int64_t pos = 0;
+    for (int k = 0; k < K; k++)
+      if ((m >> k) & 1) pos = J[(int64_t)k * stride + pos];
+    const int32_t len = rp[pos + 1] - rp[pos];
+    const bool lng = len > UNIT_NNZ;
+    const int32_t chunks = (len + UNIT_NNZ - 1) / UNIT_NNZ;
+    start[m] = (int32_t)pos;
+    ucnt[m] = lng ? chunks : 1;
+    pcnt[m] = lng ? chunks : 0;
+    islong[m] = lng ? 1 : 0;
+  }
+}
+__global__ void k_unit_fill(int64_t M, const int32_t *start, const int32_t *J0, const int32_t *rp,
+                            const int32_t *uoff, const int32_t *poff, const int32_t *loff, const int32_t *islong,
+                            int32_t *row0, int32_t *nrows, int32_t *koff, int32_t *lrow, int32_t *lpart) {
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < M; m += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t r = start[m], u = uoff[m];
+    if (islong[m]) {
+      const int32_t a = rp[r], b = rp[r + 1];
+      int32_t j = 0;
+      for (int32_t k = a; k < b; k += UNIT_NNZ, j++) {
+        row0[u + j] = r;
+        nrows[u + j] = -(poff[m] + j);
+        koff[u + j] = k;
+      }
+      lrow[loff[m]] = r;
+      lpart[loff[m]] = poff[m];
+    } else {
+      row0[u] = r;
+      nrows[u] = J0[r] - r;
+      koff[u] = rp[r];
+    }
+  }
+}
+
+// the unit tables from host vectors
+void set_units(Units &U, int64_t nnz, const std::vector<int32_t> &row0, const std::vector<int32_t> &nrows,
+               const std::vector<int32_t> &koff, const std::vector<int32_t> &lrow,
+               const std::vector<int32_t> &lpart, int64_t parts, hipStream_t s) {
   U.nunits = (int64_t)row0.size();
-  U.row_items = row_items_for(rp[n] - rp[0]);
+  U.row_items = row_items_for(nnz);
   U.nlong = (int64_t)lrow.size();
   U.nparts = parts;
   U.row0.alloc(U.nunits);
@@ -481,6 +518,96 @@ void build_units(DCsr &A, hipStream_t s) {
   to_device(U.lrow.p, lrow.data(), U.nlong, s);
   to_device(U.lpart.p, lpart.data(), U.nlong + 1, s);
   U.parts.alloc(std::max<int64_t>(parts, 1));
+  LG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+// The greedy packing on the host, row by row.
+void build_units_host(DCsr &A, hipStream_t s) {
+  const int64_t n = A.n;
+  std::vector<int32_t> rp = download(A.rp.p, n + 1, s);
+  std::vector<int32_t> row0, nrows, koff, lrow, lpart;
+  int64_t r = 0, parts = 0;
+  while (r < n) {
+    const int64_t r1 = unit_next(rp.data(), n, r);
+    if (rp[r + 1] - rp[r] > UNIT_NNZ) {
+      lrow.push_back((int32_t)r);
+      lpart.push_back((int32_t)parts);
+      for (int64_t k = rp[r]; k < rp[r + 1]; k += UNIT_NNZ) {
+        row0.push_back((int32_t)r);
+        nrows.push_back(-(int32_t)parts);
+        koff.push_back((int32_t)k);
+        parts++;
+      }
+    } else {
+      row0.push_back((int32_t)r);
+      nrows.push_back((int32_t)(r1 - r));
+      koff.push_back(rp[r]);
+    }
+    r = r1;
+  }
+  lpart.push_back((int32_t)parts);
+  set_units(A.U, rp[n] - rp[0], row0, nrows, koff, lrow, lpart, parts, s);
+}
+
+// The same packing on the device.  The unit starts are the chain 0,
+// next(0), next(next(0)), ... below n; binary lifting (tables of next applied
+// 2^k times) gives the m-th start directly, so every start is found in
+// parallel.  Small operators take the host pass, whose download and loop
+// cost less than the launches.
+void build_units(DCsr &A, hipStream_t s) {
+  const int64_t n = A.n;
+  if (n < UNITS_DEVICE_MIN_ROWS) {
+    build_units_host(A, s);
+    return;
+  }
+  int K = 1;
+  while (((int64_t)1 << K) < n + 1) K++;
+  const int64_t stride = n + 1;
+  DevBuf<int32_t> J((size_t)K * stride);
+  hipLaunchKernelGGL(k_unit_next, dim3(grid_for(stride, 256, 16384)), dim3(256), 0, s, n, A.rp.p, J.p);
+  LG_LAUNCH_CHECK();
+  for (int k = 0; k + 1 < K; k++) {
+    hipLaunchKernelGGL(k_unit_jump, dim3(grid_for(stride, 256, 16384)), dim3(256), 0, s, n, J.p + k * stride,
+                       J.p + (k + 1) * stride);
+    LG_LAUNCH_CHECK();
+  }
+  DevBuf<int64_t> Md(1);
+  hipLaunchKernelGGL(k_unit_chain_len, dim3(1), dim3(64), 0, s, n, J.p, K, stride, Md.p);
+  LG_LAUNCH_CHECK();
+  int64_t M = 0;
+  to_host(&M, Md.p, 1, s);
+  LG_HIP(hipStreamSynchronize(s));
+  DevBuf<int32_t> start(M), ucnt(M + 1), pcnt(M + 1), islong(M + 1), uoff(M + 1), poff(M + 1), loff(M + 1);
+  hipLaunchKernelGGL(k_unit_chain, dim3(grid_for(M, 256, 16384)), dim3(256), 0, s, M, J.p, K, stride, A.rp.p,
+                     start.p, ucnt.p, pcnt.p, islong.p);
+  LG_LAUNCH_CHECK();
+  fill_i32(ucnt.p + M, 0, 1, s);
+  fill_i32(pcnt.p + M, 0, 1, s);
+  fill_i32(islong.p + M, 0, 1, s);
+  const int64_t nunits = exclusive_scan_i32(ucnt.p, uoff.p, M + 1, s);
+  const int64_t parts = exclusive_scan_i32(pcnt.p, poff.p, M + 1, s);
+  const int64_t nlong = exclusive_scan_i32(islong.p, loff.p, M + 1, s);
+  Units &U = A.U;
+  int32_t ends[2];
+  to_host(&ends[0], A.rp.p, 1, s);
+  to_host(&ends[1], A.rp.p + n, 1, s);
+  LG_HIP(hipStreamSynchronize(s));
+  U.nunits = nunits;
+  U.row_items = row_items_for((int64_t)ends[1] - ends[0]);
+  U.nlong = nlong;
+  U.nparts = parts;
+  U.row0.alloc(nunits);
+  U.nrows.alloc(nunits);
+  U.koff.alloc(nunits);
+  U.lrow.alloc(std::max<int64_t>(nlong, 1));
+  U.lpart.alloc(nlong + 1);
+  U.parts.alloc(std::max<int64_t>(parts, 1));
+  hipLaunchKernelGGL(k_unit_fill, dim3(grid_for(M, 256, 16384)), dim3(256), 0, s, M, start.p, J.p, A.rp.p,
+                     uoff.p, poff.p, loff.p, islong.p, U.row0.p, U.nrows.p, U.koff.p, U.lrow.p, U.lpart.p);
+  LG_LAUNCH_CHECK();
+  fill_i32(U.lpart.p + nlong, (int32_t)parts, 1, s);
   LG_HIP(hipStreamSynchronize(s));
 }
 
@@ -586,13 +713,10 @@ double time_spmv(const DCsr &A, int prec, int iters, hipStream_t s) {
 
 void make_v32(DCsr &A, hipStream_t s, bool laplacian) {
   if (!A.v.p) return;
-  A.v32.alloc(std::max<int64_t>(A.nnz, 1));
   if (laplacian) {
-    A.d32.alloc(std::max<int64_t>(A.n, 1));
-    hipLaunchKernelGGL(k_lap32, dim3(grid_for(A.n * WAVE, 256)), dim3(256), 0, s, A.n, A.rp.p, A.ci.p,
-                       A.v.p, A.v32.p, A.d32.p);
-    LG_LAUNCH_CHECK();
+    lap32_form(A, s);
   } else {
+    A.v32.alloc(std::max<int64_t>(A.nnz, 1));
     op_d2f(A.v.p, A.v32.p, A.nnz, s);
   }
 }

@@ -358,6 +358,11 @@ struct Solver {
     for (size_t l = 0; l + 1 < H.size(); l++) {
       const DLevel &L = H[l];
       std::vector<int64_t> nb(L.blocks.size(), 0);
+      if (L.blocks.size() == 2) {  // one block: all of the next level's vertices
+        nb[1] = H[l + 1].A.n;
+        H[l + 1].blocks = nb;
+        continue;
+      }
       std::vector<int32_t> keep;
       if (L.kind == LIGMG_LEVEL_ELIM) keep = download(L.new_id.p, L.n, s);
       else {
@@ -2786,6 +2791,38 @@ int ligmg_assemble_laplacian(int64_t n, const int32_t *rp, const int32_t *ci, co
                        w, v);
     LG_LAUNCH_CHECK();
     LG_HIP(hipStreamSynchronize(0));
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_check_units(int64_t n, const int32_t *rp, int32_t *same, int64_t *nunits) {
+  if (n <= 0 || !rp || !same || !nunits) return LIGMG_ERR_INVALID;
+  try {
+    LG_CHECK(rp[0] == 0, LIGMG_ERR_INVALID, "ligmg_check_units: rp[0] must be 0");
+    for (int64_t i = 0; i < n; i++)
+      LG_CHECK(rp[i + 1] >= rp[i], LIGMG_ERR_INVALID, "ligmg_check_units: rp must be non-decreasing");
+    hipStream_t s = 0;
+    lg::DCsr A, B;
+    A.n = B.n = n;
+    A.rp.alloc(n + 1);
+    B.rp.alloc(n + 1);
+    lg::to_device(A.rp.p, rp, n + 1, s);
+    lg::to_device(B.rp.p, rp, n + 1, s);
+    lg::build_units(A, s);
+    lg::build_units_host(B, s);
+    const lg::Units &a = A.U, &b = B.U;
+    bool eq = a.nunits == b.nunits && a.nlong == b.nlong && a.nparts == b.nparts && a.row_items == b.row_items;
+    if (eq) {
+      eq = lg::download(a.row0.p, a.nunits, s) == lg::download(b.row0.p, b.nunits, s) &&
+           lg::download(a.nrows.p, a.nunits, s) == lg::download(b.nrows.p, b.nunits, s) &&
+           lg::download(a.koff.p, a.nunits, s) == lg::download(b.koff.p, b.nunits, s) &&
+           lg::download(a.lrow.p, a.nlong, s) == lg::download(b.lrow.p, b.nlong, s) &&
+           lg::download(a.lpart.p, a.nlong + 1, s) == lg::download(b.lpart.p, b.nlong + 1, s);
+    }
+    *same = eq ? 1 : 0;
+    *nunits = a.nunits;
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return fail(e);
