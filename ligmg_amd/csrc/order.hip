@@ -180,7 +180,7 @@ void degree_perm(const DCsr &A, const std::vector<int64_t> &blocks, DevBuf<int32
   hipLaunchKernelGGL(k_order_keys, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, A.rp.p, bptr.p, nb, keys.p,
                      vals.p);
   LG_LAUNCH_CHECK();
-  radix_sort_pairs(keys.p, vals.p, n, 32 + bits_for((uint64_t)nb), s);
+  radix_sort_pairs(keys, vals, n, 32 + bits_for((uint64_t)nb), s);
   hipLaunchKernelGGL(k_order_perm, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, n, vals.p, A.rp.p, perm.p,
                      iperm.p, len.p);
   LG_LAUNCH_CHECK();
@@ -246,7 +246,7 @@ void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, co
     const int cb = std::max(1, bits_for((uint64_t)n));
     DevBuf<uint64_t> keys(m), vals(m);
     for_dst_entries(n, B.rp.p, OrderTBody{A.rp.p, A.ci.p, perm, iperm, tpos, keys.p, vals.p}, s);
-    radix_sort_pairs(keys.p, vals.p, m, cb, s);
+    radix_sort_pairs(keys, vals, m, cb, s);
     hipLaunchKernelGGL(k_order_unpack_t, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, m, keys.p, vals.p,
                        A.v.p, B.ci.p, B.v.p);
     LG_LAUNCH_CHECK();
@@ -254,7 +254,7 @@ void permute_sorted(const DCsr &A, const int32_t *perm, const int32_t *iperm, co
     const int cb = bits_for((uint64_t)n);
     DevBuf<uint64_t> keys(m), vals(m);
     for_dst_entries(n, B.rp.p, OrderBody{A.rp.p, A.ci.p, perm, iperm, cb, keys.p, vals.p}, s);
-    radix_sort_pairs(keys.p, vals.p, m, 2 * cb, s);
+    radix_sort_pairs(keys, vals, m, 2 * cb, s);
     hipLaunchKernelGGL(k_order_unpack, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, m, keys.p, vals.p,
                        ((uint64_t)1 << cb) - 1, A.v.p, B.ci.p, B.v.p);
     LG_LAUNCH_CHECK();

@@ -1,4 +1,4 @@
-// Deterministic device primitives: scan, stable LSD radix sort, canonical
+// Deterministic device primitives: scan, stable radix sort, canonical
 // segmented sums.  All run on one stream; functions that return a host value
 // synchronise that stream, the others return with their work queued.
 #pragma once
@@ -11,10 +11,10 @@ namespace lg {
 int64_t exclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n, hipStream_t s);
 int64_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
 
-// Stable sort of (key, val) pairs by the low `bits` bits of the keys
-// (8-bit digits, least significant first).  Keys above 2^bits are not
-// allowed.  Sorts in place; tmp buffers are allocated internally.
-void radix_sort_pairs(uint64_t *keys, uint64_t *vals, int64_t n, int bits, hipStream_t s);
+// Stable sort of the first n (key, val) pairs by the low `bits` bits of the
+// keys (keys of 2^bits or more are not allowed).  The sorted pairs are left
+// in keys / vals: each may be exchanged for an internal buffer of n items.
+void radix_sort_pairs(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t n, int bits, hipStream_t s);
 
 // Canonical sum (the order of the CPU oracle's canon_sum) of the values of
 // each run of equal keys in a sorted key array.  Writes the run keys and

@@ -70,29 +70,57 @@ __global__ void __launch_bounds__(256) k_rows(int n, const int32_t *__restrict__
   op.finish(row, acc);
 }
 
-// Rows of WAVE + 1 .. CHUNK entries, one wavefront each: 64 strided partial
-// sums and the fixed tree.  One wavefront per row keeps every wavefront's
-// work within CHUNK / WAVE steps wherever the long rows sit (hubs are
-// neighbours in vertex order on R-MAT graphs and in the solve layout).
+// Rows of WAVE + 1 .. CHUNK entries (mrow).  Wavefront w takes the rows
+// j = w, w + W, w + 2W, ... of the list (W wavefronts in the grid) in
+// batches of 64, lane l holding the l-th row of a batch: its bounds are
+// loaded for all lanes at once, the wavefront sums the batch's rows one
+// after another (64 strided partial sums and the fixed tree each; lane l
+// keeps row l's result) and the lanes finish their rows together.  Striding
+// spreads the long rows evenly over the wavefronts wherever they sit (hubs
+// are neighbours in vertex order on R-MAT graphs and in the solve layout).
 template <class Op>
 __global__ void __launch_bounds__(256) k_rows_mid(int64_t nmid, const int32_t *__restrict__ mrow,
                                                   const int32_t *__restrict__ rp, Op op) {
   using Acc = typename Op::Acc;
   const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
-  if (j >= nmid) return;  // the whole wavefront
-  const int row = mrow[j];
-  if (row_skip(op, row)) return;
-  const int la = rp[row], lb = rp[row + 1];
-  Acc p = op.zero();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / WAVE);
+  const int64_t w = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  for (int64_t base = w; base < nmid; base += nw * WAVE) {
+    const int64_t j = base + (int64_t)lane * nw;
+    int row = -1, a = 0, b = 0;
+    if (j < nmid) {
+      row = mrow[j];
+      if (row_skip(op, row)) {
+        row = -1;
+      } else {
+        a = rp[row];
+        b = rp[row + 1];
+      }
+    }
+    Acc acc = op.zero();
+    uint64_t mask = __ballot(row >= 0);
+    while (mask) {
+      const int L = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      const int la = __shfl(a, L, WAVE), lb = __shfl(b, L, WAVE), lr = __shfl(row, L, WAVE);
+      Acc p = op.zero();
 #pragma unroll 4
-  for (int k = la + lane; k < lb; k += WAVE) op.add(p, row, k);
-  for (int off = 32; off >= 1; off >>= 1) {
-    Acc o = op.shfl_down(p, off);
-    p = op.merge(p, o);
+      for (int k = la + lane; k < lb; k += WAVE) op.add(p, lr, k);
+      for (int off = 32; off >= 1; off >>= 1) {
+        Acc o = op.shfl_down(p, off);
+        p = op.merge(p, o);
+      }
+      p = op.shfl0(p);
+      if (lane == L) acc = p;
+    }
+    if (row >= 0) op.finish(row, acc);
   }
-  if (lane == 0) op.finish(row, p);
 }
+
+// wavefronts of k_rows_mid: one per row up to about 8 per SIMD of the GPU
+// (each then sums a batch of up to 64 rows, so up to 2^19 rows take one
+// batch per wavefront)
+constexpr int64_t MID_WAVES = 256 * 4 * 8;
 
 // one wavefront per chunk of a huge row: the chunk's canonical sum
 template <class Op>
@@ -230,7 +258,8 @@ void run_rows(int64_t n, const int32_t *rp, const Op &op, const HugeRows &Hr, hi
   hipLaunchKernelGGL(k_rows<Op>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int)n, rp, op);
   LG_LAUNCH_CHECK();
   if (Hr.nmid) {
-    hipLaunchKernelGGL(k_rows_mid<Op>, dim3((unsigned)((Hr.nmid + 3) / 4)), dim3(256), 0, s, Hr.nmid, Hr.mrow.p,
+    const int64_t waves = std::min<int64_t>(Hr.nmid, MID_WAVES);
+    hipLaunchKernelGGL(k_rows_mid<Op>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, Hr.nmid, Hr.mrow.p,
                        rp, op);
     LG_LAUNCH_CHECK();
   }
@@ -1168,9 +1197,9 @@ void transpose_p(const DCsr &P, int64_t ncols, DCsr &R, hipStream_t s) {
     hipLaunchKernelGGL(k_p_transpose_keys32, dim3(g1(n)), dim3(256), 0, s, (int)n, P.rp.p, P.ci.p, P.v32.p, rb,
                        keys32.p, vals32.p);
     LG_LAUNCH_CHECK();
-    radix_sort_pairs(keys32.p, vals32.p, nnzP, rb + cb, s);
+    radix_sort_pairs(keys32, vals32, nnzP, rb + cb, s);
   }
-  radix_sort_pairs(keys.p, vals.p, nnzP, rb + cb, s);
+  radix_sort_pairs(keys, vals, nnzP, rb + cb, s);
   R.n = ncols;
   R.ncols = n;
   R.nnz = nnzP;
@@ -1201,7 +1230,7 @@ void build_members(const int32_t *agg, int64_t n, int64_t nc, DCsr &M, hipStream
   DevBuf<uint64_t> keys(n), vals(n);
   hipLaunchKernelGGL(k_member_pairs, dim3(g1(n)), dim3(256), 0, s, (int)n, agg, keys.p, vals.p);
   LG_LAUNCH_CHECK();
-  radix_sort_pairs(keys.p, vals.p, n, std::max(1, bits_for((uint64_t)nc)), s);
+  radix_sort_pairs(keys, vals, n, std::max(1, bits_for((uint64_t)nc)), s);
   M.n = nc;
   M.ncols = n;
   M.nnz = n;
@@ -1220,7 +1249,7 @@ void contract_to_csr(DevBuf<uint64_t> &keys, DevBuf<uint64_t> &vals, int64_t m, 
                      int64_t ncols, DCsr &out, hipStream_t s) {
   const int cb = std::max(1, bits_for((uint64_t)ncols));
   const int rb = std::max(1, bits_for((uint64_t)nrows));
-  radix_sort_pairs(keys.p, vals.p, m, cb + rb, s);
+  radix_sort_pairs(keys, vals, m, cb + rb, s);
   DevBuf<uint64_t> uk(m);
   DevBuf<double> us(m);
   int64_t nu = segmented_canon_sum(keys.p, vals.p, m, uk.p, us.p, s);
@@ -1345,7 +1374,7 @@ void make_elim32(DLevel &L, hipStream_t s) {
   hipLaunchKernelGGL(k_p_transpose_keys32, dim3(g1(n)), dim3(256), 0, s, (int)n, L.P.rp.p, L.P.ci.p,
                      L.P.v32.p, rb, keys.p, vals.p);
   LG_LAUNCH_CHECK();
-  radix_sort_pairs(keys.p, vals.p, nnzP, rb + cb, s);
+  radix_sort_pairs(keys, vals, nnzP, rb + cb, s);
   L.R.v32.alloc(std::max<int64_t>(nnzP, 1));
   if (nnzP) {
     hipLaunchKernelGGL(k_unpack_vals32, dim3(grid_for(nnzP, 256, 8192)), dim3(256), 0, s, vals.p, nnzP,
@@ -1394,7 +1423,7 @@ bool transpose_positions(const DCsr &A, const HugeRows &Hr, int32_t *tpos, hipSt
   if (m >= ((int64_t)1 << 20)) {
     DevBuf<uint64_t> keys(m), vals(m);
     run_rows(n, A.rp.p, OpTposKeys{{}, A.ci.p, keys.p, vals.p}, Hr, s);
-    radix_sort_pairs(keys.p, vals.p, m, std::max(1, bits_for((uint64_t)n)), s);
+    radix_sort_pairs(keys, vals, m, std::max(1, bits_for((uint64_t)n)), s);
     hipLaunchKernelGGL(k_tpos_scatter, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, m, A.rp.p, A.ci.p,
                        keys.p, vals.p, tpos, bad.p);
     LG_LAUNCH_CHECK();
@@ -1510,7 +1539,7 @@ int64_t run_voting(const DCsr &A, const double *S, const double *cs, const ligmg
       LG_LAUNCH_CHECK();
       // stable sort: each target's voters stay in ascending voter index,
       // the order in which tally_votes adds them (aggregation.hpp:539-547)
-      radix_sort_pairs(keys.p, vals.p, m, kb, s);
+      radix_sort_pairs(keys, vals, m, kb, s);
       int64_t nu = segmented_seq_sum(keys.p, vals.p, m, uk.p, us.p, s);
       hipLaunchKernelGGL(k_scatter_sums, dim3(grid_for(nu, 256, 8192)), dim3(256), 0, s, nu, uk.p, us.p,
                          nv.p);
