@@ -29,17 +29,21 @@ HBM_PEAK_GBS = 8000.0
 def traffic_of(kernel):
     """HBM bytes per launch of a kernel, from the committed counter summary
     (FETCH_SIZE x read calibration + WRITE_SIZE x write calibration, both
-    calibrated on a copy kernel of known size), or None."""
+    calibrated on a copy kernel of known size), and the source commit that
+    summary names; (None, None) without it."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_rmat23.txt")
+    value, commit = None, None
     try:
         with open(path) as f:
             for line in f:
+                if line.startswith("#") and "source commit" in line and commit is None:
+                    commit = line.split("source commit", 1)[1].split()[0].strip(",")
                 parts = line.split()
                 if len(parts) >= 2 and parts[0] == kernel + ":":
-                    return float(parts[1])
+                    value = float(parts[1])
     except OSError:
         pass
-    return None
+    return value, commit
 
 
 def cpu_full_workload():
@@ -454,6 +458,7 @@ def main():
           file=sys.stderr, flush=True)
 
     ms_step = 1e3 * wall / args.steps
+    traffic = traffic_of("smoother_fp32")
     out = {
         "metric": METRIC,
         "value": ms_step / 1e3,
@@ -490,10 +495,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": cheb32["gbs"] / HBM_PEAK_GBS,
-            "traffic": traffic_of("smoother_fp32"),
+            "traffic": traffic[0],
             "traffic_kind": ("L2 fill bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on "
                              "kernels of known size; FETCH_SIZE also counts Infinity Cache hits), read from "
                              "profiles/hbm_traffic_rmat23.txt, not measured in this run"),
+            "traffic_source_commit": traffic[1],
             "kernel": ("k_spmv_units<float, ValArr<float>, EpiDiag<float, EpiCheb<float>>, true> "
                        "(base-level FP32 Chebyshev step)"),
         },

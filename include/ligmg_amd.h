@@ -327,6 +327,12 @@ int ligmg_bench_coarse(ligmg_handle h, int32_t k, int32_t iters,
  * 0.  Also reports the unit count. */
 int ligmg_check_units(int64_t n, const int32_t *rp, int32_t *same, int64_t *nunits);
 
+/* (new) Diagnostic: Y = P X with the coarsest level's dense pseudo-inverse
+ * P (n_c x n_c) for a block of k <= 64 right-hand sides, by the batched
+ * solves' kernel (k rounded up to a power of two; the matrix cores from 16
+ * columns).  X and Y are host arrays, n_c x k, row-major. */
+int ligmg_coarse_apply_block(ligmg_handle h, int32_t k, const double *X, double *Y);
+
 /* z = M r: one cycle of the preconditioner from level `level` down (the
  * V- or K-cycle of multigrid.hpp:86-130 with a zero initial guess) in the
  * given precision.  r and z are host arrays of length n in FP64.  Mirrors

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["prims.hip", "solve.hip", "setup.hip", "batch.hip", "solver.hip", "order.hip", "graphprep.hip", "tail.hip", "pool.hip", "pc.hip"]
+HIP_SOURCES = ["prims.hip", "solve.hip", "setup.hip", "batch.hip", "solver.hip", "order.hip", "graphprep.hip", "tail.hip", "pool.hip", "pc.hip", "dense.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: sums and products round exactly as written, which the
 # bit-exact setup relies on.  Packed FP32 instructions are disabled.

@@ -291,8 +291,9 @@ __global__ void k_from_block(const double *X, int64_t n, int k, int kp, const in
 // ------------------------------------------------------- coarse dense apply
 // Y (n x kp) = Pm (n x n, FP64, row-major) * X (n x kp) with FP64 MFMA
 // 16x16x4 tiles: one wavefront per 16 x 16 output tile.  Operand layout of
-// v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16] of
-// the 16x4 / 4x16 slices and holds D[4*(l/16) + v][l%16], v = 0..3.
+// v_mfma_f64_16x16x4f64 (measured on gfx950): lane l supplies A[l%16][l/16]
+// and B[l/16][l%16] of the 16x4 / 4x16 slices and holds D[l/16 + 4v][l%16],
+// v = 0..3.
 template <class T>
 __global__ void __launch_bounds__(256) k_dense_mfma(const double *__restrict__ Pm, int n,
                                                     const T *__restrict__ X, int kp,
@@ -315,7 +316,7 @@ __global__ void __launch_bounds__(256) k_dense_mfma(const double *__restrict__ P
   }
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    const int r = i0 + 4 * lk + v;
+    const int r = i0 + lk + 4 * v;
     if (r < n) Y[(int64_t)r * kp + j0 + li] = (T)acc[v];
   }
 }

@@ -1832,15 +1832,11 @@ double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_
 // Coarse level: dense pseudo-inverse per connected component,
 // L_c^+ = (L_c + 1 1^T / n_c)^-1 - 1 1^T / n_c  (host, FP64 Cholesky).
 // ----------------------------------------------------------------------------
-std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,
-                                    const std::vector<int32_t> &ci, const std::vector<double> &v) {
-  // the hierarchy stops at min_size entries, so only coarsening that stalls
-  // until max_levels leaves a larger coarse level
-  LG_CHECK(n <= COARSE_DENSE_MAX, LIGMG_ERR_UNSUPPORTED,
-           "the coarsest level has " + std::to_string(n) + " vertices; the dense coarse solve takes at most " +
-               std::to_string(COARSE_DENSE_MAX) + " (coarsening stalled: check the aggregation options)");
-  std::vector<int32_t> comp(n, -1);
-  std::vector<int64_t> csize;
+std::vector<double> coarse_dense_spd(int64_t n, const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,
+                                     const std::vector<double> &v, std::vector<int32_t> &comp,
+                                     std::vector<int64_t> &csize) {
+  comp.assign(n, -1);
+  csize.clear();
   std::vector<int64_t> stack;
   for (int64_t s0 = 0; s0 < n; s0++) {
     if (comp[s0] >= 0) continue;
@@ -1872,6 +1868,19 @@ std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,
       double a = 0.5 * (M[i * n + j] + M[j * n + i]);
       M[i * n + j] = M[j * n + i] = a;
     }
+  return M;
+}
+
+std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,
+                                    const std::vector<int32_t> &ci, const std::vector<double> &v) {
+  // the hierarchy stops at min_size entries, so only coarsening that stalls
+  // until max_levels leaves a larger coarse level
+  LG_CHECK(n <= COARSE_DENSE_MAX, LIGMG_ERR_UNSUPPORTED,
+           "the coarsest level has " + std::to_string(n) + " vertices; the dense coarse solve takes at most " +
+               std::to_string(COARSE_DENSE_MAX) + " (coarsening stalled: check the aggregation options)");
+  std::vector<int32_t> comp;
+  std::vector<int64_t> csize;
+  std::vector<double> M = coarse_dense_spd(n, rp, ci, v, comp, csize);
   // in-place Cholesky (lower), then inverse from G^-T G^-1
   std::vector<double> &G = M;
   for (int64_t j = 0; j < n; j++) {

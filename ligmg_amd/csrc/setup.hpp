@@ -90,8 +90,20 @@ void make_elim32(DLevel &L, hipStream_t s);
 // perm (may be null): setup position of each row, for the start vector
 double eig_estimate(const DCsr &A, const double *dinv, uint64_t seed, int level_idx, int iters,
                     const int32_t *perm, hipStream_t s);
+// Dense pseudo-inverse of the coarsest level (n <= COARSE_DENSE_MAX): the
+// inverse of L + sum over components C of 1_C 1_C^T / |C|, minus those terms.
 constexpr int64_t COARSE_DENSE_MAX = 8192;
+// coarse_pinv factors on the host below COARSE_DEVICE_MIN vertices (serial
+// Cholesky, dense_pinv_host) and on the device from there (blocked
+// Gauss-Jordan on the matrix cores); the result is a device n x n array.
+constexpr int64_t COARSE_DEVICE_MIN = 256;
+void coarse_pinv(int64_t n, const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,
+                 const std::vector<double> &v, DevBuf<double> &pinv, hipStream_t s);
 std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,
                                     const std::vector<int32_t> &ci, const std::vector<double> &v);
+// The SPD matrix above (row-major n x n), its components and their sizes.
+std::vector<double> coarse_dense_spd(int64_t n, const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,
+                                     const std::vector<double> &v, std::vector<int32_t> &comp,
+                                     std::vector<int64_t> &csize);
 
 }  // namespace lg

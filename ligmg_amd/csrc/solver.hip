@@ -315,9 +315,7 @@ struct Solver {
       auto hrp = download(cur.rp.p, cur.n + 1, s);
       auto hci = download(cur.ci.p, cur.nnz, s);
       auto hv = download(cur.v.p, cur.nnz, s);
-      auto pinv = dense_pinv_host(cur.n, hrp, hci, hv);
-      C.pinv.alloc(std::max<size_t>(pinv.size(), 1));
-      to_device(C.pinv.p, pinv.data(), pinv.size(), s);
+      coarse_pinv(cur.n, hrp, hci, hv, C.pinv, s);
       C.A = std::move(cur);
       H.push_back(std::move(C));
       lap("coarse", H.size() - 1, H.back().A.n, H.back().A.nnz);
@@ -661,9 +659,7 @@ struct Solver {
         build_members(L.agg.p, n, L.nc, L.M, s);
       } else {
         L.kind = LIGMG_LEVEL_COARSE;
-        auto pinv = dense_pinv_host(n, A.rp, A.ci, A.v);
-        L.pinv.alloc(std::max<size_t>(pinv.size(), 1));
-        to_device(L.pinv.p, pinv.data(), pinv.size(), s);
+        coarse_pinv(n, A.rp, A.ci, A.v, L.pinv, s);
       }
       H.push_back(std::move(L));
     }
@@ -2657,6 +2653,33 @@ int ligmg_bench_coarse(ligmg_handle h, int32_t k, int32_t iters, double *avg_ms,
     (void)hipEventDestroy(e1);
     *avg_ms = ms / iters;
     *flops = 2.0 * (double)n * (double)n * (double)kp;
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
+int ligmg_coarse_apply_block(ligmg_handle h, int32_t k, const double *X, double *Y) {
+  if (!h || k < 1 || k > 64 || !X || !Y) return LIGMG_ERR_INVALID;
+  try {
+    lg::Solver &S = h->S;
+    lg::DLevel &L = S.H.back();
+    LG_CHECK(L.kind == LIGMG_LEVEL_COARSE && L.pinv.p, LIGMG_ERR_INVALID,
+             "ligmg_coarse_apply_block: no dense coarse level");
+    int kp = 1;
+    while (kp < k) kp <<= 1;
+    const int64_t n = L.n;
+    std::vector<double> hx((size_t)n * kp, 0.0), hy((size_t)n * kp);
+    for (int64_t i = 0; i < n; i++)
+      for (int c = 0; c < k; c++) hx[i * kp + c] = X[i * k + c];
+    lg::DevBuf<double> dx(std::max<size_t>(hx.size(), 1)), dy(std::max<size_t>(hx.size(), 1));
+    lg::to_device(dx.p, hx.data(), hx.size(), S.s);
+    if (kp == 1) lg::op_dense_apply<double>(L.pinv.p, n, dx.p, dy.p, S.s);
+    else lg::op_bdense_apply<double>(L.pinv.p, n, dx.p, dy.p, kp, S.s);
+    lg::to_host(hy.data(), dy.p, hy.size(), S.s);
+    LG_HIP(hipStreamSynchronize(S.s));
+    for (int64_t i = 0; i < n; i++)
+      for (int c = 0; c < k; c++) Y[i * k + c] = hy[i * kp + c];
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return fail(e);
