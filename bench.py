@@ -224,6 +224,41 @@ def bench_kcycle(args):
           iters=info.iters(), setup_s=s.setup_time, vcycle_cg_s=vt)
 
 
+def bench_coarse(args):
+    """The dense pseudo-inverse of a large coarsest level: a 3-D 7-point grid
+    (weights uniform[0.5, 1.5]) whose hierarchy is capped at two levels
+    (max_levels = 2), so the coarse level holds several thousand vertices.
+    A step re-factors it as setup does (blocked Gauss-Jordan on the matrix
+    cores) and applies it to a block of 64 right-hand sides."""
+    import ligmg_amd
+    from ligmg_amd import _ffi, graphs
+    side = 40
+    g = graphs.grid3d(side, side, side, wkind=_ffi.W_UNIFORM, p0=0.5, p1=1.5, seed=0)
+    s = ligmg_amd.Solver(g, ligmg_amd.Options(max_levels=2))
+    nc = s.level_info(s.num_levels() - 1)[1]
+    lib = _ffi.solver_lib()
+    tot, gj, fl = C.c_double(), C.c_double(), C.c_double()
+    for _ in range(args.warmup):
+        assert lib.ligmg_bench_pinv(s._h, C.byref(tot), C.byref(gj), C.byref(fl)) == 0
+    tots, gjs = [], []
+    for _ in range(args.steps):
+        assert lib.ligmg_bench_pinv(s._h, C.byref(tot), C.byref(gj), C.byref(fl)) == 0
+        tots.append(tot.value)
+        gjs.append(gj.value)
+    ms, fl64 = C.c_double(), C.c_double()
+    assert lib.ligmg_bench_coarse(s._h, 64, 20, C.byref(ms), C.byref(fl64)) == 0
+    tf = fl.value / (min(gjs) * 1e-3) / 1e12
+    print(f"# coarse n_c={nc}: inversion {min(gjs):.2f} ms on the device ({tf:.2f} TFLOP/s FP64), "
+          f"{min(tots):.1f} ms with host assembly and upload; apply to 64 columns {ms.value * 1e3:.1f} us "
+          f"({fl64.value / (ms.value * 1e-3) / 1e12:.2f} TFLOP/s)", file=sys.stderr, flush=True)
+    _line("dense coarse pseudo-inverse: device inversion time", min(gjs) / 1e3, "s", False, min(tots), args, "f64",
+          {"workload": f"grid3d_{side}^3_uniform0.5-1.5_max_levels2_coarse_pinv", "n_coarse": nc},
+          gj_tflops=tf, total_ms=min(tots),
+          roofline={"bound": "mfma", "achieved": tf, "peak": 78.6, "unit": "TFLOP/s", "frac": tf / 78.6,
+                    "traffic": None},
+          apply64_us=ms.value * 1e3, apply64_tflops=fl64.value / (ms.value * 1e-3) / 1e12)
+
+
 def bench_grad(args):
     """Forward + backward through x = L(w)^+ b on a random geometric graph:
     1,000,000 vertices in the unit square, radius for mean degree 12, LCC,
@@ -303,10 +338,11 @@ def main():
                     help="vertex order of the hierarchy (default: the input's)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="setup + SpMV timing + copy calibration only (for counter runs)")
-    ap.add_argument("--workload", choices=["rmat", "batched", "grad", "kcycle"], default="rmat",
+    ap.add_argument("--workload", choices=["rmat", "batched", "grad", "kcycle", "coarse"], default="rmat",
                     help="rmat: the headline solve (default); batched: 48 BA graphs x 16 right-hand "
                          "sides and a 160^3 grid x 64; grad: gradients through a 1M-vertex geometric graph; "
-                         "kcycle: the FP64 K-cycle on R-MAT 22")
+                         "kcycle: the FP64 K-cycle on R-MAT 22; coarse: the dense pseudo-inverse of a "
+                         "large coarse level")
     args = ap.parse_args()
     if args.cpu_only:
         th = [int(t) for t in args.cpu_threads.split(",") if t] or None
@@ -330,6 +366,8 @@ def main():
         return bench_grad(args)
     if args.workload == "kcycle":
         return bench_kcycle(args)
+    if args.workload == "coarse":
+        return bench_coarse(args)
     dev = torch.device("cuda:0")
     t = time.perf_counter()
     g = graphs.rmat_config(args.scale, "lognormal", pendants=True)

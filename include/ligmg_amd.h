@@ -327,6 +327,14 @@ int ligmg_bench_coarse(ligmg_handle h, int32_t k, int32_t iters,
  * 0.  Also reports the unit count. */
 int ligmg_check_units(int64_t n, const int32_t *rp, int32_t *same, int64_t *nunits);
 
+/* (new) Re-factors the coarsest level's dense pseudo-inverse as setup does
+ * (on the device from 256 vertices: blocked Gauss-Jordan on the matrix
+ * cores) and reports the wall milliseconds of the whole step (host matrix
+ * assembly, upload, inversion), the milliseconds of the device inversion
+ * kernels (HIP events; 0 on the host path) and their floating-point
+ * operations, 2 * n_p^3 for n_p = n_c rounded up to a multiple of 64. */
+int ligmg_bench_pinv(ligmg_handle h, double *total_ms, double *gj_ms, double *flops);
+
 /* (new) Diagnostic: Y = P X with the coarsest level's dense pseudo-inverse
  * P (n_c x n_c) for a block of k <= 64 right-hand sides, by the batched
  * solves' kernel (k rounded up to a power of two; the matrix cores from 16

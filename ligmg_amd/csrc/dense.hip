@@ -143,6 +143,36 @@ __global__ void __launch_bounds__(256) k_gj_colpanel(double *A, int64_t np, int 
   store_block(acc, Aik, np);
 }
 
+// The SPD matrix of the coarse level, padded to np x np: the stored entries
+// of each row (in row order), then 1 / |C| on the pairs of one component C,
+// then (A + A^T) / 2; identity on the padding.  The same steps as the host
+// assembly (coarse_dense_spd).
+__global__ void k_spd_rows(int64_t n, int64_t np, const int32_t *rp, const int32_t *ci, const double *v,
+                           double *A) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i >= n) {
+      A[i * np + i] = 1.0;
+      continue;
+    }
+    for (int32_t k = rp[i]; k < rp[i + 1]; k++) A[i * np + ci[k]] += v[k];
+  }
+}
+__global__ void k_spd_comp(int64_t n, int64_t np, const int32_t *comp, const double *cinv, double *A) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, j = e % n;
+    if (comp[i] == comp[j]) A[i * np + j] += cinv[comp[i]];
+  }
+}
+__global__ void k_spd_sym(int64_t n, int64_t np, double *A) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, j = e % n;
+    if (j <= i) continue;
+    const double a = 0.5 * (A[i * np + j] + A[j * np + i]);
+    A[i * np + j] = a;
+    A[j * np + i] = a;
+  }
+}
+
 // pinv (n x n) = inverse (padded, stride np) minus 1 / |C| on the pairs of
 // one component
 __global__ void k_pinv_compact(const double *Ainv, int64_t np, int64_t n, const int32_t *comp,
@@ -157,7 +187,8 @@ __global__ void k_pinv_compact(const double *Ainv, int64_t np, int64_t n, const 
 }  // namespace
 
 void coarse_pinv(int64_t n, const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,
-                 const std::vector<double> &v, DevBuf<double> &pinv, hipStream_t s) {
+                 const std::vector<double> &v, DevBuf<double> &pinv, hipStream_t s, double *gj_ms) {
+  if (gj_ms) *gj_ms = 0.0;
   if (n < COARSE_DEVICE_MIN) {
     const std::vector<double> h = dense_pinv_host(n, rp, ci, v);
     pinv.alloc(std::max<size_t>(h.size(), 1));
@@ -168,18 +199,56 @@ void coarse_pinv(int64_t n, const std::vector<int32_t> &rp, const std::vector<in
   LG_CHECK(n <= COARSE_DENSE_MAX, LIGMG_ERR_UNSUPPORTED,
            "the coarsest level has " + std::to_string(n) + " vertices; the dense coarse solve takes at most " +
                std::to_string(COARSE_DENSE_MAX) + " (coarsening stalled: check the aggregation options)");
-  std::vector<int32_t> comp;
+  // components (host: the coarse level's entries are few), then the matrix
+  // assembled on the device
+  std::vector<int32_t> comp(n, -1);
   std::vector<int64_t> csize;
-  std::vector<double> M = coarse_dense_spd(n, rp, ci, v, comp, csize);
-  const int64_t nb = (n + GB - 1) / GB, np = nb * GB;
-  // padded copy: identity on the padding
-  std::vector<double> P((size_t)np * np, 0.0);
-  for (int64_t i = 0; i < n; i++) std::copy(M.begin() + i * n, M.begin() + (i + 1) * n, P.begin() + i * np);
-  for (int64_t i = n; i < np; i++) P[i * np + i] = 1.0;
-  M.clear();
-  M.shrink_to_fit();
+  {
+    std::vector<int64_t> stack;
+    for (int64_t s0 = 0; s0 < n; s0++) {
+      if (comp[s0] >= 0) continue;
+      const int32_t cid = (int32_t)csize.size();
+      csize.push_back(0);
+      stack.assign(1, s0);
+      comp[s0] = cid;
+      while (!stack.empty()) {
+        const int64_t u = stack.back();
+        stack.pop_back();
+        csize[cid]++;
+        for (int32_t k = rp[u]; k < rp[u + 1]; k++)
+          if (v[k] != 0 && comp[ci[k]] < 0) {
+            comp[ci[k]] = cid;
+            stack.push_back(ci[k]);
+          }
+      }
+    }
+  }
+  std::vector<double> cinv(csize.size());
+  for (size_t c = 0; c < csize.size(); c++) cinv[c] = 1.0 / (double)csize[c];
+  const int64_t nb = (n + GB - 1) / GB, np = nb * GB, nnz = rp[n];
+  DevBuf<int32_t> drp(n + 1), dci(std::max<int64_t>(nnz, 1)), dcomp(n);
+  DevBuf<double> dv(std::max<int64_t>(nnz, 1)), dcinv(cinv.size());
+  to_device(drp.p, rp.data(), n + 1, s);
+  to_device(dci.p, ci.data(), nnz, s);
+  to_device(dv.p, v.data(), nnz, s);
+  to_device(dcomp.p, comp.data(), n, s);
+  to_device(dcinv.p, cinv.data(), cinv.size(), s);
   DevBuf<double> A((size_t)np * np), dinv(GB * GB);
-  to_device(A.p, P.data(), P.size(), s);
+  fill_f64(A.p, 0.0, np * np, s);
+  hipLaunchKernelGGL(k_spd_rows, dim3(grid_for(np, 256, 16384)), dim3(256), 0, s, n, np, (const int32_t *)drp.p,
+                     (const int32_t *)dci.p, (const double *)dv.p, A.p);
+  LG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_spd_comp, dim3(grid_for(n * n, 256, 16384)), dim3(256), 0, s, n, np,
+                     (const int32_t *)dcomp.p, (const double *)dcinv.p, A.p);
+  LG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_spd_sym, dim3(grid_for(n * n, 256, 16384)), dim3(256), 0, s, n, np, A.p);
+  LG_LAUNCH_CHECK();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (gj_ms) {
+    LG_HIP(hipEventCreate(&e0));
+    LG_HIP(hipEventCreate(&e1));
+    LG_HIP(hipEventRecord(e0, s));
+  }
   for (int k = 0; k < nb; k++) {
     hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(256), 0, s, A.p, np, k, dinv.p);
     LG_LAUNCH_CHECK();
@@ -191,12 +260,15 @@ void coarse_pinv(int64_t n, const std::vector<int32_t> &rp, const std::vector<in
     hipLaunchKernelGGL(k_gj_colpanel, dim3((unsigned)nb), dim3(256), 0, s, A.p, np, k, (const double *)dinv.p);
     LG_LAUNCH_CHECK();
   }
-  std::vector<double> cinv(csize.size());
-  for (size_t c = 0; c < csize.size(); c++) cinv[c] = 1.0 / (double)csize[c];
-  DevBuf<int32_t> dcomp(n);
-  DevBuf<double> dcinv(cinv.size());
-  to_device(dcomp.p, comp.data(), n, s);
-  to_device(dcinv.p, cinv.data(), cinv.size(), s);
+  if (gj_ms) {
+    LG_HIP(hipEventRecord(e1, s));
+    LG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *gj_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
   pinv.alloc((size_t)n * n);
   hipLaunchKernelGGL(k_pinv_compact, dim3(grid_for(n * n, 256, 16384)), dim3(256), 0, s, (const double *)A.p, np,
                      n, (const int32_t *)dcomp.p, (const double *)dcinv.p, pinv.p);

@@ -97,8 +97,10 @@ constexpr int64_t COARSE_DENSE_MAX = 8192;
 // Cholesky, dense_pinv_host) and on the device from there (blocked
 // Gauss-Jordan on the matrix cores); the result is a device n x n array.
 constexpr int64_t COARSE_DEVICE_MIN = 256;
+// gj_ms (optional): milliseconds of the device inversion kernels (0 on the
+// host path).
 void coarse_pinv(int64_t n, const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,
-                 const std::vector<double> &v, DevBuf<double> &pinv, hipStream_t s);
+                 const std::vector<double> &v, DevBuf<double> &pinv, hipStream_t s, double *gj_ms = nullptr);
 std::vector<double> dense_pinv_host(int64_t n, const std::vector<int32_t> &rp,
                                     const std::vector<int32_t> &ci, const std::vector<double> &v);
 // The SPD matrix above (row-major n x n), its components and their sizes.

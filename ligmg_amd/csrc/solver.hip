@@ -2659,6 +2659,28 @@ int ligmg_bench_coarse(ligmg_handle h, int32_t k, int32_t iters, double *avg_ms,
   }
 }
 
+int ligmg_bench_pinv(ligmg_handle h, double *total_ms, double *gj_ms, double *flops) {
+  if (!h || !total_ms || !gj_ms || !flops) return LIGMG_ERR_INVALID;
+  try {
+    lg::Solver &S = h->S;
+    lg::DLevel &L = S.H.back();
+    LG_CHECK(L.kind == LIGMG_LEVEL_COARSE, LIGMG_ERR_INVALID, "ligmg_bench_pinv: no coarse level");
+    const int64_t n = L.A.n;
+    const auto rp = lg::download(L.A.rp.p, n + 1, S.s);
+    const auto ci = lg::download(L.A.ci.p, L.A.nnz, S.s);
+    const auto v = lg::download(L.A.v.p, L.A.nnz, S.s);
+    lg::DevBuf<double> pinv;
+    const auto t0 = std::chrono::steady_clock::now();
+    lg::coarse_pinv(n, rp, ci, v, pinv, S.s, gj_ms);
+    *total_ms = 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double np = (double)((n + 63) / 64 * 64);
+    *flops = (n >= lg::COARSE_DEVICE_MIN) ? 2.0 * np * np * np : 0.0;
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
 int ligmg_coarse_apply_block(ligmg_handle h, int32_t k, const double *X, double *Y) {
   if (!h || k < 1 || k > 64 || !X || !Y) return LIGMG_ERR_INVALID;
   try {
