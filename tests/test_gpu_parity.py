@@ -510,6 +510,32 @@ def test_hierarchy_bit_exact_with_rows_beyond_one_chunk(ordering):
     assert_same_hierarchy(d, O.OracleSolver(g, oo))
 
 
+def test_million_neighbour_hub():
+    """One vertex joined to a million others that also form a ring: a row of
+    1,000,000 entries (245 chunks of 4096 in every setup row kernel, a
+    million-term sequential fold for its column sum, 489 SpMV chunks) on the
+    top levels.  Hierarchy bit-exact against the oracle; the solve reaches
+    1e-8 and matches the oracle's solution."""
+    n = 1_000_001
+    rng = np.random.default_rng(11)
+    leaves = np.arange(1, n)
+    u = np.concatenate([np.zeros(n - 1, np.int64), leaves])
+    v = np.concatenate([leaves, np.roll(leaves, -1)])
+    g = graphs.from_edges(n, u, v, rng.uniform(0.5, 1.5, len(u)))
+    assert np.diff(g.rowptr).max() == n  # the hub's row: n - 1 neighbours and the diagonal
+    do, oo = opts_pair()
+    d = ligmg_amd.Solver(g, do)
+    r = O.OracleSolver(g, oo)
+    print(f"levels {[d.level_info(l)[:3] for l in range(d.num_levels())]}", flush=True)
+    assert_same_hierarchy(d, r)
+    b = graphs.rhs(g.n, 1, 2)
+    info = d.solve(b)
+    xr, rinfo, _ = r.solve(b)
+    assert info.rel_residual() <= 1e-8 and abs(info.iters() - rinfo.iters) <= 1
+    x = info.solution
+    assert np.linalg.norm((x - x.mean()) - (xr - xr.mean())) <= 1e-6 * np.linalg.norm(xr - xr.mean())
+
+
 def test_level_spmv_large_operator():
     """Operators of at least 32768 work units take two units per workgroup
     and 16 entries per lane in their row reduction: the base level of R-MAT
