@@ -350,7 +350,7 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU only (no sharding or collectives); "
+        print("bench.py: this solver runs on one GPU only (no sharding or collectives); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
 

@@ -249,7 +249,7 @@ int ligmg_num_levels(ligmg_handle h, int32_t *out);
  * methods name them: <dir>/<level>_A.petsc for every level; P, R, Q for an
  * elimination level (elimination.hpp:62-67); P, R and strength for an
  * aggregation level (aggregation.hpp:254-272), whose AggState history the
- * reference writes per voting round and this build as the final
+ * reference writes per voting round and this library as the final
  * agg_ind.petsc / agg_state.petsc vectors.  blocks.petsc holds comp_ptr and
  * ordering.petsc the input order (LIGMG_ORDER_DEGREE). */
 int ligmg_dump_hierarchy(ligmg_handle h, const char *dir);

@@ -5,7 +5,7 @@
 // connected component C) is made SPD by adding 1_C 1_C^T / |C| for every
 // component; the pseudo-inverse is the inverse of that matrix minus the same
 // terms.  The reference solves its coarse level with a PETSc direct solver
-// (multigrid.hpp:37-84, 217-257); this build applies a dense pseudo-inverse,
+// (multigrid.hpp:37-84, 217-257); this library applies a dense pseudo-inverse,
 // factored once at setup.
 //
 // Gauss-Jordan by 64 x 64 blocks (A padded to np = 64 * nb rows with an

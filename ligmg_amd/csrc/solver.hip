@@ -685,7 +685,7 @@ struct Solver {
   }
 
   // print_csv_header / print_csv (util.hpp:187-205, options.hpp:139-164):
-  // the reference's columns; options this build has no field for are
+  // the reference's columns; options this library has no field for are
   // printed with the reference's defaults.
   static std::string csv_header() {
     return "StrengthMetric,Redistribute,MaxLevels,MinSize,ElimDegree,AlwaysElim,TopElim,Tolerance,Threshold,"
