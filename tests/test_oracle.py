@@ -424,6 +424,42 @@ def test_eig_estimate_bounds():
     assert abs(est.value - lam) < 1e-8 * lam
 
 
+@pytest.mark.parametrize("seed_level", [(0, 0), (7, 3), (42, 11)])
+def test_eig_estimate_matches_numpy_restatement(seed_level):
+    """The estimator step by step: a numpy restatement (the seeded start
+    vector from the same counter hash, left-to-right inner products, scipy's
+    row-by-row SpMV, the Lanczos tridiagonal) gives the oracle's estimate;
+    a dropped mean removal, a wrong stream or a transposed recurrence term
+    would not."""
+    import lanczos_np
+    seed, level = seed_level
+    for gseed in range(3):
+        g = random_graph(300, 0.02, gseed)
+        rp, ci, v = O.csr_args(g)
+        est = O.f64()
+        O.lib().oracle_eig_estimate(g.n, O.ptr(rp, O.i32), O.ptr(ci, O.i32), O.ptr(v, O.f64), seed, level,
+                                    10, O.C.byref(est))
+        assert lanczos_np.lambda_max(rp, ci, v, seed, level) == pytest.approx(est.value, rel=1e-13)
+
+
+def test_level_eig_estimates_match_numpy_restatement():
+    """Every aggregation level of an R-MAT hierarchy: the smoother's upper
+    bound is cheby_upper times the numpy restatement's estimate from that
+    level's start vector."""
+    import lanczos_np
+    g = graphs.rmat_config(12, "lognormal", pendants=True)
+    opts = O.default_options()
+    r = O.OracleSolver(g, opts)
+    n_agg = 0
+    for l in range(r.num_levels()):
+        if r.level_info(l)[0] != _ffi.LEVEL_AGG:
+            continue
+        lam = lanczos_np.lambda_max(*r.level_matrix(l), opts.seed, l, opts.cheby_eig_iters)
+        assert r.level_cheby(l)[1] == pytest.approx(opts.cheby_upper * lam, rel=1e-13), l
+        n_agg += 1
+    assert n_agg >= 3
+
+
 # ------------------------------------------------------------- coarse (a20)
 
 @pytest.mark.parametrize("connected", [True, False])
