@@ -290,34 +290,70 @@ __global__ void k_from_block(const double *X, int64_t n, int k, int kp, const in
 
 // ------------------------------------------------------- coarse dense apply
 // Y (n x kp) = Pm (n x n, FP64, row-major) * X (n x kp) with FP64 MFMA
-// 16x16x4 tiles: one wavefront per 16 x 16 output tile.  Operand layout of
-// v_mfma_f64_16x16x4f64 (measured on gfx950): lane l supplies A[l%16][l/16]
-// and B[l/16][l%16] of the 16x4 / 4x16 slices and holds D[l/16 + 4v][l%16],
-// v = 0..3.
-template <class T>
+// 16x16x4 tiles.  Operand layout of v_mfma_f64_16x16x4f64 (measured on
+// gfx950): lane l = 16 lk + li supplies A[li][lk] and B[lk][li] of the
+// 16x4 / 4x16 slices and holds D[lk + 4v][li], v = 0..3.
+//
+// One workgroup per 16 rows of Y and all kp = 16 NT columns, so Pm is read
+// once.  Its four wavefronts split the K range: lane (li, lk) of wavefront w
+// takes K = (4 w + lk) Q + q, q = 0 .. Q-1 (Q = ceil(n / 16)), one contiguous
+// run of its row of Pm; two accumulator sets alternate on q so consecutive
+// products into one tile do not wait on each other.  The four partial tiles are added in
+// LDS in wavefront order.
+template <class T, int NT>
 __global__ void __launch_bounds__(256) k_dense_mfma(const double *__restrict__ Pm, int n,
-                                                    const T *__restrict__ X, int kp,
+                                                    const T *__restrict__ X, int kp, int Q,
                                                     T *__restrict__ Y) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-  const int tiles_j = kp / 16;
-  const int ti = wave / tiles_j, tj = wave % tiles_j;
-  const int i0 = ti * 16, j0 = tj * 16;
-  if (i0 >= n) return;
-  const int li = lane & 15, lk = lane >> 4;
   typedef double v4d __attribute__((ext_vector_type(4)));
-  v4d acc = {0, 0, 0, 0};
-  const int arow = i0 + li;
-  for (int k0 = 0; k0 < n; k0 += 4) {
-    const int kk = k0 + lk;
-    const double a = (arow < n && kk < n) ? Pm[(int64_t)arow * n + kk] : 0.0;
-    const double b = (kk < n) ? (double)X[(int64_t)kk * kp + j0 + li] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  __shared__ double part[4][16 * 16 * NT];
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int li = lane & 15, lk = lane >> 4;
+  const int i0 = blockIdx.x * 16, row = i0 + li;
+  const double *prow = Pm + (int64_t)(row < n ? row : n - 1) * n;
+  const int kb = (4 * w + lk) * Q;
+  v4d acc[2][NT];
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int tj = 0; tj < NT; tj++) acc[h][tj] = v4d{0, 0, 0, 0};
+  // K slots go in blocks of QB (16 products), each block's operands loaded
+  // one block ahead
+  constexpr int QB = 16 / NT;
+  auto load_blk = [&](int q0, double *aa, double (*bb)[NT]) {
+#pragma unroll
+    for (int u = 0; u < QB; u++) {
+      const int kk = kb + q0 + u;
+      const bool ok = q0 + u < Q && kk < n;
+      aa[u] = (ok && row < n) ? prow[kk] : 0.0;
+#pragma unroll
+      for (int tj = 0; tj < NT; tj++) bb[u][tj] = ok ? (double)X[(int64_t)kk * kp + 16 * tj + li] : 0.0;
+    }
+  };
+  double a[QB], b[QB][NT];
+  load_blk(0, a, b);
+  for (int q0 = 0; q0 < Q; q0 += QB) {
+    double an[QB], bn[QB][NT];
+    if (q0 + QB < Q) load_blk(q0 + QB, an, bn);
+#pragma unroll
+    for (int u = 0; u < QB; u++)
+#pragma unroll
+      for (int tj = 0; tj < NT; tj++)
+        acc[u & 1][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u][tj], acc[u & 1][tj], 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < QB; u++) {
+      a[u] = an[u];
+#pragma unroll
+      for (int tj = 0; tj < NT; tj++) b[u][tj] = bn[u][tj];
+    }
   }
 #pragma unroll
-  for (int v = 0; v < 4; v++) {
-    const int r = i0 + lk + 4 * v;
-    if (r < n) Y[(int64_t)r * kp + j0 + li] = (T)acc[v];
+  for (int tj = 0; tj < NT; tj++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) part[w][(lk + 4 * v) * 16 * NT + 16 * tj + li] = acc[0][tj][v] + acc[1][tj][v];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 16 * 16 * NT; e += blockDim.x) {
+    const int r = e / (16 * NT), c = e % (16 * NT);
+    if (i0 + r < n) Y[(int64_t)(i0 + r) * kp + c] = (T)(((part[0][e] + part[1][e]) + part[2][e]) + part[3][e]);
   }
 }
 
@@ -464,9 +500,12 @@ void op_bprolong_add(const int32_t *agg, int64_t n, const T *Xc, T *X, int kp, h
 template <class T>
 void op_bdense_apply(const double *Pm, int64_t n, const T *X, T *Y, int kp, hipStream_t s) {
   if (kp >= 16) {
-    const int64_t waves = ((n + 15) / 16) * (kp / 16);
-    hipLaunchKernelGGL(k_dense_mfma<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, Pm, (int)n,
-                       X, kp, Y);
+    LG_CHECK(kp == 16 || kp == 32 || kp == 64, LIGMG_ERR_INVALID, "dense apply: kp must be 16, 32 or 64");
+    const unsigned g = (unsigned)((n + 15) / 16);
+    const int Q = (int)((n + 15) / 16);
+    if (kp == 16) hipLaunchKernelGGL((k_dense_mfma<T, 1>), dim3(g), dim3(256), 0, s, Pm, (int)n, X, kp, Q, Y);
+    else if (kp == 32) hipLaunchKernelGGL((k_dense_mfma<T, 2>), dim3(g), dim3(256), 0, s, Pm, (int)n, X, kp, Q, Y);
+    else hipLaunchKernelGGL((k_dense_mfma<T, 4>), dim3(g), dim3(256), 0, s, Pm, (int)n, X, kp, Q, Y);
   } else {
     hipLaunchKernelGGL(k_dense_small<T>, dim3(grid_for(n * kp, 256, 16384)), dim3(256), 0, s, Pm, n, X,
                        kp, Y);

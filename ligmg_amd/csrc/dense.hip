@@ -8,14 +8,17 @@
 // (multigrid.hpp:37-84, 217-257); this library applies a dense pseudo-inverse,
 // factored once at setup.
 //
-// Gauss-Jordan by 64 x 64 blocks (A padded to np = 64 * nb rows with an
+// Gauss-Jordan by GB x GB blocks (A padded to np = GB * nb rows with an
 // identity block), for k = 0 .. nb-1, with D = A_kk^-1:
 //   A_kj <- D A_kj             (j != k)        row panel
 //   A_ij <- A_ij - A_ik A_kj   (i, j != k)     trailing update (old A_ik)
 //   A_ik <- -A_ik D            (i != k)        column panel
 //   A_kk <- D
 // leaves A^-1 in place.  No pivoting: the matrix is SPD, so every diagonal
-// block met on the way is SPD (a Schur complement of an SPD matrix).
+// block met on the way is SPD (a Schur complement of an SPD matrix).  The
+// trailing update reads and writes the whole matrix once per step, so the
+// block width sets the update's flops per byte (GB / 8): at 128 the matrix
+// cores, not HBM, bound it.
 #include <cmath>
 #include <vector>
 
@@ -26,121 +29,257 @@ namespace lg {
 
 namespace {
 
-constexpr int GB = 64;       // block size
-constexpr int GLD = GB + 1;  // LDS row stride (doubles): rows fall in different banks
+constexpr int GB = COARSE_GJ_BLOCK;  // Gauss-Jordan block
+constexpr int TL = 64;               // output tile of the panel and update kernels
+constexpr int NQ = GB / 4;           // K slots of v_mfma_f64_16x16x4f64 per block
+constexpr int DT = 512;              // threads of k_gj_diag
+constexpr int DR = GB * GB / DT;     // rows of the diagonal block held per thread
+constexpr int DQ = 2;                // pivots per step of k_gj_diag
+static_assert(GB % TL == 0 && DT % GB == 0 && DR == GB / (DT / GB) && (DT / GB) % DQ == 0, "block shapes");
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// D = A_kk^-1 by unblocked Gauss-Jordan in LDS (one workgroup of 256 lanes,
-// 16 elements each); D is written to A_kk and to dinv.
-__global__ void __launch_bounds__(256) k_gj_diag(double *A, int64_t np, int k, double *dinv) {
-  __shared__ double M[GB * GLD];
-  __shared__ double f[GB];
-  const int t = threadIdx.x;
+// D = A_kk^-1 by Gauss-Jordan on DQ pivots a step (the DQ x DQ pivot block
+// inverted in registers, then a rank-DQ update).  Thread t holds column
+// t % GB of rows t / GB + (DT / GB) r (r = 0 .. DR-1) in registers for the
+// whole inversion; each step publishes its pivot rows and columns through
+// LDS (double-buffered: one barrier a step).  The steps are latency-bound
+// (publish, barrier, pivot inverse), so fewer, wider steps are faster.  D is
+// written to A_kk and to dinv.
+__global__ void __launch_bounds__(DT) k_gj_diag(double *A, int64_t np, int k, double *dinv) {
+  constexpr int RS = DT / GB;  // row phases
+  __shared__ double prow[2][DQ][GB], pcol[2][DQ][GB];
+  const int t = threadIdx.x, j = t % GB, w = t / GB;
   const int64_t o = (int64_t)k * GB;
-  for (int e = t; e < GB * GB; e += 256) M[(e / GB) * GLD + e % GB] = A[(o + e / GB) * np + o + e % GB];
-  __syncthreads();
-  for (int p = 0; p < GB; p++) {
-    const double piv = M[p * GLD + p];
-    __syncthreads();
-    // pivot row: M[p][p] <- 1, then the row divided by the pivot
-    if (t < GB) M[p * GLD + t] = ((t == p) ? 1.0 : M[p * GLD + t]) / piv;
-    if (t >= GB && t < 2 * GB) f[t - GB] = M[(t - GB) * GLD + p];
-    __syncthreads();
-    // other rows: M[i][p] <- 0, then row i minus f_i times the pivot row
-    for (int e = t; e < GB * GB; e += 256) {
-      const int i = e / GB, j = e % GB;
-      if (i == p) continue;
-      const double m = (j == p) ? 0.0 : M[i * GLD + j];
-      M[i * GLD + j] = m - f[i] * M[p * GLD + j];
+  double m[DR];
+#pragma unroll
+  for (int r = 0; r < DR; r++) m[r] = A[(o + w + RS * r) * np + o + j];
+  // the slot loop is unrolled (the pivot rows of a step share slot r0, a
+  // static register index); the steps inside a slot are not
+#pragma unroll
+  for (int r0 = 0; r0 < DR; r0++) {
+#pragma unroll 1
+    for (int h = 0; h < RS / DQ; h++) {
+      const int p = RS * r0 + DQ * h, b = (r0 * (RS / DQ) + h) & 1;
+      const int wq = w - DQ * h, jq = j - p;  // my pivot row / column in this step, when in [0, DQ)
+      if (wq >= 0 && wq < DQ) prow[b][wq][j] = m[r0];
+      if (jq >= 0 && jq < DQ) {
+#pragma unroll
+        for (int r = 0; r < DR; r++) {
+          pcol[b][jq][w * DR + r] = m[r];
+          m[r] = 0.0;  // M[i][p + jq] <- 0 before the update below
+        }
+      }
+      __syncthreads();
+      // E = inverse of the pivot block (unpivoted: it is SPD)
+      double E[DQ][DQ];
+#pragma unroll
+      for (int a = 0; a < DQ; a++)
+#pragma unroll
+        for (int c = 0; c < DQ; c++) E[a][c] = prow[b][a][p + c];
+#pragma unroll
+      for (int q = 0; q < DQ; q++) {
+        const double rp = 1.0 / E[q][q];
+#pragma unroll
+        for (int c = 0; c < DQ; c++) E[q][c] = (c == q ? 1.0 : E[q][c]) * rp;
+#pragma unroll
+        for (int a = 0; a < DQ; a++) {
+          if (a == q) continue;
+          const double f = E[a][q];
+#pragma unroll
+          for (int c = 0; c < DQ; c++) E[a][c] = fma(-f, E[q][c], c == q ? 0.0 : E[a][c]);
+        }
+      }
+      // the new pivot rows at my column: E times the old ones, E itself on
+      // the pivot columns
+      double pr[DQ];
+#pragma unroll
+      for (int a = 0; a < DQ; a++) {
+        double s = 0.0, e = 0.0;
+#pragma unroll
+        for (int c = 0; c < DQ; c++) {
+          s = fma(E[a][c], prow[b][c][j], s);
+          e = (jq == c) ? E[a][c] : e;
+        }
+        pr[a] = (jq >= 0 && jq < DQ) ? e : s;
+      }
+      // every row minus its pivot-column entries times the new pivot rows,
+      // then the pivot rows replaced by the new ones
+#pragma unroll
+      for (int r = 0; r < DR; r++) {
+        double v = m[r];
+#pragma unroll
+        for (int a = 0; a < DQ; a++) v = fma(-pcol[b][a][w * DR + r], pr[a], v);
+        m[r] = v;
+      }
+      if (wq >= 0 && wq < DQ) {
+        double y = pr[0];
+#pragma unroll
+        for (int a = 1; a < DQ; a++) y = (wq == a) ? pr[a] : y;
+        m[r0] = y;
+      }
     }
-    __syncthreads();
   }
-  for (int e = t; e < GB * GB; e += 256) {
-    const double d = M[(e / GB) * GLD + e % GB];
-    A[(o + e / GB) * np + o + e % GB] = d;
-    dinv[e] = d;
+#pragma unroll
+  for (int r = 0; r < DR; r++) {
+    const int i = w + RS * r;
+    A[(o + i) * np + o + j] = m[r];
+    dinv[i * GB + j] = m[r];
   }
 }
 
-// one 64 x 64 block from global (row stride ld) into LDS (stride GLD)
-__device__ __forceinline__ void load_block(double *S, const double *G, int64_t ld) {
-  for (int e = threadIdx.x; e < GB * GB; e += 256) S[(e / GB) * GLD + e % GB] = G[(int64_t)(e / GB) * ld + e % GB];
-}
-
-// C (64 x 64, four 16 x 16 tiles per wavefront: rows 16w .. 16w+15) =
-// sign * X Y + C, X (stride ldx) and Y (stride ldy) read in place (global
-// memory or LDS).  Operand layout of v_mfma_f64_16x16x4f64 (measured on
-// gfx950): lane l supplies X[l%16][l/16] and Y[l/16][l%16] of the 16 x 4 /
-// 4 x 16 slices and holds C[l/16 + 4v][l%16], v = 0..3.  Wavefront w reads
-// rows 16w .. 16w+15 of X only.
-__device__ __forceinline__ void mfma_block(const double *X, int64_t ldx, const double *Y, int64_t ldy, double sign,
-                                           v4d acc[4]) {
-  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-  const int li = lane & 15, lk = lane >> 4;
-  for (int kk = 0; kk < GB; kk += 4) {
-    const double a = sign * X[(16 * w + li) * ldx + kk + lk];
-#pragma unroll
-    for (int tj = 0; tj < 4; tj++) {
-      const double b = Y[(kk + lk) * ldy + 16 * tj + li];
-      acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[tj], 0, 0, 0);
-    }
+// Operand layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane
+// l = 16 lk + li supplies X[li][lk] and Y[lk][li] of the 16 x 4 / 4 x 16
+// slices and holds C[lk + 4v][li], v = 0..3.  The kernels below give lane
+// (li, lk) the K indices lk * NQ + q, q = 0 .. NQ-1, so each lane's X values
+// are one contiguous run of a row; any assignment of K indices to slots gives
+// the same sum up to rounding order.
+//
+// Y (GB rows x 16 NT columns, row stride ld, global) into LDS in slot order:
+// S[(q * NT + tj) * 64 + 16 lk + li] = Y[lk * NQ + q][16 tj + li], which each
+// MFMA step reads as 64 consecutive doubles.
+template <int NT>
+__device__ __forceinline__ void stage_y(double *S, const double *Y, int64_t ld) {
+  for (int e = threadIdx.x; e < GB * 16 * NT; e += blockDim.x) {
+    const int r = e / (16 * NT), c = e % (16 * NT);
+    const int lk = r / NQ, q = r % NQ, tj = c / 16, li = c % 16;
+    S[(q * NT + tj) * 64 + 16 * lk + li] = Y[(int64_t)r * ld + c];
   }
 }
-__device__ __forceinline__ void store_block(const v4d acc[4], double *G, int64_t ld) {
-  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-  const int li = lane & 15, lk = lane >> 4;
+// this lane's run of X (the wave's 16 rows from X, row stride ld), times sign
+__device__ __forceinline__ void load_x(double a[NQ], const double *X, int64_t ld, double sign) {
+  const int lane = threadIdx.x & (WAVE - 1), li = lane & 15, lk = lane >> 4;
+  const double *x = X + (int64_t)li * ld + lk * NQ;
 #pragma unroll
-  for (int tj = 0; tj < 4; tj++)
-#pragma unroll
-    for (int v = 0; v < 4; v++) G[(int64_t)(16 * w + lk + 4 * v) * ld + 16 * tj + li] = acc[tj][v];
+  for (int q = 0; q < NQ; q++) a[q] = sign * x[q];
 }
-__device__ __forceinline__ void load_acc(v4d acc[4], const double *G, int64_t ld) {
-  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-  const int li = lane & 15, lk = lane >> 4;
+template <int NT>
+__device__ __forceinline__ void mfma_slots(const double a[NQ], const double *S, v4d acc[NT]) {
+  const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
-  for (int tj = 0; tj < 4; tj++)
+  for (int q = 0; q < NQ; q++)
 #pragma unroll
-    for (int v = 0; v < 4; v++) acc[tj][v] = G[(int64_t)(16 * w + lk + 4 * v) * ld + 16 * tj + li];
+    for (int tj = 0; tj < NT; tj++)
+      acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], S[(q * NT + tj) * 64 + lane], acc[tj], 0, 0, 0);
+}
+// the same products in groups of 8 slots that the scheduler keeps apart, so
+// it does not hoist every LDS read of Y to the top (and run out of registers)
+template <int NT>
+__device__ __forceinline__ void mfma_slots_grouped(const double a[NQ], const double *S, v4d acc[NT]) {
+  const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+  for (int g = 0; g < NQ; g += 8) {
+#pragma unroll
+    for (int q = g; q < g + 8; q++)
+#pragma unroll
+      for (int tj = 0; tj < NT; tj++)
+        acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], S[(q * NT + tj) * 64 + lane], acc[tj], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+// C (16 rows x 16 NT columns at G, row stride ld) from / to the accumulators
+template <int NT>
+__device__ __forceinline__ void load_c(v4d acc[NT], const double *G, int64_t ld) {
+  const int lane = threadIdx.x & (WAVE - 1), li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int tj = 0; tj < NT; tj++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) acc[tj][v] = G[(int64_t)(lk + 4 * v) * ld + 16 * tj + li];
+}
+template <int NT>
+__device__ __forceinline__ void store_c(const v4d acc[NT], double *G, int64_t ld) {
+  const int lane = threadIdx.x & (WAVE - 1), li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int tj = 0; tj < NT; tj++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) G[(int64_t)(lk + 4 * v) * ld + 16 * tj + li] = acc[tj][v];
 }
 
-// A_kj <- D A_kj for block column j = blockIdx.x (skipping k).  Every
-// wavefront reads all of A_kj, which the workgroup overwrites: A_kj is
-// staged in LDS first.
+// A_kj <- D A_kj for the TL-column tile blockIdx.x outside block column k:
+// the GB x TL panel is staged in LDS (the workgroup overwrites it), D is read
+// from dinv; wavefront w writes rows 32w .. 32w+31.
 __global__ void __launch_bounds__(256) k_gj_rowpanel(double *A, int64_t np, int k, const double *dinv) {
-  __shared__ double Y[GB * GLD];
-  const int j = blockIdx.x;
-  if (j == k) return;
-  double *Akj = A + (int64_t)k * GB * np + (int64_t)j * GB;
-  load_block(Y, Akj, np);
+  __shared__ double S[GB * TL];
+  const int jt = blockIdx.x;
+  if (jt / (GB / TL) == k) return;
+  double *P = A + (int64_t)k * GB * np + (int64_t)jt * TL;
+  stage_y<TL / 16>(S, P, np);
   __syncthreads();
-  v4d acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-  mfma_block(dinv, GB, Y, GLD, 1.0, acc);
-  store_block(acc, Akj, np);
+  const int w = threadIdx.x / WAVE;
+  for (int h = 0; h < GB / 64; h++) {
+    const int r0 = 16 * (GB / 16 / 4) * w + 16 * h;  // GB / 4 rows per wavefront
+    double a[NQ];
+    load_x(a, dinv + (int64_t)r0 * GB, GB, 1.0);
+    v4d acc[TL / 16] = {};
+    mfma_slots<TL / 16>(a, S, acc);
+    store_c<TL / 16>(acc, P + (int64_t)r0 * np, np);
+  }
 }
 
-// A_ij <- A_ij - A_ik A_kj for i, j != k (blockIdx.y = i, blockIdx.x = j):
-// operands read in place (this launch writes neither panel)
-__global__ void __launch_bounds__(256) k_gj_update(double *A, int64_t np, int k) {
-  const int i = blockIdx.y, j = blockIdx.x;
-  if (i == k || j == k) return;
-  const double *Aik = A + (int64_t)i * GB * np + (int64_t)k * GB;
-  const double *Akj = A + (int64_t)k * GB * np + (int64_t)j * GB;
-  double *Aij = A + (int64_t)i * GB * np + (int64_t)j * GB;
-  v4d acc[4];
-  load_acc(acc, Aij, np);
-  mfma_block(Aik, np, Akj, np, -1.0, acc);
-  store_block(acc, Aij, np);
+// A_ij <- A_ij - A_ik A_kj for the TL x TL tiles (it, jt) outside block row
+// and column k.  Workgroup (jt, s) keeps the GB x TL panel A_kj of column
+// tile jt in LDS and walks the row tiles it = s, s + nstrip, ...; wavefront
+// w updates rows 16w .. 16w+15 of each, loading the next tile's operands
+// while the matrix cores work on the current one.  The workgroups of one s
+// walk the same row tiles at about the same time, so the column panel A_ik
+// they read stays in L2.  This launch writes neither panel.
+__global__ void __launch_bounds__(256, 2) k_gj_update(double *A, int64_t np, int k, int nstrip) {
+  __shared__ double S[GB * TL];
+  constexpr int KT = GB / TL;  // tiles per block
+  const int jt = blockIdx.x, nt = (int)(np / TL);
+  if (jt / KT == k) return;
+  auto next_tile = [&](int it) {
+    while (it < nt && it / KT == k) it += nstrip;
+    return it;
+  };
+  int it = next_tile(blockIdx.y);
+  if (it >= nt) return;
+  stage_y<TL / 16>(S, A + (int64_t)k * GB * np + (int64_t)jt * TL, np);
+  __syncthreads();
+  const int w = threadIdx.x / WAVE;
+  const double *Xk = A + (int64_t)k * GB + (int64_t)16 * w * np;  // column panel, wavefront rows
+  double *Cj = A + (int64_t)jt * TL + (int64_t)16 * w * np;
+  // the next tile's X run and C tile are loaded before the current tile's
+  // products: loads complete in order, so anything issued later would hold
+  // up the wait for them
+  double a[NQ];
+  v4d acc[TL / 16];
+  load_x(a, Xk + (int64_t)it * TL * np, np, -1.0);
+  load_c<TL / 16>(acc, Cj + (int64_t)it * TL * np, np);
+  for (;;) {
+    const int nx = next_tile(it + nstrip);
+    double an[NQ];
+    v4d accn[TL / 16];
+    if (nx < nt) {
+      load_x(an, Xk + (int64_t)nx * TL * np, np, -1.0);
+      load_c<TL / 16>(accn, Cj + (int64_t)nx * TL * np, np);
+    }
+    mfma_slots_grouped<TL / 16>(a, S, acc);
+    store_c<TL / 16>(acc, Cj + (int64_t)it * TL * np, np);
+    if (nx >= nt) break;
+    it = nx;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) a[q] = an[q];
+#pragma unroll
+    for (int tj = 0; tj < TL / 16; tj++) acc[tj] = accn[tj];
+  }
 }
 
-// A_ik <- -A_ik D for block row i = blockIdx.x (skipping k): wavefront w
-// reads and then writes rows 16w .. 16w+15 of A_ik only
+// A_ik <- -A_ik D for the TL-row tile blockIdx.x outside block row k, D
+// staged in LDS: wavefront w reads and then writes rows 16w .. 16w+15 of the
+// TL x GB panel only.
 __global__ void __launch_bounds__(256) k_gj_colpanel(double *A, int64_t np, int k, const double *dinv) {
-  const int i = blockIdx.x;
-  if (i == k) return;
-  double *Aik = A + (int64_t)i * GB * np + (int64_t)k * GB;
-  v4d acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-  mfma_block(Aik, np, dinv, GB, -1.0, acc);
-  store_block(acc, Aik, np);
+  __shared__ double S[GB * GB];
+  const int it = blockIdx.x;
+  if (it / (GB / TL) == k) return;
+  stage_y<GB / 16>(S, dinv, GB);
+  __syncthreads();
+  const int w = threadIdx.x / WAVE;
+  double *X = A + (int64_t)(it * TL + 16 * w) * np + (int64_t)k * GB;
+  double a[NQ];
+  load_x(a, X, np, -1.0);
+  v4d acc[GB / 16] = {};
+  mfma_slots<GB / 16>(a, S, acc);
+  store_c<GB / 16>(acc, X, np);
 }
 
 // The SPD matrix of the coarse level, padded to np x np: the stored entries
@@ -249,15 +388,23 @@ void coarse_pinv(int64_t n, const std::vector<int32_t> &rp, const std::vector<in
     LG_HIP(hipEventCreate(&e1));
     LG_HIP(hipEventRecord(e0, s));
   }
+  const int nt = (int)(np / TL), KT = GB / TL, mt = nt - KT;
+  // row strips of the update: as many as keep every workgroup resident at
+  // once, two per CU (the update's LDS and registers allow two).  A second
+  // round of workgroups measured slower: the rounds no longer walk the same
+  // rows of the column panel together.
+  int dev = 0, cus = 0;
+  LG_HIP(hipGetDevice(&dev));
+  LG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int nstrip = (int)std::max<int64_t>(1, std::min<int64_t>(mt, 2 * (int64_t)std::max(cus, 1) / nt));
   for (int k = 0; k < nb; k++) {
-    hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(256), 0, s, A.p, np, k, dinv.p);
+    hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(DT), 0, s, A.p, np, k, dinv.p);
     LG_LAUNCH_CHECK();
-    if (nb == 1) break;
-    hipLaunchKernelGGL(k_gj_rowpanel, dim3((unsigned)nb), dim3(256), 0, s, A.p, np, k, (const double *)dinv.p);
+    hipLaunchKernelGGL(k_gj_rowpanel, dim3((unsigned)nt), dim3(256), 0, s, A.p, np, k, (const double *)dinv.p);
     LG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gj_update, dim3((unsigned)nb, (unsigned)nb), dim3(256), 0, s, A.p, np, k);
+    hipLaunchKernelGGL(k_gj_update, dim3((unsigned)nt, (unsigned)nstrip), dim3(256), 0, s, A.p, np, k, nstrip);
     LG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gj_colpanel, dim3((unsigned)nb), dim3(256), 0, s, A.p, np, k, (const double *)dinv.p);
+    hipLaunchKernelGGL(k_gj_colpanel, dim3((unsigned)nt), dim3(256), 0, s, A.p, np, k, (const double *)dinv.p);
     LG_LAUNCH_CHECK();
   }
   if (gj_ms) {

@@ -97,6 +97,8 @@ constexpr int64_t COARSE_DENSE_MAX = 8192;
 // Cholesky, dense_pinv_host) and on the device from there (blocked
 // Gauss-Jordan on the matrix cores); the result is a device n x n array.
 constexpr int64_t COARSE_DEVICE_MIN = 256;
+// block width of the device Gauss-Jordan (the matrix is padded to a multiple)
+constexpr int COARSE_GJ_BLOCK = 128;
 // gj_ms (optional): milliseconds of the device inversion kernels (0 on the
 // host path).
 void coarse_pinv(int64_t n, const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,

@@ -2673,8 +2673,9 @@ int ligmg_bench_pinv(ligmg_handle h, double *total_ms, double *gj_ms, double *fl
     const auto t0 = std::chrono::steady_clock::now();
     lg::coarse_pinv(n, rp, ci, v, pinv, S.s, gj_ms);
     *total_ms = 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    const double np = (double)((n + 63) / 64 * 64);
-    *flops = (n >= lg::COARSE_DEVICE_MIN) ? 2.0 * np * np * np : 0.0;
+    // Gauss-Jordan inversion of the n x n matrix: 2 n^3 flops (the padding's
+    // extra work is not counted)
+    *flops = (n >= lg::COARSE_DEVICE_MIN) ? 2.0 * (double)n * n * n : 0.0;
     return LIGMG_OK;
   } catch (const lg::Error &e) {
     return fail(e);
