@@ -331,8 +331,9 @@ int ligmg_check_units(int64_t n, const int32_t *rp, int32_t *same, int64_t *nuni
  * (on the device from 256 vertices: blocked Gauss-Jordan on the matrix
  * cores) and reports the wall milliseconds of the whole step (host matrix
  * assembly, upload, inversion), the milliseconds of the device inversion
- * kernels (HIP events; 0 on the host path) and their floating-point
- * operations, 2 * n_p^3 for n_p = n_c rounded up to a multiple of 64. */
+ * kernels (HIP events; 0 on the host path) and the floating-point
+ * operations of inverting an n_c x n_c matrix, 2 * n_c^3 (the padding to a
+ * multiple of the block width is not counted; 0 on the host path). */
 int ligmg_bench_pinv(ligmg_handle h, double *total_ms, double *gj_ms, double *flops);
 
 /* (new) Diagnostic: Y = P X with the coarsest level's dense pseudo-inverse
