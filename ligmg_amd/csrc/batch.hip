@@ -399,15 +399,23 @@ __global__ void __launch_bounds__(256) k_bred1(int np, int kp, int lk, const T *
   }
 }
 
-__global__ void k_bred2(int np, int nb, int kp, const int32_t *bseg, const double *part,
-                        double *out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nb * kp) return;
-  const int blk = t / kp, c = t % kp;
-  for (int j = 0; j < np; j++) {
-    double s = 0;
-    for (int g = bseg[blk]; g < bseg[blk + 1]; g++) s += part[((int64_t)g * np + j) * kp + c];
-    out[((int64_t)blk * np + j) * kp + c] = s;
+// out[blk, j, c] = sum of the segment partials of block blk: one workgroup
+// per (block, j); thread (g, c) adds segments g, g + G, ... (G = 256 / kp)
+// in order, and the G partials of a column are added in g order.
+__global__ void __launch_bounds__(256) k_bred2(int np, int kp, int lk, const int32_t *bseg, const double *part,
+                                               double *out) {
+  const int blk = blockIdx.x, j = blockIdx.y;
+  const int G = 256 >> lk;
+  const int g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
+  double acc = 0;
+  for (int sg = bseg[blk] + g; sg < bseg[blk + 1]; sg += G) acc += part[((int64_t)sg * np + j) * kp + c];
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (g == 0) {
+    double t = 0;
+    for (int gg = 0; gg < G; gg++) t += red[gg * kp + c];
+    out[((int64_t)blk * np + j) * kp + c] = t;
   }
 }
 
@@ -532,7 +540,8 @@ void RedPlan::bdots(int np, int kp, const T *a0, const T *b0, const T *a1, const
   hipLaunchKernelGGL(k_bred1<T>, dim3(nseg), dim3(256), 0, s, np, kp, log2i(kp), a0, b0, a1, b1, seg_lo.p,
                      seg_hi.p, bpart.p);
   LG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bred2, dim3((nb * kp + 63) / 64), dim3(64), 0, s, np, nb, kp, bseg.p, bpart.p, out);
+  hipLaunchKernelGGL(k_bred2, dim3((unsigned)nb, (unsigned)np), dim3(256), 0, s, np, kp, log2i(kp), bseg.p, bpart.p,
+                     out);
   LG_LAUNCH_CHECK();
 }
 
