@@ -431,6 +431,18 @@ __global__ void __launch_bounds__(256) k_bsub_mean(T *a, int kp, int lk, const i
   for (int64_t i = lo[s] + g; i < hi[s]; i += G) a[i * kp + c] -= m;
 }
 
+__global__ void __launch_bounds__(256) k_bsub_mean_f2d(double *z, const float *cv, int kp, int lk,
+                                                       const int64_t *lo, const int64_t *hi,
+                                                       const int32_t *segblk, const int64_t *bptr,
+                                                       const double *sums) {
+  const int s = blockIdx.x;
+  const int blk = segblk[s];
+  const int G = 256 >> lk;
+  const int g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
+  const double m = sums[(int64_t)blk * kp + c] / (double)(bptr[blk + 1] - bptr[blk]);
+  for (int64_t i = lo[s] + g; i < hi[s]; i += G) z[i * kp + c] = (double)cv[i * kp + c] - m;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -542,6 +554,21 @@ void RedPlan::bdots(int np, int kp, const T *a0, const T *b0, const T *a1, const
   LG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_bred2, dim3((unsigned)nb, (unsigned)np), dim3(256), 0, s, np, kp, log2i(kp), bseg.p, bpart.p,
                      out);
+  LG_LAUNCH_CHECK();
+}
+
+void RedPlan::bdots_finish(int np, int kp, double *out, const DevBuf<double> &bpart, hipStream_t s) const {
+  if (nseg == 0) return;
+  LG_CHECK(bpart.n >= (size_t)nseg * np * kp, LIGMG_ERR_INVALID, "block reduction buffer too small");
+  hipLaunchKernelGGL(k_bred2, dim3((unsigned)nb, (unsigned)np), dim3(256), 0, s, np, kp, log2i(kp), bseg.p, bpart.p,
+                     out);
+  LG_LAUNCH_CHECK();
+}
+
+void op_bsub_block_mean_f2d(double *z, const float *c, int kp, const RedPlan &P, const double *sums, hipStream_t s) {
+  if (P.nseg == 0) return;
+  hipLaunchKernelGGL(k_bsub_mean_f2d, dim3(P.nseg), dim3(256), 0, s, z, c, kp, log2i(kp), P.seg_lo.p, P.seg_hi.p,
+                     P.bseg.p + P.nb + 1, P.bptr.p, sums);
   LG_LAUNCH_CHECK();
 }
 

@@ -77,6 +77,9 @@ struct RedPlan {
   template <class T>
   void bdots(int np, int kp, const T *a0, const T *b0, const T *a1, const T *b1, double *out,
              DevBuf<double> &bpart, hipStream_t s) const;
+  // the second stage of bdots alone, from segment partials a fused kernel
+  // wrote to bpart in bdots' layout
+  void bdots_finish(int np, int kp, double *out, const DevBuf<double> &bpart, hipStream_t s) const;
 };
 
 // a -= mean of a over each block, means taken from sums[c] (device)
@@ -108,6 +111,9 @@ template <class T>
 void op_bdense_apply(const double *Pm, int64_t n, const T *X, T *Y, int kp, hipStream_t s);
 template <class T>
 void op_bsub_block_mean(T *a, int kp, const RedPlan &P, const double *sums, hipStream_t s);
+// z = c (FP32) minus the block means of c from sums, in FP64: the same bits as
+// converting c and then subtracting the means
+void op_bsub_block_mean_f2d(double *z, const float *c, int kp, const RedPlan &P, const double *sums, hipStream_t s);
 // column-major n x k caller block <-> row-major n x kp internal block; the
 // caller's vertex of internal row i is perm[i] (identity when perm is null)
 void op_to_block(const double *B, int64_t n, int k, int kp, double *X, const int32_t *perm, hipStream_t s);

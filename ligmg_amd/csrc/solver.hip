@@ -108,19 +108,39 @@ __global__ void k_kr_bstore(int nb, int kp, int np, const double *out, int s0, i
   sc[t * SC_N + s0] = out[((int64_t)blk * np) * kp + c];
   if (np > 1) sc[t * SC_N + s1] = out[((int64_t)blk * np + 1) * kp + c];
 }
+// x += alpha p, r -= alpha q per (block, column); alpha = 0 marks a finished
+// column, whose x and r stay.  Also writes the segment partials of r.r
+// exactly as the block dots form them (part[s * kp + c]: the same loop and
+// combining order), and r in FP32 when r32 is set (the mixed outer
+// preconditioner's input).
 template <class T>
 __global__ void __launch_bounds__(256) k_kr_bupdate(T *x, T *r, const T *p, const T *q, int kp,
                                                     int lk, const int64_t *lo, const int64_t *hi,
-                                                    const int32_t *segblk, const double *sc) {
+                                                    const int32_t *segblk, const double *sc, double *part,
+                                                    float *r32) {
   const int s = blockIdx.x;
   const int G = 256 >> lk, g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
   const double a = sc[((int64_t)segblk[s] * kp + c) * SC_N + SC_ALPHA];
-  if (a == 0.0) return;
   const T at = (T)a;
+  double acc = 0;
   for (int64_t i = lo[s] + g; i < hi[s]; i += G) {
     const int64_t o = i * kp + c;
-    x[o] += at * p[o];
-    r[o] -= at * q[o];
+    T rv = r[o];
+    if (a != 0.0) {
+      x[o] += at * p[o];
+      rv -= at * q[o];
+      r[o] = rv;
+    }
+    acc += (double)rv * (double)rv;
+    if (r32) r32[o] = (float)rv;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (g == 0) {
+    double t = 0;
+    for (int gg = 0; gg < G; gg++) t += red[gg * kp + c];
+    part[(int64_t)s * kp + c] = t;
   }
 }
 template <class T>
@@ -1738,15 +1758,22 @@ struct Solver {
     return K.x.p;
   }
 
+  // z = (cycle applied to r) minus its block means.  r32_ready: the FP32
+  // copy of r for the mixed outer preconditioner is already in place.
   template <class T>
-  void bprec(size_t lvl, const T *r, T *z, int kp, bool outer_prec) {
+  void bprec(size_t lvl, const T *r, T *z, int kp, bool outer_prec, bool r32_ready = false) {
     const int64_t n = H[lvl].A.n;
     if constexpr (std::is_same<T, double>::value) {
       if (outer_prec && mixed) {
+        // the block sums come from the FP32 result itself (the FP64 copy has
+        // the same values), and one pass converts and subtracts the means
         BLevelWork<float> &w = BW32[lvl];
-        op_d2f(r, w.b.p, n * kp, s);
+        if (!r32_ready) op_d2f(r, w.b.p, n * kp, s);
         const float *c = bcycle<float>(lvl, w.b.p, kp);
-        op_f2d(c, z, n * kp, s);
+        const RedPlan &R = red[lvl];
+        R.bdots<float>(1, kp, c, nullptr, nullptr, nullptr, bsc_out.p, bpart_red, s);
+        op_bsub_block_mean_f2d(z, c, kp, R, bsc_out.p, s);
+        return;
       } else {
         d2d(z, bcycle<T>(lvl, r, kp), n * kp, s);
       }
@@ -1800,11 +1827,19 @@ struct Solver {
       if (fcg) bdot<T>(R, kp, K.p.p, K.q.p, K.p.p, K.r.p, K, SC_DELTA, SC_T1);
       else bdot<T>(R, kp, K.p.p, K.q.p, (const T *)nullptr, (const T *)nullptr, K, SC_DELTA, SC_DELTA);
       hipLaunchKernelGGL(k_kr_alpha, grid1, dim3(64), 0, s, ns, fcg ? 1 : 0, K.sc.p);
+      // the update also forms r.r's segment partials and, for the mixed
+      // outer preconditioner, the FP32 copy of r
+      float *r32 = nullptr;
+      if constexpr (std::is_same<T, double>::value)
+        if (outer_prec && mixed) r32 = BW32[lvl].b.p;
+      LG_CHECK(bpart_red.n >= (size_t)R.nseg * kp, LIGMG_ERR_INVALID, "block reduction buffer too small");
       hipLaunchKernelGGL(k_kr_bupdate<T>, dim3(R.nseg), dim3(256), 0, s, x, K.r.p, K.p.p, K.q.p, kp, lk,
-                         R.seg_lo.p, R.seg_hi.p, R.bseg.p + R.nb + 1, K.sc.p);
+                         R.seg_lo.p, R.seg_hi.p, R.bseg.p + R.nb + 1, K.sc.p, bpart_red.p, r32);
       LG_LAUNCH_CHECK();
       it++;
-      bdot<T>(R, kp, K.r.p, K.r.p, (const T *)nullptr, (const T *)nullptr, K, SC_RN2, SC_RN2);
+      R.bdots_finish(1, kp, K.out.p, bpart_red, s);
+      hipLaunchKernelGGL(k_kr_bstore, dim3((R.nb * kp + 63) / 64), dim3(64), 0, s, R.nb, kp, 1, K.out.p, SC_RN2,
+                         SC_RN2, K.sc.p);
       hipLaunchKernelGGL(k_kr_check, grid1, dim3(64), 0, s, ns, rtol, K.sc.p);
       if (host_check) {
         hsc = download(K.sc.p, (size_t)ns * SC_N, s);
@@ -1823,7 +1858,7 @@ struct Solver {
         if (!any_active(hsc, ns)) return it;
       }
       if (it >= maxit) break;
-      bprec<T>(lvl, K.r.p, K.z.p, kp, outer_prec);
+      bprec<T>(lvl, K.r.p, K.z.p, kp, outer_prec, true);
       if (fcg) bdot<T>(R, kp, K.z.p, K.q.p, (const T *)nullptr, (const T *)nullptr, K, SC_T1, SC_T1);
       else bdot<T>(R, kp, K.r.p, K.z.p, (const T *)nullptr, (const T *)nullptr, K, SC_GN, SC_GN);
       hipLaunchKernelGGL(k_kr_beta, grid1, dim3(64), 0, s, ns, fcg ? 1 : 0, K.sc.p);
