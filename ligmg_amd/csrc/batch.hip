@@ -431,16 +431,33 @@ __global__ void __launch_bounds__(256) k_bsub_mean(T *a, int kp, int lk, const i
   for (int64_t i = lo[s] + g; i < hi[s]; i += G) a[i * kp + c] -= m;
 }
 
+// z = c - (block mean), in FP64; with w set, also the segment partials of
+// z.w exactly as the block dots form them (part[s * kp + c])
 __global__ void __launch_bounds__(256) k_bsub_mean_f2d(double *z, const float *cv, int kp, int lk,
                                                        const int64_t *lo, const int64_t *hi,
                                                        const int32_t *segblk, const int64_t *bptr,
-                                                       const double *sums) {
+                                                       const double *sums, const double *w, double *part) {
   const int s = blockIdx.x;
   const int blk = segblk[s];
   const int G = 256 >> lk;
   const int g = threadIdx.x >> lk, c = threadIdx.x & (kp - 1);
   const double m = sums[(int64_t)blk * kp + c] / (double)(bptr[blk + 1] - bptr[blk]);
-  for (int64_t i = lo[s] + g; i < hi[s]; i += G) z[i * kp + c] = (double)cv[i * kp + c] - m;
+  double acc = 0;
+  for (int64_t i = lo[s] + g; i < hi[s]; i += G) {
+    const int64_t o = i * kp + c;
+    const double zv = (double)cv[o] - m;
+    z[o] = zv;
+    if (w) acc += zv * w[o];
+  }
+  if (!w) return;  // uniform over the workgroup
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (g == 0) {
+    double t = 0;
+    for (int gg = 0; gg < G; gg++) t += red[gg * kp + c];
+    part[(int64_t)s * kp + c] = t;
+  }
 }
 
 }  // namespace
@@ -565,10 +582,12 @@ void RedPlan::bdots_finish(int np, int kp, double *out, const DevBuf<double> &bp
   LG_LAUNCH_CHECK();
 }
 
-void op_bsub_block_mean_f2d(double *z, const float *c, int kp, const RedPlan &P, const double *sums, hipStream_t s) {
+void op_bsub_block_mean_f2d(double *z, const float *c, int kp, const RedPlan &P, const double *sums, hipStream_t s,
+                            const double *w, DevBuf<double> *bpart) {
   if (P.nseg == 0) return;
+  if (w) LG_CHECK(bpart && bpart->n >= (size_t)P.nseg * kp, LIGMG_ERR_INVALID, "block reduction buffer too small");
   hipLaunchKernelGGL(k_bsub_mean_f2d, dim3(P.nseg), dim3(256), 0, s, z, c, kp, log2i(kp), P.seg_lo.p, P.seg_hi.p,
-                     P.bseg.p + P.nb + 1, P.bptr.p, sums);
+                     P.bseg.p + P.nb + 1, P.bptr.p, sums, w, w ? bpart->p : nullptr);
   LG_LAUNCH_CHECK();
 }
 

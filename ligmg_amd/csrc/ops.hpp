@@ -112,8 +112,11 @@ void op_bdense_apply(const double *Pm, int64_t n, const T *X, T *Y, int kp, hipS
 template <class T>
 void op_bsub_block_mean(T *a, int kp, const RedPlan &P, const double *sums, hipStream_t s);
 // z = c (FP32) minus the block means of c from sums, in FP64: the same bits as
-// converting c and then subtracting the means
-void op_bsub_block_mean_f2d(double *z, const float *c, int kp, const RedPlan &P, const double *sums, hipStream_t s);
+// converting c and then subtracting the means.  With w set, also the segment
+// partials of z.w in bpart, as RedPlan::bdots forms them (finish them with
+// bdots_finish).
+void op_bsub_block_mean_f2d(double *z, const float *c, int kp, const RedPlan &P, const double *sums, hipStream_t s,
+                            const double *w = nullptr, DevBuf<double> *bpart = nullptr);
 // column-major n x k caller block <-> row-major n x kp internal block; the
 // caller's vertex of internal row i is perm[i] (identity when perm is null)
 void op_to_block(const double *B, int64_t n, int k, int kp, double *X, const int32_t *perm, hipStream_t s);

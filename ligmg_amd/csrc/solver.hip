@@ -1759,21 +1759,24 @@ struct Solver {
   }
 
   // z = (cycle applied to r) minus its block means.  r32_ready: the FP32
-  // copy of r for the mixed outer preconditioner is already in place.
+  // copy of r for the mixed outer preconditioner is already in place.  On
+  // that path, with w set, the segment partials of z.w are also left in
+  // bpart_red (returns true); otherwise returns false.
   template <class T>
-  void bprec(size_t lvl, const T *r, T *z, int kp, bool outer_prec, bool r32_ready = false) {
+  bool bprec(size_t lvl, const T *r, T *z, int kp, bool outer_prec, bool r32_ready = false,
+             const T *w = nullptr) {
     const int64_t n = H[lvl].A.n;
     if constexpr (std::is_same<T, double>::value) {
       if (outer_prec && mixed) {
         // the block sums come from the FP32 result itself (the FP64 copy has
         // the same values), and one pass converts and subtracts the means
-        BLevelWork<float> &w = BW32[lvl];
-        if (!r32_ready) op_d2f(r, w.b.p, n * kp, s);
-        const float *c = bcycle<float>(lvl, w.b.p, kp);
+        BLevelWork<float> &bw = BW32[lvl];
+        if (!r32_ready) op_d2f(r, bw.b.p, n * kp, s);
+        const float *c = bcycle<float>(lvl, bw.b.p, kp);
         const RedPlan &R = red[lvl];
         R.bdots<float>(1, kp, c, nullptr, nullptr, nullptr, bsc_out.p, bpart_red, s);
-        op_bsub_block_mean_f2d(z, c, kp, R, bsc_out.p, s);
-        return;
+        op_bsub_block_mean_f2d(z, c, kp, R, bsc_out.p, s, w, &bpart_red);
+        return w != nullptr;
       } else {
         d2d(z, bcycle<T>(lvl, r, kp), n * kp, s);
       }
@@ -1783,6 +1786,7 @@ struct Solver {
     const RedPlan &R = red[lvl];
     R.bdots<T>(1, kp, z, nullptr, nullptr, nullptr, bsc_out.p, bpart_red, s);
     op_bsub_block_mean<T>(z, kp, R, bsc_out.p, s);
+    return false;
   }
 
   template <class T>
@@ -1858,9 +1862,17 @@ struct Solver {
         if (!any_active(hsc, ns)) return it;
       }
       if (it >= maxit) break;
-      bprec<T>(lvl, K.r.p, K.z.p, kp, outer_prec, true);
-      if (fcg) bdot<T>(R, kp, K.z.p, K.q.p, (const T *)nullptr, (const T *)nullptr, K, SC_T1, SC_T1);
-      else bdot<T>(R, kp, K.r.p, K.z.p, (const T *)nullptr, (const T *)nullptr, K, SC_GN, SC_GN);
+      // z.q (FCG) or r.z (CG): formed by the preconditioner's last pass
+      // where it can, else a block dot
+      if (bprec<T>(lvl, K.r.p, K.z.p, kp, outer_prec, true, fcg ? K.q.p : K.r.p)) {
+        R.bdots_finish(1, kp, K.out.p, bpart_red, s);
+        hipLaunchKernelGGL(k_kr_bstore, dim3((R.nb * kp + 63) / 64), dim3(64), 0, s, R.nb, kp, 1, K.out.p,
+                           fcg ? SC_T1 : SC_GN, fcg ? SC_T1 : SC_GN, K.sc.p);
+      } else if (fcg) {
+        bdot<T>(R, kp, K.z.p, K.q.p, (const T *)nullptr, (const T *)nullptr, K, SC_T1, SC_T1);
+      } else {
+        bdot<T>(R, kp, K.r.p, K.z.p, (const T *)nullptr, (const T *)nullptr, K, SC_GN, SC_GN);
+      }
       hipLaunchKernelGGL(k_kr_beta, grid1, dim3(64), 0, s, ns, fcg ? 1 : 0, K.sc.p);
       hipLaunchKernelGGL(k_kr_bpupdate<T>, dim3(R.nseg), dim3(256), 0, s, K.p.p, K.z.p, kp, lk,
                          R.seg_lo.p, R.seg_hi.p, R.bseg.p + R.nb + 1, K.sc.p);
