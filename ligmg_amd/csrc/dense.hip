@@ -216,23 +216,27 @@ __global__ void __launch_bounds__(256) k_gj_rowpanel(double *A, int64_t np, int 
 }
 
 // A_ij <- A_ij - A_ik A_kj for the TL x TL tiles (it, jt) outside block row
-// and column k.  Workgroup (jt, s) keeps the GB x TL panel A_kj of column
-// tile jt in LDS and walks the row tiles it = s, s + nstrip, ...; wavefront
-// w updates rows 16w .. 16w+15 of each, loading the next tile's operands
-// while the matrix cores work on the current one.  The workgroups of one s
-// walk the same row tiles at about the same time, so the column panel A_ik
-// they read stays in L2.  This launch writes neither panel.
-__global__ void __launch_bounds__(256, 2) k_gj_update(double *A, int64_t np, int k, int nstrip) {
+// and column k with jt = c0 + blockIdx.x and it = r0 + blockIdx.y + s rstep
+// < r1, leaving out the tiles of block (la, la) when la >= 0.  The workgroup
+// keeps the GB x TL panel A_kj of column tile jt in LDS and walks its row
+// tiles; wavefront w updates rows 16w .. 16w+15 of each, loading the next
+// tile's operands while the matrix cores work on the current one.  The
+// workgroups of one blockIdx.y walk the same row tiles at about the same
+// time, so the column panel A_ik they read stays in L2.  This launch writes
+// neither panel.
+__global__ void __launch_bounds__(256, 2) k_gj_update(double *A, int64_t np, int k, int c0, int r0, int r1,
+                                                      int rstep, int la) {
   __shared__ double S[GB * TL];
   constexpr int KT = GB / TL;  // tiles per block
-  const int jt = blockIdx.x, nt = (int)(np / TL);
+  const int jt = c0 + blockIdx.x;
   if (jt / KT == k) return;
+  const bool colla = la >= 0 && jt / KT == la;
   auto next_tile = [&](int it) {
-    while (it < nt && it / KT == k) it += nstrip;
+    while (it < r1 && (it / KT == k || (colla && it / KT == la))) it += rstep;
     return it;
   };
-  int it = next_tile(blockIdx.y);
-  if (it >= nt) return;
+  int it = next_tile(r0 + blockIdx.y);
+  if (it >= r1) return;
   stage_y<TL / 16>(S, A + (int64_t)k * GB * np + (int64_t)jt * TL, np);
   __syncthreads();
   const int w = threadIdx.x / WAVE;
@@ -246,16 +250,16 @@ __global__ void __launch_bounds__(256, 2) k_gj_update(double *A, int64_t np, int
   load_x(a, Xk + (int64_t)it * TL * np, np, -1.0);
   load_c<TL / 16>(acc, Cj + (int64_t)it * TL * np, np);
   for (;;) {
-    const int nx = next_tile(it + nstrip);
+    const int nx = next_tile(it + rstep);
     double an[NQ];
     v4d accn[TL / 16];
-    if (nx < nt) {
+    if (nx < r1) {
       load_x(an, Xk + (int64_t)nx * TL * np, np, -1.0);
       load_c<TL / 16>(accn, Cj + (int64_t)nx * TL * np, np);
     }
     mfma_slots_grouped<TL / 16>(a, S, acc);
     store_c<TL / 16>(acc, Cj + (int64_t)it * TL * np, np);
-    if (nx >= nt) break;
+    if (nx >= r1) break;
     it = nx;
 #pragma unroll
     for (int q = 0; q < NQ; q++) a[q] = an[q];
@@ -397,14 +401,53 @@ void coarse_pinv(int64_t n, const std::vector<int32_t> &rp, const std::vector<in
   LG_HIP(hipGetDevice(&dev));
   LG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int nstrip = (int)std::max<int64_t>(1, std::min<int64_t>(mt, 2 * (int64_t)std::max(cus, 1) / nt));
+  // Look-ahead: while step k's trailing update runs on this stream, a
+  // second stream updates block (k+1, k+1) alone and inverts it.  The
+  // diagonal kernel's workgroup (8 wavefronts, 104 registers each) fits on a
+  // CU beside one update workgroup, and the update leaves such CUs.  The two
+  // D buffers alternate with k.
+  DevBuf<double> dinv2(GB * GB);
+  double *dbuf[2] = {dinv.p, dinv2.p};
+  struct Side {
+    hipStream_t st = nullptr;
+    hipEvent_t rp = nullptr, blk = nullptr, diag = nullptr;
+    ~Side() {
+      if (rp) (void)hipEventDestroy(rp);
+      if (blk) (void)hipEventDestroy(blk);
+      if (diag) (void)hipEventDestroy(diag);
+      if (st) (void)hipStreamDestroy(st);
+    }
+  } side;
+  LG_HIP(hipStreamCreateWithFlags(&side.st, hipStreamNonBlocking));
+  LG_HIP(hipEventCreateWithFlags(&side.rp, hipEventDisableTiming));
+  LG_HIP(hipEventCreateWithFlags(&side.blk, hipEventDisableTiming));
+  LG_HIP(hipEventCreateWithFlags(&side.diag, hipEventDisableTiming));
+  hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(DT), 0, s, A.p, np, 0, dbuf[0]);
+  LG_LAUNCH_CHECK();
   for (int k = 0; k < nb; k++) {
-    hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(DT), 0, s, A.p, np, k, dinv.p);
+    const double *D = dbuf[k & 1];
+    const int la = (k + 1 < nb) ? k + 1 : -1;
+    if (k > 0) LG_HIP(hipStreamWaitEvent(s, side.diag, 0));
+    hipLaunchKernelGGL(k_gj_rowpanel, dim3((unsigned)nt), dim3(256), 0, s, A.p, np, k, D);
     LG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gj_rowpanel, dim3((unsigned)nt), dim3(256), 0, s, A.p, np, k, (const double *)dinv.p);
+    if (la >= 0) {
+      // side: block (la, la) of step k's update, then its inverse
+      LG_HIP(hipEventRecord(side.rp, s));
+      LG_HIP(hipStreamWaitEvent(side.st, side.rp, 0));
+      hipLaunchKernelGGL(k_gj_update, dim3((unsigned)KT), dim3(256), 0, side.st, A.p, np, k, la * KT, la * KT,
+                         la * KT + KT, 1, -1);
+      LG_LAUNCH_CHECK();
+      LG_HIP(hipEventRecord(side.blk, side.st));
+      hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(DT), 0, side.st, A.p, np, la, dbuf[la & 1]);
+      LG_LAUNCH_CHECK();
+      LG_HIP(hipEventRecord(side.diag, side.st));
+    }
+    hipLaunchKernelGGL(k_gj_update, dim3((unsigned)nt, (unsigned)nstrip), dim3(256), 0, s, A.p, np, k, 0, 0, nt,
+                       nstrip, la);
     LG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gj_update, dim3((unsigned)nt, (unsigned)nstrip), dim3(256), 0, s, A.p, np, k, nstrip);
-    LG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gj_colpanel, dim3((unsigned)nt), dim3(256), 0, s, A.p, np, k, (const double *)dinv.p);
+    // the column panel overwrites A_(la)k, which the side's block update reads
+    if (la >= 0) LG_HIP(hipStreamWaitEvent(s, side.blk, 0));
+    hipLaunchKernelGGL(k_gj_colpanel, dim3((unsigned)nt), dim3(256), 0, s, A.p, np, k, D);
     LG_LAUNCH_CHECK();
   }
   if (gj_ms) {
