@@ -342,6 +342,13 @@ int ligmg_bench_pinv(ligmg_handle h, double *total_ms, double *gj_ms, double *fl
  * columns).  X and Y are host arrays, n_c x k, row-major. */
 int ligmg_coarse_apply_block(ligmg_handle h, int32_t k, const double *X, double *Y);
 
+/* (new) Diagnostic: the dense pseudo-inverse that setup forms for a
+ * coarsest level, here for any graph Laplacian given as CSR in host memory
+ * (n <= 8192 vertices; symmetric, every column in [0, n)): the device
+ * blocked Gauss-Jordan from 256 vertices, the host Cholesky below.  Writes
+ * the n x n result row-major to the host array pinv. */
+int ligmg_dense_pinv(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, double *pinv);
+
 /* z = M r: one cycle of the preconditioner from level `level` down (the
  * V- or K-cycle of multigrid.hpp:86-130 with a zero initial guess) in the
  * given precision.  r and z are host arrays of length n in FP64.  Mirrors

@@ -117,6 +117,7 @@ SOLVER_PROTOS = {
     "ligmg_check_units": (C.c_int, [i64, P(i32), P(i32), P(i64)]),
     "ligmg_coarse_apply_block": (C.c_int, [HANDLE, i32, P(f64), P(f64)]),
     "ligmg_bench_pinv": (C.c_int, [HANDLE, P(f64), P(f64), P(f64)]),
+    "ligmg_dense_pinv": (C.c_int, [i64, P(i32), P(i32), P(f64), P(f64)]),
     "ligmg_pc_create": (C.c_int, [P(C.c_void_p)]),
     "ligmg_pc_begin": (C.c_int, [C.c_void_p, i64]),
     "ligmg_pc_add_row": (C.c_int, [C.c_void_p, i64, i64, P(i64), P(f64)]),

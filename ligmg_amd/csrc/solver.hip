@@ -2890,6 +2890,34 @@ int ligmg_assemble_laplacian(int64_t n, const int32_t *rp, const int32_t *ci, co
   }
 }
 
+int ligmg_dense_pinv(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, double *pinv) {
+  if (n <= 0 || !rp || !ci || !v || !pinv) return LIGMG_ERR_INVALID;
+  try {
+    LG_CHECK(n <= lg::COARSE_DENSE_MAX, LIGMG_ERR_UNSUPPORTED,
+             "ligmg_dense_pinv: at most " + std::to_string(lg::COARSE_DENSE_MAX) + " vertices");
+    LG_CHECK(rp[0] == 0, LIGMG_ERR_INVALID, "ligmg_dense_pinv: rp[0] must be 0");
+    for (int64_t i = 0; i < n; i++)
+      LG_CHECK(rp[i + 1] >= rp[i], LIGMG_ERR_INVALID, "ligmg_dense_pinv: rp must be non-decreasing");
+    for (int64_t k = 0; k < rp[n]; k++)
+      LG_CHECK(ci[k] >= 0 && ci[k] < n, LIGMG_ERR_INVALID, "ligmg_dense_pinv: column out of range");
+    const std::vector<int32_t> hrp(rp, rp + n + 1), hci(ci, ci + rp[n]);
+    const std::vector<double> hv(v, v + rp[n]);
+    hipStream_t s = nullptr;
+    LG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{s};
+    lg::DevBuf<double> P;
+    lg::coarse_pinv(n, hrp, hci, hv, P, s);
+    lg::to_host(pinv, P.p, (size_t)n * n, s);
+    LG_HIP(hipStreamSynchronize(s));
+    return LIGMG_OK;
+  } catch (const lg::Error &e) {
+    return fail(e);
+  }
+}
+
 int ligmg_check_units(int64_t n, const int32_t *rp, int32_t *same, int64_t *nunits) {
   if (n <= 0 || !rp || !same || !nunits) return LIGMG_ERR_INVALID;
   try {

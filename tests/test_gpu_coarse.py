@@ -126,3 +126,51 @@ def test_device_pinv_matches_numpy_pinv(shape):
     err = np.linalg.norm(Y - ref) / np.linalg.norm(ref)
     print(f"coarse n={n}: |device - numpy pinv| {err:.2e}", flush=True)
     assert err <= 1e-11
+
+
+def _ring_laplacian(n, seed, comps):
+    """Laplacian of `comps` weighted rings with random chords (CSR, sorted
+    columns, diagonal stored)."""
+    rng = np.random.default_rng(seed)
+    sizes = [n // comps + (1 if c < n % comps else 0) for c in range(comps)]
+    rows, cols = [], []
+    off = 0
+    for m in sizes:
+        i = np.arange(m)
+        rows += list(off + i)
+        cols += list(off + (i + 1) % m)
+        a = rng.integers(0, m, m // 2)
+        b = (a + 1 + rng.integers(1, m - 2, m // 2)) % m
+        rows += list(off + a)
+        cols += list(off + b)
+        off += m
+    w = rng.uniform(0.5, 1.5, len(rows))
+    W = sp.coo_matrix((w, (rows, cols)), shape=(n, n)).tocsr()
+    W = (W + W.T).tocsr()
+    L = (sp.diags(np.asarray(W.sum(axis=1)).ravel()) - W).tocsr()
+    L.sum_duplicates()
+    L.sort_indices()
+    return L
+
+
+@pytest.mark.parametrize("n,comps", [(255, 1), (256, 1), (257, 1), (383, 1), (384, 1), (385, 1), (640, 1),
+                                     (300, 2), (513, 3)])
+def test_dense_pinv_block_boundaries(n, comps):
+    """The coarse pseudo-inverse at sizes around the host/device threshold
+    (256 vertices) and the 128-wide Gauss-Jordan blocks (a full last block,
+    one more vertex, one fewer), connected and with several components,
+    against numpy's SVD pseudo-inverse."""
+    L = _ring_laplacian(n, seed=n, comps=comps)
+    assert csg.connected_components(L, directed=False)[0] == comps
+    rp = L.indptr.astype(np.int32)
+    ci = L.indices.astype(np.int32)
+    v = L.data.astype(np.float64)
+    out = np.empty((n, n))
+    P32, PD = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    ligmg_amd.solver._check(
+        _ffi.solver_lib().ligmg_dense_pinv(n, rp.ctypes.data_as(P32), ci.ctypes.data_as(P32),
+                                           v.ctypes.data_as(PD), out.ctypes.data_as(PD)), "ligmg_dense_pinv")
+    ref = np.linalg.pinv(L.toarray())
+    err = np.abs(out - ref).max() / np.abs(ref).max()
+    print(f"n={n} components={comps}: |device - numpy pinv| {err:.2e}", flush=True)
+    assert err <= 1e-10
