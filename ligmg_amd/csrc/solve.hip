@@ -148,15 +148,18 @@ __global__ void __launch_bounds__(SPMV_BLOCK)
   }
 }
 
+// One wavefront per long row: its chunk partials added left to right from
+// 0.0 (wave_seq_fold keeps a block of partials in flight; one lane per row
+// waited on each load in turn).
 template <class Epi>
-__global__ void k_spmv_long(const int32_t *lrow, const int32_t *lpart, int64_t nlong,
-                            const double *parts, Epi epi) {
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nlong;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0;
-    for (int p = lpart[j]; p < lpart[j + 1]; p++) s += parts[p];
-    epi(lrow[j], s);
-  }
+__global__ void __launch_bounds__(256) k_spmv_long(const int32_t *lrow, const int32_t *lpart, int64_t nlong,
+                                                   const double *parts, Epi epi) {
+  __shared__ double buf[(256 / WAVE) * SEQ_BLOCK];
+  const int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+  if (j >= nlong) return;  // the whole wavefront
+  const double s = wave_seq_fold(lpart[j], lpart[j + 1], buf + (threadIdx.x / WAVE) * SEQ_BLOCK,
+                                 [&](int64_t k) { return parts[k]; });
+  if ((threadIdx.x & (WAVE - 1)) == 0) epi(lrow[j], s);
 }
 
 template <class T, class Val, class Epi>
@@ -176,7 +179,7 @@ void launch_spmv(const DCsr &A, Val val, const T *x, Epi epi, hipStream_t s) {
                        upw, A.U.row_items, parts, epi);
   LG_LAUNCH_CHECK();
   if (A.U.nlong) {
-    hipLaunchKernelGGL((k_spmv_long<Epi>), dim3(grid_for(A.U.nlong, 64)), dim3(64), 0, s, A.U.lrow.p,
+    hipLaunchKernelGGL((k_spmv_long<Epi>), dim3((unsigned)((A.U.nlong + 3) / 4)), dim3(256), 0, s, A.U.lrow.p,
                        A.U.lpart.p, A.U.nlong, (const double *)parts, epi);
     LG_LAUNCH_CHECK();
   }
