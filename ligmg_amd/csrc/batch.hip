@@ -298,8 +298,8 @@ __global__ void k_from_block(const double *X, int64_t n, int k, int kp, const in
 // once.  Its four wavefronts split the K range: lane (li, lk) of wavefront w
 // takes K = (4 w + lk) Q + q, q = 0 .. Q-1 (Q = ceil(n / 16)), one contiguous
 // run of its row of Pm; two accumulator sets alternate on q so consecutive
-// products into one tile do not wait on each other.  The four partial tiles are added in
-// LDS in wavefront order.
+// products into one tile do not wait on each other.  The four partial tiles
+// are added in LDS in wavefront order.
 template <class T, int NT>
 __global__ void __launch_bounds__(256) k_dense_mfma(const double *__restrict__ Pm, int n,
                                                     const T *__restrict__ X, int kp, int Q,
