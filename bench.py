@@ -114,6 +114,34 @@ def _line(metric, value, unit, hib, ms, args, dtype, config, **extra):
     print(json.dumps(out), flush=True)
 
 
+def _demean(X, comp_ptr):
+    """X (n x k, host) with its mean removed on every connected component
+    (comp_ptr: component offsets; None: one component)."""
+    import numpy as np
+    X = X.copy()
+    bounds = [0, X.shape[0]] if comp_ptr is None else list(comp_ptr)
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        X[a:b] -= X[a:b].mean(axis=0)
+    return X
+
+
+def _fp64_proxy_err(L, B, X, solve, comp_ptr=None):
+    """Error of the solutions X (n x k) against an FP64 proxy for the
+    sparse-direct answer: X plus one iterative-refinement step,
+    X_p = X + L^+ (B - L X), the residual formed in FP64 on the host and
+    means removed per component (a direct factorisation of these graphs does
+    not fit).  Returns the largest relative error over the columns and the
+    proxy's largest full relative residual."""
+    import numpy as np
+    Bh, Xh = np.asarray(B, dtype=np.float64), np.asarray(X, dtype=np.float64)
+    R = _demean(Bh - L @ Xh, comp_ptr)
+    Xp = _demean(Xh + solve(R), comp_ptr)
+    Xd = _demean(Xh, comp_ptr)
+    err = max(float(np.linalg.norm(Xd[:, c] - Xp[:, c]) / np.linalg.norm(Xp[:, c])) for c in range(Xh.shape[1]))
+    res = max(float(np.linalg.norm(Bh[:, c] - L @ Xp[:, c]) / np.linalg.norm(Bh[:, c])) for c in range(Xh.shape[1]))
+    return err, res
+
+
 def bench_batched(args):
     """Per-right-hand-side time of batched solves: 48 independent
     Barabasi-Albert graphs (m=5, n uniform in [20k, 400k], seeds 0..47) with
@@ -144,7 +172,7 @@ def bench_batched(args):
             s.solve_batch(B)
     torch.cuda.synchronize()
     per = {name: [] for name, *_ in cases}
-    its = {}
+    its, sols = {}, {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         for name, g, k, s, B, _s in cases:
@@ -152,13 +180,24 @@ def bench_batched(args):
             assert info.status == _ffi.CONVERGED and info.full_rel_residual <= 1e-7
             per[name].append(info.time)
             its[name] = (int(it.min()), int(it.max()))
+            sols[name] = (X, info.full_rel_residual)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     nrhs = sum(k for _, _, k, *_ in cases)
     ms_step = 1e3 * wall / args.steps
     detail = {name: {"n": g.n, "nnz": g.nnz, "rhs": k, "setup_s": st, "batch_solve_s": min(per[name]),
-                     "per_rhs_ms": 1e3 * min(per[name]) / k, "iters_min_max": its[name]}
+                     "per_rhs_ms": 1e3 * min(per[name]) / k, "iters_min_max": its[name],
+                     "full_rel_residual_max": sols[name][1]}
               for name, g, k, s, B, st in cases}
+    # accuracy, outside the timed region: against the FP64 proxy
+    for name, g, k, s, B, st in cases:
+        cp = g.comp_ptr if name == "ba48" else None
+        err, res = _fp64_proxy_err(g.to_scipy(), B.cpu().numpy(), sols[name][0].cpu().numpy(),
+                                   lambda R: s.solve_batch(torch.from_numpy(np.ascontiguousarray(R)).to(dev))[0]
+                                   .cpu().numpy(), cp)
+        detail[name]["err_vs_fp64_proxy"] = {"max_over_columns": err, "proxy_full_rel_residual_max": res}
+        print(f"# {name}: error against the FP64 proxy {err:.2e} (proxy residual {res:.2e})", file=sys.stderr,
+              flush=True)
     # base-level SpMM (the batched solves' dominant kernel shape) and the
     # coarse dense apply on the matrix cores, timed with HIP events inside
     # the library on the solver's stream
@@ -194,6 +233,7 @@ def bench_kcycle(args):
     weights uniform[0.5, 1.5], one mean-zero N(0,1) right-hand side, FP64
     K-cycle (FCG(2) on every non-exact coarse level) inside FP64 FCG to 1e-8.
     A step is one solve; the fused coarse tail is on (its default)."""
+    import numpy as np
     import torch
     import ligmg_amd
     from ligmg_amd import _ffi, graphs
@@ -215,13 +255,20 @@ def bench_kcycle(args):
         times.append(info.time())
     torch.cuda.synchronize()
     ms_step = 1e3 * (time.perf_counter() - t0) / args.steps
+    # accuracy, outside the timed region: against the FP64 proxy
+    err, res = _fp64_proxy_err(g.to_scipy(), b.cpu().numpy()[:, None], info.solution.cpu().numpy()[:, None],
+                               lambda R: s.solve(torch.from_numpy(np.ascontiguousarray(R[:, 0])).to(dev))
+                               .solution.cpu().numpy()[:, None])
     v = ligmg_amd.Solver(A, ligmg_amd.Options())
     vt = min(v.solve(b).time() for _ in range(3))
-    print(f"# kcycle solve {min(times):.4f}s, {info.iters()} iterations; V-cycle/CG on the same graph {vt:.4f}s",
-          file=sys.stderr, flush=True)
+    print(f"# kcycle solve {min(times):.4f}s, {info.iters()} iterations; V-cycle/CG on the same graph {vt:.4f}s; "
+          f"error against the FP64 proxy {err:.2e} (proxy residual {res:.2e})", file=sys.stderr, flush=True)
     _line("time to 1e-8 rel. residual, FP64 K-cycle inside FCG", ms_step / 1e3, "s", False, ms_step, args,
           "f64", {"workload": "rmat22_lcc_uniform0.5-1.5_kcycle_fcg", "n": g.n, "nnz": g.nnz},
-          iters=info.iters(), setup_s=s.setup_time, vcycle_cg_s=vt)
+          iters=info.iters(), setup_s=s.setup_time, vcycle_cg_s=vt, full_rel_residual=info.full_rel_residual,
+          err_vs_fp64_proxy={"error": err, "proxy_full_rel_residual": res,
+                             "proxy": "K-cycle solution + one iterative-refinement step (residual b - L x in "
+                                      "FP64 on the host), mean removed"})
 
 
 def bench_coarse(args):
