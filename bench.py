@@ -359,12 +359,27 @@ def bench_grad(args):
         bw.append(c)
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / args.steps
-    print(f"# rgg n={g.n} edges={len(w)}: forward {min(fw):.3f}s backward {min(bw):.3f}s loss {loss:.6g}",
-          file=sys.stderr, flush=True)
+    # accuracy, outside the timed region: the forward solution against the
+    # FP64 proxy of L(w)^+ b (the gradients are checked against central
+    # differences and a dense pseudo-inverse in the tests)
+    import scipy.sparse as sp
+    with torch.no_grad():
+        x = laplacian_solve(w32, bt, pat)
+        W = sp.coo_matrix((w32.double().cpu().numpy(), (ei, ej)), shape=(g.n, g.n)).tocsr()
+        W = (W + W.T).tocsr()
+        L = (sp.diags(np.asarray(W.sum(axis=1)).ravel()) - W).tocsr()
+        err, res = _fp64_proxy_err(L, (b - b.mean())[:, None], x.cpu().numpy()[:, None],
+                                   lambda R: laplacian_solve(w32, torch.tensor(R[:, 0], device=dev), pat)
+                                   .cpu().numpy()[:, None])
+    print(f"# rgg n={g.n} edges={len(w)}: forward {min(fw):.3f}s backward {min(bw):.3f}s loss {loss:.6g}; "
+          f"forward error against the FP64 proxy {err:.2e} (proxy residual {res:.2e})", file=sys.stderr, flush=True)
     _line("forward+backward time of x = L(w)^+ b (gradients in w and b)", ms_step / 1e3, "s", False, ms_step,
           args, "f64", {"workload": "rgg1M_deg12_gauss_weights_256pairs_10kpair_loss", "n": g.n,
                         "edges": int(len(w))},
-          forward_s=min(fw), backward_s=min(bw))
+          forward_s=min(fw), backward_s=min(bw),
+          err_vs_fp64_proxy={"forward_error": err, "proxy_full_rel_residual": res,
+                             "proxy": "forward solution + one iterative-refinement step (residual b - L x in "
+                                      "FP64 on the host), mean removed"})
 
 
 def main():
