@@ -109,3 +109,24 @@ def test_solve_rejects_wrong_shapes_before_the_library():
               torch.zeros(10, 0)):
         with pytest.raises(ValueError):
             s.solve_batch(B)
+
+
+def test_dense_pinv_rejects_bad_input_before_any_device_work():
+    """ligmg_dense_pinv checks its CSR on the host and fails with a status
+    code and a message, never touching the device for a bad matrix."""
+    lib = _ffi.solver_lib()
+    P32, PD = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    out = np.zeros(16)
+
+    def call(n, rp, ci, v):
+        rp, ci, v = (np.asarray(rp, np.int32), np.asarray(ci, np.int32), np.asarray(v, np.float64))
+        return lib.ligmg_dense_pinv(n, rp.ctypes.data_as(P32), ci.ctypes.data_as(P32), v.ctypes.data_as(PD),
+                                    out.ctypes.data_as(PD))
+
+    assert call(0, [0], [0], [0.0]) == _ffi.ERR_INVALID
+    assert call(9000, [0], [0], [0.0]) == _ffi.ERR_UNSUPPORTED
+    assert b"8192" in lib.ligmg_last_error()
+    assert call(2, [1, 2, 3], [0, 1, 1], [1.0, 1.0, 1.0]) == _ffi.ERR_INVALID       # rp[0] != 0
+    assert call(2, [0, 2, 1], [0, 1, 1], [1.0, -1.0, 1.0]) == _ffi.ERR_INVALID      # rp decreasing
+    assert call(2, [0, 2, 4], [0, 1, 0, 2], [1.0, -1.0, -1.0, 1.0]) == _ffi.ERR_INVALID  # column 2 of 2
+    assert b"column" in lib.ligmg_last_error()
